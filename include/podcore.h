@@ -1,0 +1,168 @@
+/*
+ * podcore.h — C ABI of the MI355X POD core (tall-skinny SVD / QR, POD).
+ *
+ * Every entry point replaces one function of the reference's native or
+ * Python path (pyLOM, ArnauMiro/pyLowOrder); the reference interface is cited
+ * beside each declaration.  Conventions that differ from the reference:
+ *   - pointers are DEVICE pointers (HBM) owned by the caller; the library never
+ *     frees caller memory.  Library workspace lives in the context and is
+ *     counted against its HBM budget;
+ *   - sizes are int64_t and every matrix carries a leading dimension (row
+ *     stride, in elements), so column views such as U[:, :N] pass uncopied;
+ *     all matrices are row-major;
+ *   - inputs are never modified (the reference's ?svd destroys its input,
+ *     pyLOM/vmmath/svd.pyx:138-144);
+ *   - the return value is PC_OK (0) or a negative pc_status; nothing aborts
+ *     (the reference calls MPI_Abort through raiseError,
+ *     pyLOM/utils/errors.py:22-28).  pc_last_error() gives the message;
+ *   - work is ordered on the context's stream (pc_set_stream) and calls are
+ *     asynchronous unless documented otherwise.  One context per host thread;
+ *     concurrent calls on one context race.
+ *
+ * Element types: PC_F32 / PC_F64 real, PC_C64 / PC_C128 complex (interleaved
+ * re, im).  Singular values are always real, stored as float for PC_F32 /
+ * PC_C64 and as double for PC_F64 / PC_C128.
+ *
+ * Precision modes (argument `precision`):
+ *   PC_PREC_FP64  — all arithmetic in FP64 (FP64 MFMA), whatever the storage;
+ *   PC_PREC_MIXED — FP32/complex64 storage only: products on fp16 hi/lo
+ *                   split MFMA with FP32 accumulation, every n x n step
+ *                   (Gram reductions, Cholesky, small SVD) in FP64.
+ */
+#ifndef PODCORE_H
+#define PODCORE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pc_ctx pc_ctx;
+
+typedef enum { PC_F32 = 0, PC_F64 = 1, PC_C64 = 2, PC_C128 = 3 } pc_dtype;
+typedef enum { PC_PREC_FP64 = 0, PC_PREC_MIXED = 1 } pc_precision;
+typedef enum {
+  PC_CENTER_NONE = 0,  /* use the rows as given                                   */
+  PC_CENTER_MEAN = 1,  /* subtract the row (temporal) mean                        */
+  PC_CENTER_MEAN_VAR = 2 /* subtract the mean and divide by the row VARIANCE      */
+} pc_center;
+
+typedef enum {
+  PC_OK = 0,
+  PC_ERR_INVALID = -1,     /* null pointer, bad enum, bad leading dimension     */
+  PC_ERR_SHAPE = -2,       /* unsupported or inconsistent sizes                 */
+  PC_ERR_NOMEM = -3,       /* workspace would exceed the context's HBM budget   */
+  PC_ERR_HIP = -4,         /* a HIP runtime call failed                         */
+  PC_ERR_LINALG = -5,      /* non-finite input or a factorisation broke down    */
+  PC_ERR_UNSUPPORTED = -6  /* dtype / precision combination not provided        */
+} pc_status;
+
+/* ---- context ------------------------------------------------------------
+ * No reference counterpart: the reference keeps MPI_COMM_WORLD and the cupy
+ * device implicit (pyLOM/utils/mpi.py:17-19, pyLOM/utils/gpu.py:18-23).     */
+int pc_ctx_create(int device, int64_t hbm_budget_bytes, uint32_t flags, pc_ctx** out);
+int pc_ctx_destroy(pc_ctx* ctx);
+int pc_set_stream(pc_ctx* ctx, void* hip_stream);
+const char* pc_last_error(const pc_ctx* ctx);
+/* bytes of workspace currently held, and the most a solve of (m, n) needs.   */
+int64_t pc_workspace_bytes(const pc_ctx* ctx);
+int64_t pc_workspace_query(int op, pc_dtype dtype, int64_t m, int64_t n, int64_t r, int precision);
+int pc_synchronize(pc_ctx* ctx);
+const char* pc_version(void);
+
+/* ---- averaging -----------------------------------------------------------
+ * temporal_mean   pyLOM/vmmath/averaging.py:17-29, src/averaging.c:10-46
+ * temporal_variance pyLOM/vmmath/averaging.py:46-59, src/averaging.c:48-90
+ * mean / var are length m, same real type as X (var may be NULL).           */
+int pc_row_stats(pc_ctx* ctx, pc_dtype dtype, const void* X, int64_t ldx, int64_t m, int64_t n,
+                 void* mean, void* var);
+/* subtract_mean  pyLOM/vmmath/averaging.py:31-44, src/averaging.c:92-124
+ * norm_variance  pyLOM/vmmath/averaging.py:61-75, src/averaging.c:126-157
+ * Y = (X - mean 1^T) [./ var 1^T]; var may be NULL.  Y may alias X.         */
+int pc_row_center(pc_ctx* ctx, pc_dtype dtype, const void* X, int64_t ldx, int64_t m, int64_t n,
+                  const void* mean, const void* var, void* Y, int64_t ldy);
+
+/* ---- SVD / QR ------------------------------------------------------------
+ * tsqr_svd  pyLOM/vmmath/svd.py:227-252; src/svd.c:529-563 (s), 678-712 (d),
+ *           827-861 (c), 976-1010 (z); int dtsqr_svd(double*Ui,double*S,
+ *           double*VT,double*Ai,int m,int n) at src/svd.h:25.
+ * With center != PC_CENTER_NONE the row centering of POD.run
+ * (pyLOM/POD/wrapper.py:33-45) is fused into the passes over A.
+ * U is m x n (ldu), S n, VT n x n (ldvt, rows = right singular vectors).   */
+int pc_tsqr_svd(pc_ctx* ctx, pc_dtype dtype, const void* A, int64_t lda, int64_t m, int64_t n,
+                int center, void* U, int64_t ldu, void* S, void* VT, int64_t ldvt, int precision);
+/* tsqr  pyLOM/vmmath/svd.py:49-118; src/svd.c:416-527 (s), 565-676 (d),
+ *       714-825 (c), 863-974 (z).  Also serves qr (svd.py:28-36, svd.c:237-407).
+ * Q m x n (ldq), R n x n upper triangular (ldr).                            */
+int pc_tsqr(pc_ctx* ctx, pc_dtype dtype, const void* A, int64_t lda, int64_t m, int64_t n,
+            void* Q, int64_t ldq, void* R, int64_t ldr, int precision);
+/* svd (dense, thin)  pyLOM/vmmath/svd.py:38-47; src/svd.c:35-235.
+ * A is m x n with min(m, n) <= 512; U m x k, S k, VT k x n, k = min(m, n).  */
+int pc_svd(pc_ctx* ctx, pc_dtype dtype, const void* A, int64_t lda, int64_t m, int64_t n,
+           void* U, int64_t ldu, void* S, void* VT, int64_t ldvt);
+/* randomized_qr   pyLOM/vmmath/svd.py:120-144; src/svd.c:1012-1064, 1267-1319
+ * randomized_svd  pyLOM/vmmath/svd.py:254-273; src/svd.c:1198-1265, 1453-1520
+ * omega (n x r, device, ldo = r) is the caller's sketch matrix (the reference
+ * draws it from numpy's seeded stream, svd.py:131-133; pass it for parity).
+ * If omega is NULL it is drawn on the device from `seed` (uniform [0,1)).  */
+int pc_randomized_qr(pc_ctx* ctx, pc_dtype dtype, const void* A, int64_t lda, int64_t m, int64_t n,
+                     int64_t r, int q, uint64_t seed, const void* omega,
+                     void* Q, int64_t ldq, void* B, int64_t ldb, int precision);
+int pc_randomized_svd(pc_ctx* ctx, pc_dtype dtype, const void* A, int64_t lda, int64_t m, int64_t n,
+                      int64_t r, int q, uint64_t seed, const void* omega,
+                      void* U, int64_t ldu, void* S, void* VT, int64_t ldvt, int precision);
+
+/* ---- products ------------------------------------------------------------
+ * matmul   pyLOM/vmmath/maths.py:76-90; src/vector_matrix.c:158-242
+ * matmulp  pyLOM/vmmath/maths.py:92-110; src/vector_matrix.c:330-384 (on one
+ *          GPU the cross-rank allreduce is empty, the product is matmul).
+ * C = alpha * op(A) op(B) + beta * C, op = identity or (conjugate) transpose
+ * (trans = 0 none, 1 transpose, 2 conjugate transpose).  op(A) is m x k.   */
+int pc_gemm(pc_ctx* ctx, pc_dtype dtype, int transA, int transB, int64_t m, int64_t n, int64_t k,
+            double alpha, const void* A, int64_t lda, const void* B, int64_t ldb, double beta,
+            void* C, int64_t ldc, int precision);
+/* vecmat  pyLOM/vmmath/maths.py:112-129; src/vector_matrix.c:386-445:
+ * C[i,:] = v[i] * A[i,:] (v real for real dtypes, same type otherwise).    */
+int pc_vecmat(pc_ctx* ctx, pc_dtype dtype, const void* v, const void* A, int64_t lda, int64_t m,
+              int64_t n, void* C, int64_t ldc);
+/* transpose  pyLOM/vmmath/maths.py:17-29; src/vector_matrix.c:30-52.       */
+int pc_transpose(pc_ctx* ctx, pc_dtype dtype, const void* A, int64_t lda, int64_t m, int64_t n,
+                 void* B, int64_t ldb);
+
+/* ---- POD -----------------------------------------------------------------
+ * reconstruct  pyLOM/POD/wrapper.py:85-103; POD/wrapper.pyx:284-377:
+ *   X = U diag(S) VT, U m x N, S N (real), VT N x n, X m x n.
+ * project (no reference function; idiom matmulp(transpose(U), Y) at
+ *   pyLOM/DMD/wrapper.py:85): C = U^H X, C N x n.                           */
+int pc_reconstruct(pc_ctx* ctx, pc_dtype dtype, const void* U, int64_t ldu, const void* S,
+                   const void* VT, int64_t ldvt, int64_t m, int64_t N, int64_t n, void* X, int64_t ldx,
+                   int precision);
+int pc_project(pc_ctx* ctx, pc_dtype dtype, const void* U, int64_t ldu, const void* X, int64_t ldx,
+               int64_t m, int64_t N, int64_t n, void* C, int64_t ldc, int precision);
+
+/* ---- statistics / truncation ---------------------------------------------
+ * RMSE    pyLOM/vmmath/stats.py:16-34; src/stats.c:13-40
+ * energy  pyLOM/vmmath/truncation.py:41-60; src/truncation.c:128-187
+ * out[0] = sum |A-B|^2, out[1] = sum |A|^2 (host doubles; synchronous).     */
+int pc_diff_norms(pc_ctx* ctx, pc_dtype dtype, const void* A, int64_t lda, const void* B, int64_t ldb,
+                  int64_t m, int64_t n, double* out);
+/* compute_truncation_residual  pyLOM/vmmath/truncation.py:17-39,
+ * src/truncation.c:14-74 (host function; S host array, r<0 returns i+1).   */
+int pc_truncation_residual(const double* S, int64_t n, double r, int64_t* N);
+
+/* ---- batched (grouped) solves --------------------------------------------
+ * The per-frequency loop of pyLOM/SPOD/wrapper.py:81-86 (tsqr_svd per block)
+ * and any list of independent POD problems, run as one grouped launch per
+ * pass.  Results are identical to pc_tsqr_svd on each problem alone.       */
+typedef struct {
+  const void* A; int64_t lda; int64_t m; int64_t n; int32_t center; int32_t pad_;
+  void* U; int64_t ldu; void* S; void* VT; int64_t ldvt;
+} pc_problem;
+int pc_grouped_tsqr_svd(pc_ctx* ctx, pc_dtype dtype, const pc_problem* problems, int64_t count,
+                        int precision);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PODCORE_H */

@@ -1,0 +1,379 @@
+"""numpy restatement of the reference POD core (test oracle, CPU only).
+
+Test infrastructure: imported only by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py.  The MPI ranks of the reference are simulated:
+a matrix distributed over P ranks is a Python list of P row blocks split
+exactly as pyLOM/utils/parall.py:24-48 (worksplit) splits rows, and the
+butterfly messages of the TSQR tree are exchanged level by level in-process.
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+import scipy.fft
+
+__all__ = [
+    "next_power_of_2", "worksplit", "split_rows", "qr", "svd", "tsqr",
+    "tsqr_svd", "randomized_qr", "randomized_svd", "reference_omega",
+    "temporal_mean", "subtract_mean", "temporal_variance", "norm_variance",
+    "transpose", "matmul", "matmulp", "vecmat", "compute_truncation_residual",
+    "energy", "RMSE", "pod_run", "pod_truncate", "pod_reconstruct",
+    "pod_project", "hammwin", "spod_blocks", "spod_run",
+]
+
+
+# ---------------------------------------------------------------------------
+# rank bookkeeping
+# ---------------------------------------------------------------------------
+def next_power_of_2(n: int) -> int:
+    """pyLOM/vmmath/svd.py:17-25."""
+    p = 1
+    if n and not (n & (n - 1)):
+        return n
+    while p < n:
+        p <<= 1
+    return p
+
+
+def worksplit(istart: int, iend: int, who: int, nworkers: int):
+    """Row range owned by rank ``who`` — pyLOM/utils/parall.py:24-48."""
+    irange = iend - istart
+    if nworkers < irange:
+        per = irange // nworkers
+        lo = istart + who * per
+        hi = lo + per
+        rem = irange - per * nworkers
+        if rem > who:
+            lo += who
+            hi += who + 1
+        else:
+            lo += rem
+            hi += rem
+        return lo, hi
+    lo = who if who < iend else iend
+    hi = who + 1 if who < iend else iend
+    return lo, hi
+
+
+def split_rows(A: np.ndarray, nranks: int):
+    """Distribute the rows of A over ``nranks`` simulated ranks."""
+    m = A.shape[0]
+    out = []
+    for r in range(nranks):
+        lo, hi = worksplit(0, m, r, nranks)
+        out.append(A[lo:hi])
+    return out
+
+
+# ---------------------------------------------------------------------------
+# dense kernels (numpy LAPACK, as the reference's Python path uses)
+# ---------------------------------------------------------------------------
+def qr(A):
+    """Reduced QR — pyLOM/vmmath/svd.py:28-36 (numpy geqrf + orgqr)."""
+    return np.linalg.qr(A)
+
+
+def svd(A):
+    """Thin SVD, V returned as V^H — pyLOM/vmmath/svd.py:38-47."""
+    return np.linalg.svd(A, full_matrices=False)
+
+
+def tsqr(blocks):
+    """Demmel TSQR with the reference's butterfly tree.
+
+    pyLOM/vmmath/svd.py:49-118 (compiled twin: pyLOM/vmmath/src/svd.c:565-676).
+    ``blocks`` is the list of per-rank row blocks A_i (each with m_i >= n).
+    Returns (list of Q_i, R) where R is the rank-0 R after the down-sweep.
+    Senders and receivers of one level are matched, so executing every rank's
+    send before any receive of the same level reproduces the MPI schedule.
+    """
+    P = len(blocks)
+    n = blocks[0].shape[1]
+    dt = np.result_type(*[b.dtype for b in blocks])
+    # svd.py:60 leaf QR on every rank
+    Q1 = []
+    R = []
+    for b in blocks:
+        q, r = qr(b)
+        if r.shape[0] != n:
+            raise ValueError("tsqr needs m_i >= n on every rank (svd.py:69)")
+        Q1.append(q)
+        R.append(r)
+    nxt = next_power_of_2(P)
+    nlev = int(np.log2(nxt))
+    QW = [np.eye(n, dtype=dt) for _ in range(P)]
+    C = [np.zeros((2 * n, n), dt) for _ in range(P)]
+    Q2l = [np.zeros((2 * n * nlev, n), dt) for _ in range(P)]
+    # up-sweep svd.py:67-84
+    blevel = 1
+    for ilev in range(nlev):
+        mail = {}
+        for rk in range(P):
+            C[rk][:n, :] = R[rk]
+            pr = rk ^ blevel
+            if rk & blevel and pr < P:
+                mail[pr] = R[rk].copy()
+        for rk in range(P):
+            pr = rk ^ blevel
+            if not (rk & blevel) and pr < P:
+                C[rk][n:, :] = mail[rk]
+                q2, R[rk] = qr(C[rk])
+                Q2l[rk][2 * n * ilev:2 * n * ilev + 2 * n, :] = q2
+        blevel <<= 1
+    # down-sweep svd.py:87-115
+    if P > 1:
+        blevel = 1 << (nlev - 1)
+        mask = blevel - 1
+    for ilev in reversed(range(nlev)):
+        mail = {}
+        active = [rk for rk in range(P) if (rk & mask) == 0]
+        q2i = {}
+        for rk in active:
+            q2i[rk] = Q2l[rk][2 * n * ilev:2 * n * ilev + 2 * n, :] @ QW[rk]
+        for rk in active:  # senders first
+            pr = rk ^ blevel
+            if not (rk & blevel) and pr < P:
+                msg = np.empty((2 * n, n), dt)
+                msg[:n, :] = R[rk]
+                msg[n:, :] = q2i[rk][n:, :]
+                QW[rk] = q2i[rk][:n, :]
+                mail[pr] = msg
+        for rk in active:
+            pr = rk ^ blevel
+            if rk & blevel and pr < P:
+                msg = mail[rk]
+                R[rk] = msg[:n, :]
+                QW[rk] = msg[n:, :]
+        blevel >>= 1
+        mask >>= 1
+    # svd.py:117
+    Qi = [Q1[rk] @ QW[rk] for rk in range(P)]
+    return Qi, R[0]
+
+
+def tsqr_svd(blocks):
+    """pyLOM/vmmath/svd.py:227-252: tsqr, svd(R), U_i = Q_i U_R."""
+    Qi, R = tsqr(blocks)
+    Ur, S, V = svd(R)
+    return [q @ Ur for q in Qi], S, V
+
+
+def reference_omega(n: int, r: int, seed: int, dtype):
+    """Omega of pyLOM/vmmath/svd.py:131-133 (numpy legacy MT19937 stream)."""
+    seed = int(time.time()) if seed < 0 else seed
+    np.random.seed(seed=seed)
+    return np.random.rand(n, r).astype(dtype)
+
+
+def randomized_qr(blocks, r: int, q: int, seed: int = -1, omega=None):
+    """pyLOM/vmmath/svd.py:120-144 on simulated ranks.
+
+    Every rank seeds identically, so every rank draws the same Omega.
+    ``matmulp`` (maths.py:92-110) is the sum over ranks of the local products.
+    """
+    n = blocks[0].shape[1]
+    dt = blocks[0].dtype
+    if omega is None:
+        omega = reference_omega(n, r, seed, dt)
+    Y = [b @ omega for b in blocks]
+    for _ in range(q):
+        Qi, _ = tsqr(Y)
+        Q2 = matmulp([b.T for b in blocks], Qi)
+        Y = [b @ Q2 for b in blocks]
+    Qi, _ = tsqr(Y)
+    B = matmulp([qq.T for qq in Qi], blocks)
+    return Qi, B
+
+
+def randomized_svd(blocks, r: int, q: int, seed: int = -1, omega=None):
+    """pyLOM/vmmath/svd.py:254-273."""
+    Qi, B = randomized_qr(blocks, r, q, seed=seed, omega=omega)
+    Ur, S, V = svd(B)
+    return [qq @ Ur for qq in Qi], S, V
+
+
+# ---------------------------------------------------------------------------
+# averaging.py, maths.py
+# ---------------------------------------------------------------------------
+def temporal_mean(X):
+    """pyLOM/vmmath/averaging.py:17-29 (row mean over snapshots)."""
+    return np.mean(X, axis=1)
+
+
+def subtract_mean(X, X_mean):
+    """pyLOM/vmmath/averaging.py:31-44."""
+    return X - np.tile(X_mean, (X.shape[1], 1)).T
+
+
+def temporal_variance(X, X_mean):
+    """pyLOM/vmmath/averaging.py:46-59 (population variance, keepdims)."""
+    return np.var(X, axis=1, keepdims=True)
+
+
+def norm_variance(X, X_mean, X_var):
+    """pyLOM/vmmath/averaging.py:61-75 — divides by the VARIANCE."""
+    return subtract_mean(X, X_mean) / X_var
+
+
+def transpose(A):
+    """pyLOM/vmmath/maths.py:17-29."""
+    return np.transpose(A)
+
+
+def matmul(A, B):
+    """pyLOM/vmmath/maths.py:76-90."""
+    return np.matmul(A, B)
+
+
+def matmulp(As, Bs):
+    """pyLOM/vmmath/maths.py:92-110: local products summed over ranks."""
+    out = None
+    for a, b in zip(As, Bs):
+        c = np.matmul(a, b)
+        out = c if out is None else out + c
+    return out
+
+
+def vecmat(v, A):
+    """pyLOM/vmmath/maths.py:112-129: C[i,:] = v[i] * A[i,:]."""
+    C = np.zeros_like(A)
+    for ii in range(v.shape[0]):
+        C[ii, :] = v[ii] * A[ii, :]
+    return C
+
+
+# ---------------------------------------------------------------------------
+# truncation.py, stats.py
+# ---------------------------------------------------------------------------
+def compute_truncation_residual(S, r):
+    """pyLOM/vmmath/truncation.py:17-39 (r<0 returns i+1, as the Python path)."""
+    N = 0
+    if r > 0:
+        normS = np.linalg.norm(S[0:], 2)
+        for ii in range(S.shape[0]):
+            acc = np.linalg.norm(S[ii:], 2) / normS
+            if acc < r:
+                break
+            N += 1
+    else:
+        r = abs(r)
+        normS = np.sum(S[0:])
+        acc = 0
+        for ii in range(S.shape[0]):
+            acc += S[ii] / normS
+            N += 1
+            if acc > r:
+                break
+    return N
+
+
+def energy(original, rec):
+    """pyLOM/vmmath/truncation.py:41-60."""
+    num = np.sum((original - rec) ** 2)
+    den = np.sum(original ** 2)
+    return 1 - num / den
+
+
+def RMSE(A, B, relative=True):
+    """pyLOM/vmmath/stats.py:16-34."""
+    diff = A - B
+    s1 = np.sum(diff * diff)
+    s2 = np.sum(A * A) if relative else np.prod(np.array(A.shape))
+    return np.sqrt(s1 / s2)
+
+
+# ---------------------------------------------------------------------------
+# POD
+# ---------------------------------------------------------------------------
+def pod_run(X, remove_mean=True, divide_variance=False, randomized=False,
+            r=1, q=3, seed=-1, nranks=1, omega=None):
+    """pyLOM/POD/wrapper.py:15-51 over ``nranks`` simulated ranks.
+
+    Returns (U, S, V) with U stacked over the ranks and V = V^H.
+    """
+    if remove_mean:
+        mu = temporal_mean(X)
+        if divide_variance:
+            Y = norm_variance(X, mu, temporal_variance(X, mu))
+        else:
+            Y = subtract_mean(X, mu)
+    else:
+        Y = X.copy()
+    blocks = split_rows(Y, nranks)
+    if randomized:
+        Ui, S, V = randomized_svd(blocks, r, q, seed=seed, omega=omega)
+    else:
+        Ui, S, V = tsqr_svd(blocks)
+    return np.concatenate(Ui, axis=0), S, V
+
+
+def pod_truncate(U, S, V, r=1e-8):
+    """pyLOM/POD/wrapper.py:54-82."""
+    N = int(r) if r >= 1 else compute_truncation_residual(S, r)
+    return U[:, :N], S[:N], V[:N, :]
+
+
+def pod_reconstruct(U, S, V):
+    """pyLOM/POD/wrapper.py:85-103: U @ vecmat(S, V)."""
+    return matmul(U, vecmat(S, V))
+
+
+def pod_project(U, X):
+    """Projection coefficients U^H X (idiom of pyLOM/DMD/wrapper.py:85)."""
+    return np.conj(U).T @ X
+
+
+# ---------------------------------------------------------------------------
+# SPOD
+# ---------------------------------------------------------------------------
+def hammwin(N):
+    """pyLOM/vmmath/fft.py:16-27."""
+    return np.transpose(0.54 - 0.46 * np.cos(2 * np.pi * np.arange(N) / (N - 1)))
+
+
+def spod_blocks(X, t, nDFT=0, nolap=0, remove_mean=True):
+    """Welch blocks of pyLOM/SPOD/wrapper.py:39-76.
+
+    Returns (Q, f, nBlks) with Q of shape (nf, M, nBlks), already divided by
+    sqrt(nBlks) as at wrapper.py:82.  complex64 blocks (quirk: wrapper.py:42
+    never selects complex128 because a dtype object is never np.double).
+    """
+    M, N = X.shape
+    dt = t[1] - t[0]
+    cdtype = np.complex64
+    if nDFT == 0:
+        nDFT = int(np.power(2, np.floor(np.log2(N / 10))))
+    window = hammwin(nDFT)
+    if nolap == 0:
+        nolap = int(np.floor(nDFT / 2))
+    nBlks = int(np.floor((N - nolap) / (nDFT - nolap)))
+    winWeight = 1 / np.mean(window)
+    Y = subtract_mean(X, temporal_mean(X)) if remove_mean else X.copy()
+    f = np.arange(np.ceil(nDFT / 2) + 1) / dt / nDFT
+    nf = f.shape[0]
+    Q = np.zeros((nf, M, nBlks), cdtype)
+    for iblk in range(nBlks):
+        i0 = iblk * (nDFT - nolap)
+        ix = np.arange(nDFT) + i0
+        Xf = Y[:, ix].copy() * window
+        qk = (winWeight / nDFT) * scipy.fft.fft2(Xf, axes=(-1,))[:, :nf]
+        qk[:, 1:-1] *= 2
+        Q[:, :, iblk] = qk.T
+    return Q / np.sqrt(nBlks), f, nBlks
+
+
+def spod_run(X, t, nDFT=0, nolap=0, remove_mean=True):
+    """pyLOM/SPOD/wrapper.py:24-95: returns (L, P, f) sorted by L[:,0] desc."""
+    M = X.shape[0]
+    Q, f, nBlks = spod_blocks(X, t, nDFT, nolap, remove_mean)
+    nf = f.shape[0]
+    L = np.zeros((nf, nBlks), X.dtype)
+    P = np.zeros((M * nBlks, nf), X.dtype)
+    for ifreq in range(nf):
+        U, S, V = tsqr_svd([Q[ifreq]])
+        U = U[0]
+        P[:, ifreq] = np.real(U.reshape((M * nBlks), order="F"))
+        L[ifreq, :] = np.abs(S * S)
+    order = np.argsort(L[:, 0])[::-1]
+    return L[order, :], P[:, order], f[order]

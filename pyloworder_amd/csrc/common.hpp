@@ -1,0 +1,125 @@
+// Shared definitions of the POD core library (HIP, gfx950).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+#include <mutex>
+
+#include "../../include/podcore.h"
+
+#define PC_HD __host__ __device__ __forceinline__
+#define PC_DEV __device__ __forceinline__
+
+// ---------------------------------------------------------------------------
+// complex FP64/FP32 scalars (interleaved re, im — numpy's complex layout)
+// ---------------------------------------------------------------------------
+struct cdouble {
+  double x, y;
+  cdouble() = default;
+  PC_HD cdouble(double a) : x(a), y(0) {}
+  PC_HD cdouble(double a, double b) : x(a), y(b) {}
+};
+struct cfloat {
+  float x, y;
+};
+PC_HD cdouble operator+(cdouble a, cdouble b) { return cdouble(a.x + b.x, a.y + b.y); }
+PC_HD cdouble operator-(cdouble a, cdouble b) { return cdouble(a.x - b.x, a.y - b.y); }
+PC_HD cdouble operator-(cdouble a) { return cdouble(-a.x, -a.y); }
+PC_HD cdouble operator*(cdouble a, cdouble b) { return cdouble(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+PC_HD cdouble operator*(double s, cdouble a) { return cdouble(s * a.x, s * a.y); }
+PC_HD cdouble operator*(cdouble a, double s) { return cdouble(s * a.x, s * a.y); }
+PC_HD cdouble operator/(cdouble a, double s) { return cdouble(a.x / s, a.y / s); }
+PC_HD cdouble& operator+=(cdouble& a, cdouble b) { a.x += b.x; a.y += b.y; return a; }
+PC_HD cdouble& operator-=(cdouble& a, cdouble b) { a.x -= b.x; a.y -= b.y; return a; }
+
+// scalar traits used by the small dense kernels (real double or cdouble)
+PC_HD double conj_(double a) { return a; }
+PC_HD cdouble conj_(cdouble a) { return cdouble(a.x, -a.y); }
+PC_HD double abs2_(double a) { return a * a; }
+PC_HD double abs2_(cdouble a) { return a.x * a.x + a.y * a.y; }
+PC_HD double real_(double a) { return a; }
+PC_HD double real_(cdouble a) { return a.x; }
+PC_HD double cmul_re(double a, double b) { return a * b; }
+// fma-style accumulate conj(a)*b
+PC_HD double dotc_(double a, double b) { return a * b; }
+PC_HD cdouble dotc_(cdouble a, cdouble b) { return cdouble(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x); }
+template <class T> PC_HD T mk_(double re, double im);
+template <> PC_HD double mk_<double>(double re, double) { return re; }
+template <> PC_HD cdouble mk_<cdouble>(double re, double im) { return cdouble(re, im); }
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+struct pc_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int64_t budget = 0;     // HBM budget for the workspace, bytes (0 = unlimited)
+  uint32_t flags = 0;
+  char* ws = nullptr;     // device workspace
+  int64_t ws_bytes = 0;
+  int64_t ws_skip = 0;    // bytes at the front of ws held by the calling entry point
+  int* dflag = nullptr;   // device status words (error flags, ranks)
+  int* hflag = nullptr;   // pinned host mirror
+  std::string err;
+};
+
+// status helpers --------------------------------------------------------------
+int pc_fail(pc_ctx* ctx, int code, const char* fmt, ...);
+#define PC_HIP(ctx, call)                                                        \
+  do {                                                                           \
+    hipError_t e_ = (call);                                                      \
+    if (e_ != hipSuccess)                                                        \
+      return pc_fail(ctx, PC_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_)); \
+  } while (0)
+#define PC_CHECK(ctx, expr)                                                      \
+  do {                                                                           \
+    int rc_ = (expr);                                                            \
+    if (rc_ != PC_OK) return rc_;                                                \
+  } while (0)
+
+// grows ctx->ws to at least `bytes` (sync + realloc when growing)
+int pc_ws_reserve(pc_ctx* ctx, int64_t bytes);
+
+// bump allocator over the workspace (256-byte aligned)
+struct WsCursor {
+  char* base;
+  int64_t off = 0;
+  int64_t cap;
+  template <class T> T* take(int64_t count) {
+    off = (off + 255) & ~int64_t(255);
+    T* p = reinterpret_cast<T*>(base + off);
+    off += count * int64_t(sizeof(T));
+    return p;
+  }
+};
+
+PC_HD int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+PC_HD int64_t round_up(int64_t a, int64_t b) { return ceil_div(a, b) * b; }
+
+// number of row chunks a streaming pass over m rows uses.  It depends only
+// on (m, rows-per-tile) so that every reduction order — and thus every result —
+// is a function of the problem alone, never of what else runs on the GPU.
+inline int stream_chunks(int64_t m, int tile_rows) {
+  int64_t tiles = ceil_div(m, tile_rows);
+  int64_t per = ceil_div(tiles, 1024);  // at most 1024 chunks
+  if (per < 4) per = 4;                 // at least 4 tiles per chunk
+  int64_t chunks = ceil_div(tiles, per);
+  return chunks < 1 ? 1 : int(chunks);
+}
+
+// element sizes
+inline int64_t dtype_bytes(pc_dtype t) {
+  switch (t) {
+    case PC_F32: return 4;
+    case PC_F64: return 8;
+    case PC_C64: return 8;
+    case PC_C128: return 16;
+  }
+  return 0;
+}
+inline bool dtype_complex(pc_dtype t) { return t == PC_C64 || t == PC_C128; }
+inline bool dtype_single(pc_dtype t) { return t == PC_F32 || t == PC_C64; }

@@ -1,0 +1,248 @@
+// Streaming (HBM-bound, non-MFMA) kernels: row statistics, centering,
+// vecmat, transpose, difference norms.
+#include "common.hpp"
+
+namespace {
+
+template <class T> PC_DEV double to_d(T v) { return double(v); }
+
+// One wave per row: temporal mean (pyLOM/vmmath/averaging.py:17-29) and
+// population variance (averaging.py:46-59), both accumulated in FP64.
+template <class T>
+__global__ __launch_bounds__(256) void k_row_stats(const T* __restrict__ X, int64_t ldx, int64_t m,
+                                                   int64_t n, T* __restrict__ mean, T* __restrict__ var) {
+  int lane = threadIdx.x & 63;
+  int64_t row = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
+  if (row >= m) return;
+  const T* x = X + row * ldx;
+  double s = 0.0;
+  for (int64_t j = lane; j < n; j += 64) s += to_d(x[j]);
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+  double mu = s / double(n);
+  if (var) {
+    double v = 0.0;
+    for (int64_t j = lane; j < n; j += 64) { double d = to_d(x[j]) - mu; v += d * d; }
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    if (lane == 0) var[row] = T(v / double(n));
+  }
+  if (lane == 0 && mean) mean[row] = T(mu);
+}
+
+// Y = (X - mean) [/ var]   (averaging.py:31-44 and 61-75)
+template <class T>
+__global__ __launch_bounds__(256) void k_row_center(const T* X, int64_t ldx, int64_t m, int64_t n,
+                                                    const T* __restrict__ mean, const T* __restrict__ var,
+                                                    T* Y, int64_t ldy) {
+  int lane = threadIdx.x & 63;
+  int64_t row = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
+  if (row >= m) return;
+  T mu = mean[row];
+  T vv = var ? var[row] : T(1);
+  const T* x = X + row * ldx;
+  T* y = Y + row * ldy;
+  if (var) {
+    for (int64_t j = lane; j < n; j += 64) y[j] = (x[j] - mu) / vv;
+  } else {
+    for (int64_t j = lane; j < n; j += 64) y[j] = x[j] - mu;
+  }
+}
+
+// C[i,:] = v[i] * A[i,:]  (pyLOM/vmmath/maths.py:112-129); T = element type,
+// V = type of v (real for real data).
+template <class T, class V>
+__global__ __launch_bounds__(256) void k_vecmat(const V* __restrict__ v, const T* A, int64_t lda, int64_t m,
+                                                int64_t n, T* C, int64_t ldc) {
+  int64_t idx = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  int64_t total = m * n;
+  for (; idx < total; idx += int64_t(gridDim.x) * 256) {
+    int64_t i = idx / n, j = idx - i * n;
+    C[i * ldc + j] = A[i * lda + j] * v[i];
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_vecmat_c(const T* __restrict__ v, const T* A, int64_t lda, int64_t m,
+                                                  int64_t n, T* C, int64_t ldc) {
+  int64_t idx = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  int64_t total = m * n;
+  for (; idx < total; idx += int64_t(gridDim.x) * 256) {
+    int64_t i = idx / n, j = idx - i * n;
+    T a = A[i * lda + j], s = v[i];
+    T c;
+    c.x = a.x * s.x - a.y * s.y;
+    c.y = a.x * s.y + a.y * s.x;
+    C[i * ldc + j] = c;
+  }
+}
+
+// B = A^T through a 32x32 LDS tile (maths.py:17-29)
+template <class T>
+__global__ __launch_bounds__(256) void k_transpose(const T* __restrict__ A, int64_t lda, int64_t m, int64_t n,
+                                                   T* __restrict__ B, int64_t ldb) {
+  __shared__ T tile[32][33];
+  int64_t bi = int64_t(blockIdx.y) * 32, bj = int64_t(blockIdx.x) * 32;
+  int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (int r = ty; r < 32; r += 8) {
+    int64_t i = bi + r, j = bj + tx;
+    if (i < m && j < n) tile[r][tx] = A[i * lda + j];
+  }
+  __syncthreads();
+  for (int r = ty; r < 32; r += 8) {
+    int64_t j = bj + r, i = bi + tx;
+    if (i < m && j < n) B[j * ldb + i] = tile[tx][r];
+  }
+}
+
+// partial sums of |A-B|^2 and |A|^2 per block (RMSE / energy)
+template <class T>
+__global__ __launch_bounds__(256) void k_diff_norms(const T* A, int64_t lda, const T* B, int64_t ldb, int64_t m,
+                                                    int64_t n, double* part) {
+  double s1 = 0.0, s2 = 0.0;
+  int64_t total = m * n;
+  for (int64_t idx = int64_t(blockIdx.x) * 256 + threadIdx.x; idx < total; idx += int64_t(gridDim.x) * 256) {
+    int64_t i = idx / n, j = idx - i * n;
+    double a = double(A[i * lda + j]), b = double(B[i * ldb + j]);
+    s1 += (a - b) * (a - b);
+    s2 += a * a;
+  }
+  __shared__ double r1[256], r2[256];
+  r1[threadIdx.x] = s1;
+  r2[threadIdx.x] = s2;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) { r1[threadIdx.x] += r1[threadIdx.x + o]; r2[threadIdx.x] += r2[threadIdx.x + o]; }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) { part[2 * blockIdx.x] = r1[0]; part[2 * blockIdx.x + 1] = r2[0]; }
+}
+
+__global__ void k_sum_pairs(const double* part, int nb, double* out) {
+  // deterministic serial-per-lane reduction of a small number of partials
+  __shared__ double r1[256], r2[256];
+  double s1 = 0, s2 = 0;
+  for (int i = threadIdx.x; i < nb; i += 256) { s1 += part[2 * i]; s2 += part[2 * i + 1]; }
+  r1[threadIdx.x] = s1;
+  r2[threadIdx.x] = s2;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) { r1[threadIdx.x] += r1[threadIdx.x + o]; r2[threadIdx.x] += r2[threadIdx.x + o]; }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) { out[0] = r1[0]; out[1] = r2[0]; }
+}
+
+int check_mat(pc_ctx* ctx, const void* p, int64_t ld, int64_t m, int64_t n, const char* what) {
+  if (m < 0 || n < 0) return pc_fail(ctx, PC_ERR_SHAPE, "%s: negative size", what);
+  if (m > 0 && n > 0 && !p) return pc_fail(ctx, PC_ERR_INVALID, "%s: NULL pointer", what);
+  if (m > 0 && ld < n) return pc_fail(ctx, PC_ERR_INVALID, "%s: leading dimension %lld < %lld", what, (long long)ld, (long long)n);
+  return PC_OK;
+}
+
+}  // namespace
+
+int pc_check_mat(pc_ctx* ctx, const void* p, int64_t ld, int64_t m, int64_t n, const char* what) {
+  return check_mat(ctx, p, ld, m, n, what);
+}
+
+extern "C" int pc_row_stats(pc_ctx* ctx, pc_dtype dt, const void* X, int64_t ldx, int64_t m, int64_t n,
+                            void* mean, void* var) {
+  if (!ctx) return pc_fail(nullptr, PC_ERR_INVALID, "pc_row_stats: ctx is NULL");
+  PC_CHECK(ctx, check_mat(ctx, X, ldx, m, n, "pc_row_stats X"));
+  if (n <= 0) return pc_fail(ctx, PC_ERR_SHAPE, "pc_row_stats: n must be positive");
+  if (m == 0) return PC_OK;
+  dim3 g = dim3(unsigned(ceil_div(m, 4)));
+  if (dt == PC_F64)
+    hipLaunchKernelGGL(k_row_stats<double>, g, dim3(256), 0, ctx->stream, (const double*)X, ldx, m, n, (double*)mean, (double*)var);
+  else if (dt == PC_F32)
+    hipLaunchKernelGGL(k_row_stats<float>, g, dim3(256), 0, ctx->stream, (const float*)X, ldx, m, n, (float*)mean, (float*)var);
+  else
+    return pc_fail(ctx, PC_ERR_UNSUPPORTED, "pc_row_stats: real dtypes only");
+  PC_HIP(ctx, hipGetLastError());
+  return PC_OK;
+}
+
+extern "C" int pc_row_center(pc_ctx* ctx, pc_dtype dt, const void* X, int64_t ldx, int64_t m, int64_t n,
+                             const void* mean, const void* var, void* Y, int64_t ldy) {
+  if (!ctx) return pc_fail(nullptr, PC_ERR_INVALID, "pc_row_center: ctx is NULL");
+  PC_CHECK(ctx, check_mat(ctx, X, ldx, m, n, "pc_row_center X"));
+  PC_CHECK(ctx, check_mat(ctx, Y, ldy, m, n, "pc_row_center Y"));
+  if (m > 0 && !mean) return pc_fail(ctx, PC_ERR_INVALID, "pc_row_center: mean is NULL");
+  if (m == 0 || n == 0) return PC_OK;
+  dim3 g = dim3(unsigned(ceil_div(m, 4)));
+  if (dt == PC_F64)
+    hipLaunchKernelGGL(k_row_center<double>, g, dim3(256), 0, ctx->stream, (const double*)X, ldx, m, n,
+                       (const double*)mean, (const double*)var, (double*)Y, ldy);
+  else if (dt == PC_F32)
+    hipLaunchKernelGGL(k_row_center<float>, g, dim3(256), 0, ctx->stream, (const float*)X, ldx, m, n,
+                       (const float*)mean, (const float*)var, (float*)Y, ldy);
+  else
+    return pc_fail(ctx, PC_ERR_UNSUPPORTED, "pc_row_center: real dtypes only");
+  PC_HIP(ctx, hipGetLastError());
+  return PC_OK;
+}
+
+extern "C" int pc_vecmat(pc_ctx* ctx, pc_dtype dt, const void* v, const void* A, int64_t lda, int64_t m,
+                         int64_t n, void* C, int64_t ldc) {
+  if (!ctx) return pc_fail(nullptr, PC_ERR_INVALID, "pc_vecmat: ctx is NULL");
+  PC_CHECK(ctx, check_mat(ctx, A, lda, m, n, "pc_vecmat A"));
+  PC_CHECK(ctx, check_mat(ctx, C, ldc, m, n, "pc_vecmat C"));
+  if (m == 0 || n == 0) return PC_OK;
+  if (!v) return pc_fail(ctx, PC_ERR_INVALID, "pc_vecmat: v is NULL");
+  int64_t blocks = ceil_div(m * n, 256);
+  if (blocks > 65536) blocks = 65536;
+  dim3 g = dim3(unsigned(blocks));
+  switch (dt) {
+    case PC_F64: hipLaunchKernelGGL((k_vecmat<double, double>), g, dim3(256), 0, ctx->stream, (const double*)v, (const double*)A, lda, m, n, (double*)C, ldc); break;
+    case PC_F32: hipLaunchKernelGGL((k_vecmat<float, float>), g, dim3(256), 0, ctx->stream, (const float*)v, (const float*)A, lda, m, n, (float*)C, ldc); break;
+    case PC_C64: hipLaunchKernelGGL((k_vecmat_c<cfloat>), g, dim3(256), 0, ctx->stream, (const cfloat*)v, (const cfloat*)A, lda, m, n, (cfloat*)C, ldc); break;
+    case PC_C128: hipLaunchKernelGGL((k_vecmat_c<cdouble>), g, dim3(256), 0, ctx->stream, (const cdouble*)v, (const cdouble*)A, lda, m, n, (cdouble*)C, ldc); break;
+  }
+  PC_HIP(ctx, hipGetLastError());
+  return PC_OK;
+}
+
+extern "C" int pc_transpose(pc_ctx* ctx, pc_dtype dt, const void* A, int64_t lda, int64_t m, int64_t n,
+                            void* B, int64_t ldb) {
+  if (!ctx) return pc_fail(nullptr, PC_ERR_INVALID, "pc_transpose: ctx is NULL");
+  PC_CHECK(ctx, check_mat(ctx, A, lda, m, n, "pc_transpose A"));
+  PC_CHECK(ctx, check_mat(ctx, B, ldb, n, m, "pc_transpose B"));
+  if (m == 0 || n == 0) return PC_OK;
+  if (ceil_div(m, 32) > 65535) return pc_fail(ctx, PC_ERR_SHAPE, "pc_transpose: m too large");
+  dim3 g(unsigned(ceil_div(n, 32)), unsigned(ceil_div(m, 32)));
+  switch (dt) {
+    case PC_F64: hipLaunchKernelGGL(k_transpose<double>, g, dim3(256), 0, ctx->stream, (const double*)A, lda, m, n, (double*)B, ldb); break;
+    case PC_F32: hipLaunchKernelGGL(k_transpose<float>, g, dim3(256), 0, ctx->stream, (const float*)A, lda, m, n, (float*)B, ldb); break;
+    case PC_C64: hipLaunchKernelGGL(k_transpose<cfloat>, g, dim3(256), 0, ctx->stream, (const cfloat*)A, lda, m, n, (cfloat*)B, ldb); break;
+    case PC_C128: hipLaunchKernelGGL(k_transpose<cdouble>, g, dim3(256), 0, ctx->stream, (const cdouble*)A, lda, m, n, (cdouble*)B, ldb); break;
+  }
+  PC_HIP(ctx, hipGetLastError());
+  return PC_OK;
+}
+
+extern "C" int pc_diff_norms(pc_ctx* ctx, pc_dtype dt, const void* A, int64_t lda, const void* B, int64_t ldb,
+                             int64_t m, int64_t n, double* out) {
+  if (!ctx || !out) return pc_fail(ctx, PC_ERR_INVALID, "pc_diff_norms: NULL argument");
+  PC_CHECK(ctx, check_mat(ctx, A, lda, m, n, "pc_diff_norms A"));
+  PC_CHECK(ctx, check_mat(ctx, B, ldb, m, n, "pc_diff_norms B"));
+  out[0] = out[1] = 0.0;
+  if (m == 0 || n == 0) return PC_OK;
+  // complex data: treat each (re, im) pair as two reals; |a-b|^2 sums match
+  int64_t mm = m, nn = n, la = lda, lb = ldb;
+  pc_dtype base = dt;
+  if (dt == PC_C64) { base = PC_F32; nn *= 2; la *= 2; lb *= 2; }
+  if (dt == PC_C128) { base = PC_F64; nn *= 2; la *= 2; lb *= 2; }
+  const int nb = 1024;
+  PC_CHECK(ctx, pc_ws_reserve(ctx, (2 * nb + 2) * 8));
+  double* part = reinterpret_cast<double*>(ctx->ws);
+  double* res = part + 2 * nb;
+  if (base == PC_F64)
+    hipLaunchKernelGGL(k_diff_norms<double>, dim3(nb), dim3(256), 0, ctx->stream, (const double*)A, la, (const double*)B, lb, mm, nn, part);
+  else
+    hipLaunchKernelGGL(k_diff_norms<float>, dim3(nb), dim3(256), 0, ctx->stream, (const float*)A, la, (const float*)B, lb, mm, nn, part);
+  hipLaunchKernelGGL(k_sum_pairs, dim3(1), dim3(256), 0, ctx->stream, part, nb, res);
+  PC_HIP(ctx, hipGetLastError());
+  PC_HIP(ctx, hipMemcpyAsync(ctx->hflag, res, 16, hipMemcpyDeviceToHost, ctx->stream));
+  PC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  memcpy(out, ctx->hflag, 16);
+  return PC_OK;
+}

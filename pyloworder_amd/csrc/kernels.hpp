@@ -1,0 +1,32 @@
+// Host-side launchers of the device kernels (one declaration per kernel family).
+#pragma once
+#include "common.hpp"
+
+// FP64 MFMA streaming kernels (k_mfma_f64.hip); return 0, or -1 when n is too large
+int f64_nb(int n);
+int f64_tile_rows();
+int launch_gram_f64(hipStream_t st, pc_dtype dt, const void* X, int64_t ldx, int64_t m, int n, int center,
+                    int stats_in, const double* mean_in, const double* var_in, double* mean_out, double* var_out,
+                    const double* W, int chunks, int64_t tpc, double* part);
+int launch_gemm_f64(hipStream_t st, pc_dtype dt, const void* X, int64_t ldx, int64_t m, int n, int center,
+                    const double* mean_in, const double* var_in, const double* W, int nc, int chunks, int64_t tpc,
+                    void* C, int64_t ldc);
+int launch_xgram_f64(hipStream_t st, pc_dtype dt, const void* A, int64_t lda, int p, const void* B, int64_t ldb, int q,
+                     int64_t m, int chunks, int64_t tpc, double* part);
+
+// mixed (fp16 split) MFMA streaming kernels (k_mfma_mx.hip)
+int mx_nb(int n);
+int mx_gram_rows();
+int mx_gemm_rows();
+int launch_gram_mx(hipStream_t st, const float* X, int64_t ldx, int64_t m, int n, int center, double* mean_out,
+                   double* var_out, int chunks, int64_t tpc, double* part);
+int launch_zgram_mx(hipStream_t st, const float* X, int64_t ldx, int64_t m, int n, int center, const double* mean_in,
+                    const double* var_in, const _Float16* Wf, const float* wscale, int chunks, int64_t tpc,
+                    double* part);
+int launch_gemm_mx(hipStream_t st, const float* X, int64_t ldx, int64_t m, int n, int center, const double* mean_in,
+                   const double* var_in, const _Float16* Wh, const _Float16* Wl, const float* wscale, int nc,
+                   int chunks, int64_t tpc, float* C, int64_t ldc);
+// pack an FP64 n x nc matrix (ld ldw) into fp16 hi (and lo) fragment order with
+// per-column power-of-two scales; also writes the FP64 value of hi*2^-s (Wr)
+int launch_pack_w_mx(hipStream_t st, const double* W, int ldw, int n, int nc, int NB, int NCB, _Float16* Wh,
+                     _Float16* Wl, float* wscale, double* Wr, int ldwr);

@@ -1,0 +1,583 @@
+// Orchestration of the POD core: tall-skinny SVD / QR pipelines, randomized
+// SVD, products and grouped solves.  All device work is queued on the
+// context stream; no host synchronisation happens inside a solve.
+//
+// Tall-skinny SVD of Y (m x n, optionally row-centred), "two-pass
+// Gram-Cholesky" (replaces Demmel TSQR + svd(R) of pyLOM/vmmath/svd.py:49-252):
+//   P1  G1 = Y^H Y                       (MFMA streaming pass, fused centring)
+//       R1 = chol(G1 + tau I), W1 = R1^-1 (FP64; W1 rounded to fp16 when mixed)
+//   P2  G2 = (Y W1)^H (Y W1)             (MFMA streaming pass)
+//       P^T G2 P = R2^H R2 (pivoted, rank r), M = R2 P^T W1^-1 (r x n)
+//       one-sided Jacobi: M J = H,  sigma = column norms, VT = J^H
+//       W = W1 P R2^-1 U_M               (U_M = H sigma^-1)
+//   P3  U = Y W                          (MFMA streaming pass)
+// plus an orthonormal completion of U for numerically null singular values.
+#include "common.hpp"
+#include "kernels.hpp"
+#include "k_small.hip"
+
+int pc_check_mat(pc_ctx* ctx, const void* p, int64_t ld, int64_t m, int64_t n, const char* what);
+
+namespace {
+
+constexpr double kShiftF64 = 1e-8;   // tau / trace(G1), FP64 path
+constexpr double kShiftMx = 1e-6;    // tau / trace(G1), mixed path
+constexpr double kTolF64 = 1e-15;    // pivoted Cholesky rank tolerance (relative to max diag)
+constexpr double kTolMx = 1e-7;
+constexpr double kNullF64 = 1e-13;   // sigma <= tol * sigma_max counts as null
+constexpr double kNullMx = 1e-6;
+
+inline unsigned nblk(int64_t count, int per = 256) { return unsigned(ceil_div(count, per)); }
+
+// output element type for (compute type T, real storage type ST)
+template <class T, class ST> struct VtType;
+template <> struct VtType<double, double> { typedef double type; };
+template <> struct VtType<double, float> { typedef float type; };
+template <> struct VtType<cdouble, double> { typedef cdouble type; };
+template <> struct VtType<cdouble, float> { typedef cfloat type; };
+
+template <class T, class TS> PC_DEV T load_elem(const TS* X, int64_t off, int j);
+template <> PC_DEV double load_elem<double, double>(const double* X, int64_t off, int j) { return X[off + j]; }
+template <> PC_DEV double load_elem<double, float>(const float* X, int64_t off, int j) { return double(X[off + j]); }
+template <> PC_DEV cdouble load_elem<cdouble, cdouble>(const cdouble* X, int64_t off, int j) { return X[off + j]; }
+template <> PC_DEV cdouble load_elem<cdouble, cfloat>(const cfloat* X, int64_t off, int j) {
+  return cdouble(X[off + j].x, X[off + j].y);
+}
+PC_DEV void add_elem(double* p, double v) { *p += v; }
+PC_DEV void add_elem(float* p, double v) { *p = float(double(*p) + v); }
+PC_DEV void add_elem(cdouble* p, cdouble v) { p->x += v.x; p->y += v.y; }
+PC_DEV void add_elem(cfloat* p, cdouble v) { p->x = float(p->x + v.x); p->y = float(p->y + v.y); }
+
+// QR output: R (n x n upper, rows >= rank zero); marks the n - rank null columns of Q
+template <class T, class TO>
+__global__ __launch_bounds__(256) void k_store_r(const T* Rm, int ldi, int n, TO* R, int64_t ldr, const int* rank,
+                                                 int* nnull) {
+  int64_t idx = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  int rk = rank[0];
+  if (idx == 0) nnull[0] = n - rk;
+  if (idx >= int64_t(n) * n) return;
+  int i = int(idx / n), j = int(idx % n);
+  T v = (i <= j && i < rk) ? Rm[int64_t(i) * ldi + j] : T(0.0);
+  store_vt(R + int64_t(i) * ldr + j, v);
+}
+
+
+template <class T> struct SmallBufs {
+  T *G1, *R1, *W1, *R1h, *G2, *Awk, *R2, *Hc, *J, *UM, *R2i, *tmp, *Wf;
+  double* sig;
+  int *perm, *order, *rank, *nnull, *status;
+  double* Wpad;  // NB x NB real (embedded) operand for the FP64 streaming kernels
+  // null completion
+  double* lev;
+  T* urows;
+  int* jsel;
+  T* cnull;
+  // mixed operands
+  _Float16 *Wh, *Wl;
+  float* wscale;
+  double* Wr;
+};
+
+struct Plan {
+  pc_dtype dt;
+  bool cplx;
+  bool mixed;
+  int n;       // logical columns
+  int nr;      // real columns of the streaming view (2n for complex)
+  int NB;      // padded real width of the streaming kernels
+  int64_t m;
+  int tr;      // rows per tile of the P1 kernel
+  int chunks;
+  int64_t tpc;
+  int tr3, chunks3;  // P2/P3 kernels (mixed uses other tiles)
+  int64_t tpc3;
+  int nparts;  // partial slabs per P1 launch
+};
+
+int make_plan(pc_ctx* ctx, pc_dtype dt, int64_t m, int64_t n, int precision, Plan& p) {
+  p.dt = dt;
+  p.cplx = dtype_complex(dt);
+  p.mixed = precision == PC_PREC_MIXED;
+  if (p.mixed && !dtype_single(dt))
+    return pc_fail(ctx, PC_ERR_UNSUPPORTED, "mixed precision needs float32 or complex64 storage");
+  if (precision != PC_PREC_MIXED && precision != PC_PREC_FP64)
+    return pc_fail(ctx, PC_ERR_INVALID, "unknown precision %d", precision);
+  p.n = int(n);
+  p.nr = p.cplx ? 2 * int(n) : int(n);
+  p.m = m;
+  if (p.mixed) {
+    p.NB = mx_nb(p.nr);
+    p.tr = mx_gram_rows();
+    p.tr3 = mx_gemm_rows();
+  } else {
+    p.NB = f64_nb(p.nr);
+    p.tr = f64_tile_rows();
+    p.tr3 = f64_tile_rows();
+  }
+  if (p.NB < 0)
+    return pc_fail(ctx, PC_ERR_SHAPE, "n = %lld is larger than this build supports (%s)", (long long)n,
+                   p.cplx ? "complex: 128" : "256");
+  p.chunks = stream_chunks(m, p.tr);
+  p.tpc = ceil_div(ceil_div(m, p.tr), p.chunks);
+  p.chunks3 = stream_chunks(m, p.tr3);
+  p.tpc3 = ceil_div(ceil_div(m, p.tr3), p.chunks3);
+  p.nparts = p.mixed ? 2 * p.chunks : p.chunks;
+  return PC_OK;
+}
+
+template <class T>
+int64_t carve(WsCursor& c, const Plan& p, SmallBufs<T>& b, double** mean, double** var, double** part,
+              int64_t part_slabs) {
+  int n = p.n;
+  int64_t nn = int64_t(n) * n;
+  *mean = c.take<double>(p.m);
+  *var = c.take<double>(p.m);
+  *part = c.take<double>(part_slabs * int64_t(p.NB) * p.NB);
+  b.G1 = c.take<T>(nn); b.R1 = c.take<T>(nn); b.W1 = c.take<T>(nn); b.R1h = c.take<T>(nn);
+  b.G2 = c.take<T>(nn); b.Awk = c.take<T>(nn); b.R2 = c.take<T>(nn); b.Hc = c.take<T>(nn);
+  b.J = c.take<T>(nn); b.UM = c.take<T>(nn); b.R2i = c.take<T>(nn); b.tmp = c.take<T>(nn); b.Wf = c.take<T>(nn);
+  b.sig = c.take<double>(n);
+  b.perm = c.take<int>(n); b.order = c.take<int>(n); b.rank = c.take<int>(4); b.nnull = c.take<int>(4);
+  b.status = c.take<int>(4);
+  b.Wpad = c.take<double>(int64_t(p.NB) * p.NB);
+  int ncand = int(std::min<int64_t>(p.m, 4 * int64_t(n) + 16));
+  b.lev = c.take<double>(ncand);
+  b.urows = c.take<T>(int64_t(ncand) * n);
+  b.jsel = c.take<int>(n);
+  b.cnull = c.take<T>(nn);
+  b.Wh = c.take<_Float16>(int64_t(p.NB) * p.NB);
+  b.Wl = c.take<_Float16>(int64_t(p.NB) * p.NB);
+  b.wscale = c.take<float>(p.NB);
+  b.Wr = c.take<double>(int64_t(p.NB) * p.NB);
+  return c.off;
+}
+
+// ---------------------------------------------------------------------------
+// null completion kernels
+// ---------------------------------------------------------------------------
+// leverage of candidate rows: u_c = y_c W[:, :g] (g = n - nnull), lev_c = ||u_c||^2
+template <class T, class TS>
+__global__ __launch_bounds__(256) void k_null_lev(const TS* X, int64_t ldx, int64_t m, int n, int center,
+                                                  const double* mean, const double* var, const T* W, int ldw,
+                                                  const int* nnull, int ncand, double* lev, T* urows) {
+  int d = nnull[0];
+  if (d == 0) return;
+  int c = blockIdx.x;
+  int64_t row = ncand >= m ? c : (int64_t(c) * m) / ncand;
+  __shared__ T yrow[512];
+  for (int j = threadIdx.x; j < n; j += 256) {
+    T v = load_elem<T, TS>(X, row * ldx, j);
+    if (center) {
+      v = v - T(mean[row]);
+      if (center == 2) v = v * (1.0 / var[row]);
+    }
+    yrow[j] = v;
+  }
+  __syncthreads();
+  int g = n - d;
+  double s = 0;
+  for (int k = threadIdx.x; k < n; k += 256) {
+    T u = T(0.0);
+    if (k < g)
+      for (int j = 0; j < n; j++) u += yrow[j] * W[int64_t(j) * ldw + k];
+    urows[int64_t(c) * n + k] = u;
+    s += abs2_(u);
+  }
+  __shared__ double sh[256];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) lev[c] = sh[0];
+}
+
+
+template <class T>
+void launch_null_lev(hipStream_t st, const Plan& p, const void* A, int64_t lda, int center, const double* mean,
+                     const double* var, SmallBufs<T>& b, int ncand);
+template <>
+void launch_null_lev<double>(hipStream_t st, const Plan& p, const void* A, int64_t lda, int center, const double* mean,
+                             const double* var, SmallBufs<double>& b, int ncand) {
+  if (p.dt == PC_F64)
+    hipLaunchKernelGGL((k_null_lev<double, double>), dim3(ncand), dim3(256), 0, st, (const double*)A, lda, p.m, p.n,
+                       center, mean, var, b.Wf, p.n, b.nnull, ncand, b.lev, b.urows);
+  else
+    hipLaunchKernelGGL((k_null_lev<double, float>), dim3(ncand), dim3(256), 0, st, (const float*)A, lda, p.m, p.n,
+                       center, mean, var, b.Wf, p.n, b.nnull, ncand, b.lev, b.urows);
+}
+template <>
+void launch_null_lev<cdouble>(hipStream_t st, const Plan& p, const void* A, int64_t lda, int center, const double* mean,
+                              const double* var, SmallBufs<cdouble>& b, int ncand) {
+  if (p.dt == PC_C128)
+    hipLaunchKernelGGL((k_null_lev<cdouble, cdouble>), dim3(ncand), dim3(256), 0, st, (const cdouble*)A, lda, p.m,
+                       p.n, center, mean, var, b.Wf, p.n, b.nnull, ncand, b.lev, b.urows);
+  else
+    hipLaunchKernelGGL((k_null_lev<cdouble, cfloat>), dim3(ncand), dim3(256), 0, st, (const cfloat*)A, lda, p.m, p.n,
+                       center, mean, var, b.Wf, p.n, b.nnull, ncand, b.lev, b.urows);
+}
+
+// select the d lowest-leverage candidates, build the completion:
+//   Md = I - U_J U_J^H = R^H R,  C = R^-1,  W[:, g:] = -W[:, :g] U_J^H C,
+//   jsel[a] = row index j_a, cnull[a][b] = C[a][b]  (U[j_a][g+b] += C[a][b])
+template <class T>
+__global__ __launch_bounds__(1024) void k_null_build(int64_t m, int n, const int* nnull, int ncand, const double* lev,
+                                                     const T* urows, T* W, int ldw, int* jsel, T* cnull, T* work,
+                                                     int* status) {
+  int d = nnull[0];
+  if (d == 0) return;
+  int g = n - d;
+  __shared__ int pick[512];
+  __shared__ double sh[32];
+  // rank candidates by leverage (ties by index), keep the first d
+  for (int c = threadIdx.x; c < ncand; c += 1024) {
+    double v = lev[c];
+    int r = 0;
+    for (int e = 0; e < ncand; e++) r += (lev[e] < v) || (lev[e] == v && e < c);
+    if (r < d) pick[r] = c;
+  }
+  __syncthreads();
+  for (int a = threadIdx.x; a < d; a += 1024) {
+    int c = pick[a];
+    jsel[a] = int(ncand >= m ? c : (int64_t(c) * m) / ncand);
+  }
+  // Md (d x d) in work, row-major
+  T* Md = work;
+  T* Rd = work + int64_t(d) * d;
+  for (int64_t idx = threadIdx.x; idx < int64_t(d) * d; idx += 1024) {
+    int a = int(idx / d), b = int(idx % d);
+    const T* ua = urows + int64_t(pick[a]) * n;
+    const T* ub = urows + int64_t(pick[b]) * n;
+    T s = T(0.0);
+    for (int k = 0; k < g; k++) s += ua[k] * conj_(ub[k]);
+    Md[idx] = T(a == b ? 1.0 : 0.0) - s;
+  }
+  __syncthreads();
+  // Cholesky Md = Rd^H Rd (upper), in place into Rd
+  for (int64_t idx = threadIdx.x; idx < int64_t(d) * d; idx += 1024) {
+    int a = int(idx / d), b = int(idx % d);
+    Rd[idx] = a <= b ? Md[idx] : T(0.0);
+  }
+  __syncthreads();
+  __shared__ double piv;
+  for (int k = 0; k < d; k++) {
+    if (threadIdx.x == 0) {
+      double v = real_(Rd[int64_t(k) * d + k]);
+      if (!(v > 1e-12)) { status[0] |= 4; v = 1e-12; }
+      piv = sqrt(v);
+    }
+    __syncthreads();
+    for (int j = k + 1 + threadIdx.x; j < d; j += 1024) Rd[int64_t(k) * d + j] = Rd[int64_t(k) * d + j] * (1.0 / piv);
+    if (threadIdx.x == 0) Rd[int64_t(k) * d + k] = T(piv);
+    __syncthreads();
+    int w = d - k - 1;
+    for (int64_t idx = threadIdx.x; idx < int64_t(w) * w; idx += 1024) {
+      int i = k + 1 + int(idx / w), j = k + 1 + int(idx % w);
+      if (i <= j) Rd[int64_t(i) * d + j] -= conj_(Rd[int64_t(k) * d + i]) * Rd[int64_t(k) * d + j];
+    }
+    __syncthreads();
+  }
+  // C = Rd^-1 (upper) by column back substitution into Md
+  for (int b = threadIdx.x; b < d; b += 1024) {
+    for (int a = d - 1; a >= 0; a--) {
+      T s = T(a == b ? 1.0 : 0.0);
+      for (int k = a + 1; k <= b; k++) s -= Rd[int64_t(a) * d + k] * Md[int64_t(k) * d + b];
+      T raa = Rd[int64_t(a) * d + a];
+      Md[int64_t(a) * d + b] = a > b ? T(0.0) : s * conj_(raa) * (1.0 / abs2_(raa));
+    }
+  }
+  __syncthreads();
+  for (int64_t idx = threadIdx.x; idx < int64_t(d) * d; idx += 1024) cnull[idx] = Md[idx];
+  // T2 = U_J^H C  (g x d), then W[:, g+b] = -W[:, :g] T2[:, b]
+  T* T2 = Rd;  // reuse (g x d fits: d*d may be < g*d, so use tail of work)
+  T2 = work + 2 * int64_t(d) * d;
+  for (int64_t idx = threadIdx.x; idx < int64_t(g) * d; idx += 1024) {
+    int k = int(idx / d), b = int(idx % d);
+    T s = T(0.0);
+    for (int a = 0; a < d; a++) s += conj_(urows[int64_t(pick[a]) * n + k]) * Md[int64_t(a) * d + b];
+    T2[idx] = s;
+  }
+  __syncthreads();
+  for (int64_t idx = threadIdx.x; idx < int64_t(n) * d; idx += 1024) {
+    int j = int(idx / d), b = int(idx % d);
+    T s = T(0.0);
+    for (int k = 0; k < g; k++) s += W[int64_t(j) * ldw + k] * T2[int64_t(k) * d + b];
+    W[int64_t(j) * ldw + g + b] = -s;
+  }
+}
+
+template <class T, class TO>
+__global__ __launch_bounds__(256) void k_null_fix(const int* nnull, int n, const int* jsel, const T* cnull, TO* U,
+                                                  int64_t ldu) {
+  int d = nnull[0];
+  int idx = blockIdx.x * 256 + threadIdx.x;
+  if (idx >= d * d) return;
+  int a = idx / d, b = idx % d;
+  int g = n - d;
+  TO* p = U + int64_t(jsel[a]) * ldu + g + b;
+  add_elem(p, cnull[idx]);
+}
+
+// set unit diagonal on the rows >= rank (keeps R2 invertible)
+template <class T>
+__global__ void k_pad_identity(T* R, int ldr, int n, const int* rank) {
+  int r = rank[0];
+  for (int i = r + threadIdx.x; i < n; i += blockDim.x) R[int64_t(i) * ldr + i] = T(1.0);
+}
+
+// real n x nc FP64 -> padded NB x NCB operand
+__global__ __launch_bounds__(256) void k_pad_real(const double* W, int ldw, int n, int nc, double* P, int NB, int NCB) {
+  int64_t idx = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (idx >= int64_t(NB) * NCB) return;
+  int i = int(idx / NCB), j = int(idx % NCB);
+  P[idx] = (i < n && j < nc) ? W[int64_t(i) * ldw + j] : 0.0;
+}
+
+__global__ __launch_bounds__(256) void k_embed_pad(const cdouble* W, int ldw, int n, int nc, double* P, int NB, int NCB) {
+  int64_t idx = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (idx >= int64_t(NB) * NCB) return;
+  int i = int(idx / NCB), j = int(idx % NCB);
+  double v = 0.0;
+  if (i < 2 * n && j < 2 * nc) {
+    cdouble w = W[int64_t(i >> 1) * ldw + (j >> 1)];
+    int pi = i & 1, pj = j & 1;
+    v = (pi == 0) ? (pj == 0 ? w.x : w.y) : (pj == 0 ? -w.y : w.x);
+  }
+  P[idx] = v;
+}
+
+// mixed: Wr (real FP64 NB x NB, values of the rounded operand) -> T n x n
+__global__ __launch_bounds__(256) void k_unpad_real(const double* P, int NB, int n, double* W) {
+  int64_t idx = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (idx >= int64_t(n) * n) return;
+  int i = int(idx / n), j = int(idx % n);
+  W[idx] = P[int64_t(i) * NB + j];
+}
+__global__ __launch_bounds__(256) void k_unembed(const double* P, int NB, int n, cdouble* W) {
+  int64_t idx = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (idx >= int64_t(n) * n) return;
+  int a = int(idx / n), b = int(idx % n);
+  W[idx] = cdouble(P[int64_t(2 * a) * NB + 2 * b], P[int64_t(2 * a) * NB + 2 * b + 1]);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// core pipeline
+// ---------------------------------------------------------------------------
+enum CoreMode { CORE_SVD = 0, CORE_QR = 1 };
+
+struct CoreOut {
+  void* U; int64_t ldu;    // SVD: U ; QR: Q
+  void* S;                 // SVD
+  void* VT; int64_t ldvt;  // SVD: VT ; QR: R (ld = ldvt)
+};
+
+template <class T>
+static int launch_streams(pc_ctx* ctx, const Plan& p, const void* A, int64_t lda, int center, double* mean,
+                          double* var, double* part, SmallBufs<T>& b, int pass) {
+  hipStream_t st = ctx->stream;
+  int rc = 0;
+  pc_dtype sdt = p.cplx ? (p.dt == PC_C64 ? PC_F32 : PC_F64) : p.dt;
+  int64_t ldr = p.cplx ? 2 * lda : lda;
+  if (p.mixed) {
+    if (pass == 1) rc = launch_gram_mx(st, (const float*)A, ldr, p.m, p.nr, center, mean, var, p.chunks, p.tpc, part);
+    else rc = launch_zgram_mx(st, (const float*)A, ldr, p.m, p.nr, center, mean, var, b.Wh, b.wscale, p.chunks3,
+                              p.tpc3, part);
+  } else {
+    if (pass == 1)
+      rc = launch_gram_f64(st, sdt, A, ldr, p.m, p.nr, center, 0, nullptr, nullptr, center ? mean : nullptr,
+                           center == 2 ? var : nullptr, nullptr, p.chunks, p.tpc, part);
+    else
+      rc = launch_gram_f64(st, sdt, A, ldr, p.m, p.nr, center, 1, mean, var, nullptr, nullptr, b.Wpad, p.chunks,
+                           p.tpc, part);
+  }
+  if (rc != 0) return pc_fail(ctx, PC_ERR_SHAPE, "streaming kernel has no variant for n = %d", p.n);
+  PC_HIP(ctx, hipGetLastError());
+  return PC_OK;
+}
+
+template <class T>
+static void reduce_gram(hipStream_t st, const Plan& p, const double* part, int nparts, T* G);
+template <>
+void reduce_gram<double>(hipStream_t st, const Plan& p, const double* part, int nparts, double* G) {
+  hipLaunchKernelGGL(k_reduce_gram_real, dim3(nblk(int64_t(p.n) * p.n)), dim3(256), 0, st, part, nparts,
+                     int64_t(p.NB) * p.NB, p.NB, p.n, G, p.n);
+}
+template <>
+void reduce_gram<cdouble>(hipStream_t st, const Plan& p, const double* part, int nparts, cdouble* G) {
+  hipLaunchKernelGGL(k_reduce_gram_cplx, dim3(nblk(int64_t(p.n) * p.n)), dim3(256), 0, st, part, nparts,
+                     int64_t(p.NB) * p.NB, p.NB, p.n, G, p.n);
+}
+
+template <class T>
+static void to_operand(hipStream_t st, const Plan& p, const T* W, int nc, SmallBufs<T>& b, bool split);
+// FP64 path: padded / embedded FP64 operand; mixed path: fp16 hi (and lo) + scales,
+// with Wr the FP64 value of the rounded hi part (used as the exact preconditioner)
+template <>
+void to_operand<double>(hipStream_t st, const Plan& p, const double* W, int nc, SmallBufs<double>& b, bool split) {
+  if (!p.mixed) {
+    hipLaunchKernelGGL(k_pad_real, dim3(nblk(int64_t(p.NB) * p.NB)), dim3(256), 0, st, W, p.n, p.n, nc, b.Wpad, p.NB,
+                       p.NB);
+  } else {
+    hipLaunchKernelGGL(k_pad_real, dim3(nblk(int64_t(p.NB) * p.NB)), dim3(256), 0, st, W, p.n, p.n, nc, b.Wpad, p.NB,
+                       p.NB);
+    launch_pack_w_mx(st, b.Wpad, p.NB, p.NB, p.NB, p.NB, p.NB, b.Wh, split ? b.Wl : nullptr, b.wscale, b.Wr, p.NB);
+  }
+}
+template <>
+void to_operand<cdouble>(hipStream_t st, const Plan& p, const cdouble* W, int nc, SmallBufs<cdouble>& b, bool split) {
+  hipLaunchKernelGGL(k_embed_pad, dim3(nblk(int64_t(p.NB) * p.NB)), dim3(256), 0, st, W, p.n, p.n, nc, b.Wpad, p.NB,
+                     p.NB);
+  if (p.mixed)
+    launch_pack_w_mx(st, b.Wpad, p.NB, p.NB, p.NB, p.NB, p.NB, b.Wh, split ? b.Wl : nullptr, b.wscale, b.Wr, p.NB);
+}
+
+template <class T> static void rounded_back(hipStream_t st, const Plan& p, SmallBufs<T>& b, T* out);
+template <> void rounded_back<double>(hipStream_t st, const Plan& p, SmallBufs<double>& b, double* out) {
+  hipLaunchKernelGGL(k_unpad_real, dim3(nblk(int64_t(p.n) * p.n)), dim3(256), 0, st, b.Wr, p.NB, p.n, out);
+}
+template <> void rounded_back<cdouble>(hipStream_t st, const Plan& p, SmallBufs<cdouble>& b, cdouble* out) {
+  hipLaunchKernelGGL(k_unembed, dim3(nblk(int64_t(p.n) * p.n)), dim3(256), 0, st, b.Wr, p.NB, p.n, out);
+}
+
+template <class T, class ST>
+static int core_t(pc_ctx* ctx, const Plan& p, const void* A, int64_t lda, int center, int mode, CoreOut out) {
+  hipStream_t st = ctx->stream;
+  int n = p.n;
+  int64_t nn = int64_t(n) * n;
+  // workspace
+  SmallBufs<T> b;
+  double *mean, *var, *part;
+  int64_t slabs = std::max(p.nparts, p.mixed ? p.chunks3 : p.chunks);
+  int64_t base = ctx->ws_skip;
+  {
+    WsCursor probe{nullptr, 0, 0};
+    int64_t need = carve<T>(probe, p, b, &mean, &var, &part, slabs) + (int64_t(3) * nn + 64) * sizeof(T);
+    PC_CHECK(ctx, pc_ws_reserve(ctx, base + need + 4096));
+  }
+  WsCursor cur{ctx->ws + base, 0, ctx->ws_bytes - base};
+  carve<T>(cur, p, b, &mean, &var, &part, slabs);
+  T* nwork = cur.take<T>(3 * nn + 64);
+  PC_HIP(ctx, hipMemsetAsync(b.status, 0, 16, st));
+
+  // ---- P1
+  PC_CHECK(ctx, launch_streams<T>(ctx, p, A, lda, center, mean, var, part, b, 1));
+  reduce_gram<T>(st, p, part, p.nparts, b.G1);
+  hipLaunchKernelGGL((k_chol_shift<T>), dim3(1), dim3(1024), 0, st, b.G1, n, n, p.mixed ? kShiftMx : kShiftF64, b.R1,
+                     n, b.status);
+  hipLaunchKernelGGL((k_trinv_upper<T>), dim3(1), dim3(1024), 0, st, b.R1, n, n, b.W1, n);
+  to_operand<T>(st, p, b.W1, n, b, false);
+  if (p.mixed) rounded_back<T>(st, p, b, b.W1);  // W1 := the operand actually applied
+  hipLaunchKernelGGL((k_trinv_upper<T>), dim3(1), dim3(1024), 0, st, b.W1, n, n, b.R1h, n);
+  // ---- P2
+  PC_CHECK(ctx, launch_streams<T>(ctx, p, A, lda, center, mean, var, part, b, 2));
+  reduce_gram<T>(st, p, part, p.mixed ? p.chunks3 : p.chunks, b.G2);
+  hipLaunchKernelGGL((k_pchol<T>), dim3(1), dim3(1024), 0, st, b.G2, n, n, p.mixed ? kTolMx : kTolF64, b.Awk, b.R2, n,
+                     b.perm, b.rank, b.status);
+  // M = R2 (rows < rank) * R1h[perm, :], stored column-major into Hc
+  dim3 gg(unsigned(ceil_div(n, 32)), unsigned(ceil_div(n, 32)));
+  hipLaunchKernelGGL((k_gemm_small<T>), gg, dim3(256), 0, st, 0, 0, n, n, n, T(1.0), b.R2, n, (const int*)nullptr,
+                     b.R1h, n, (const int*)b.perm, T(0.0), b.Hc, n, 1);
+  hipLaunchKernelGGL((k_pad_identity<T>), dim3(1), dim3(256), 0, st, b.R2, n, n, b.rank);
+  hipLaunchKernelGGL((k_trinv_upper<T>), dim3(1), dim3(1024), 0, st, b.R2, n, n, b.R2i, n);
+  int nc = n;
+  if (mode == CORE_SVD) {
+    hipLaunchKernelGGL((k_jacobi<T>), dim3(1), dim3(1024), 0, st, b.Hc, n, n, n, b.J, n, b.sig, b.status, 60,
+                       1e-15, 1);
+    hipLaunchKernelGGL(k_sort_desc, dim3(1), dim3(1024), 0, st, b.sig, n, b.order);
+    hipLaunchKernelGGL((k_svd_finish<T, ST, typename VtType<T, ST>::type>), dim3(nblk(nn)), dim3(256), 0, st, b.Hc,
+                       n, n, n, b.sig, b.order, b.J, n, p.mixed ? kNullMx : kNullF64, (const int*)b.rank,
+                       (ST*)out.S, (typename VtType<T, ST>::type*)out.VT, out.ldvt, b.UM, n, b.nnull);
+  } else {
+    // QR of M (n x n, rows >= rank zero): M = Q_M R_M;  R output, UM := Q_M
+    hipLaunchKernelGGL((k_copy2d<T>), dim3(nblk(nn)), dim3(256), 0, st, b.Hc, n, n, n, b.tmp, n, 1);
+    hipLaunchKernelGGL((k_house_qr<T>), dim3(1), dim3(1024), 0, st, b.tmp, n, n, n, b.UM, n, b.J, n, nwork);
+    hipLaunchKernelGGL((k_store_r<T, typename VtType<T, ST>::type>), dim3(nblk(nn)), dim3(256), 0, st, b.J, n, n,
+                       (typename VtType<T, ST>::type*)out.VT, out.ldvt, b.rank, b.nnull);
+  }
+  // W = W1[:, perm] * (R2^-1 * UM)
+  hipLaunchKernelGGL((k_gemm_small<T>), gg, dim3(256), 0, st, 0, 0, n, n, n, T(1.0), b.R2i, n, (const int*)nullptr,
+                     b.UM, n, (const int*)nullptr, T(0.0), b.tmp, n, 0);
+  hipLaunchKernelGGL((k_gemm_small<T>), gg, dim3(256), 0, st, 0, 0, n, n, n, T(1.0), b.W1, n, (const int*)b.perm,
+                     b.tmp, n, (const int*)nullptr, T(0.0), b.Wf, n, 0);
+  // null completion (kernels return at once when nnull == 0)
+  int ncand = int(std::min<int64_t>(p.m, 4 * int64_t(n) + 16));
+  launch_null_lev<T>(st, p, A, lda, center, mean, var, b, ncand);
+  hipLaunchKernelGGL((k_null_build<T>), dim3(1), dim3(1024), 0, st, p.m, n, b.nnull, ncand, b.lev, b.urows, b.Wf, n,
+                     b.jsel, b.cnull, nwork, b.status);
+  // ---- P3
+  to_operand<T>(st, p, b.Wf, nc, b, true);
+  {
+    int rc;
+    pc_dtype sdt = p.cplx ? (p.dt == PC_C64 ? PC_F32 : PC_F64) : p.dt;
+    int64_t ldr = p.cplx ? 2 * lda : lda;
+    int64_t ldu = p.cplx ? 2 * out.ldu : out.ldu;
+    if (p.mixed)
+      rc = launch_gemm_mx(st, (const float*)A, ldr, p.m, p.nr, center, mean, var, b.Wh, b.Wl, b.wscale, p.nr,
+                          p.chunks3, p.tpc3, (float*)out.U, ldu);
+    else
+      rc = launch_gemm_f64(st, sdt, A, ldr, p.m, p.nr, center, mean, var, b.Wpad, p.nr, p.chunks3, p.tpc3, out.U, ldu);
+    if (rc != 0) return pc_fail(ctx, PC_ERR_SHAPE, "no gemm variant for n = %d", n);
+  }
+  hipLaunchKernelGGL((k_null_fix<T, typename VtType<T, ST>::type>), dim3(nblk(nn)), dim3(256), 0, st, b.nnull, n,
+                     b.jsel, b.cnull, (typename VtType<T, ST>::type*)out.U, out.ldu);
+  PC_HIP(ctx, hipGetLastError());
+  // status check: one small D2H copy, synchronous (errors must surface here)
+  PC_HIP(ctx, hipMemcpyAsync(ctx->hflag, b.status, 16, hipMemcpyDeviceToHost, st));
+  PC_HIP(ctx, hipStreamSynchronize(st));
+  int s0 = ctx->hflag[0];
+  if (s0 & 1) return pc_fail(ctx, PC_ERR_LINALG, "non-finite values in the input (Gram pivot is not finite)");
+  if (s0 & 2) return pc_fail(ctx, PC_ERR_LINALG, "Jacobi SVD did not converge");
+  if (s0 & 4) return pc_fail(ctx, PC_ERR_LINALG, "null-space completion failed (rows of high leverage)");
+  return PC_OK;
+}
+
+static int core_dispatch(pc_ctx* ctx, pc_dtype dt, const void* A, int64_t lda, int64_t m, int64_t n, int center,
+                         int mode, CoreOut out, int precision) {
+  Plan p;
+  PC_CHECK(ctx, make_plan(ctx, dt, m, n, precision, p));
+  switch (dt) {
+    case PC_F64: return core_t<double, double>(ctx, p, A, lda, center, mode, out);
+    case PC_F32: return core_t<double, float>(ctx, p, A, lda, center, mode, out);
+    case PC_C128: return core_t<cdouble, double>(ctx, p, A, lda, center, mode, out);
+    case PC_C64: return core_t<cdouble, float>(ctx, p, A, lda, center, mode, out);
+  }
+  return pc_fail(ctx, PC_ERR_INVALID, "bad dtype");
+}
+
+static int check_common(pc_ctx* ctx, pc_dtype dt, int64_t m, int64_t n) {
+  if (!ctx) return pc_fail(nullptr, PC_ERR_INVALID, "ctx is NULL");
+  if (dt < PC_F32 || dt > PC_C128) return pc_fail(ctx, PC_ERR_INVALID, "bad dtype %d", int(dt));
+  if (n <= 0 || m <= 0) return pc_fail(ctx, PC_ERR_SHAPE, "empty matrix (m=%lld, n=%lld)", (long long)m, (long long)n);
+  if (m < n) return pc_fail(ctx, PC_ERR_SHAPE, "tall-skinny solve needs m >= n (m=%lld, n=%lld)", (long long)m, (long long)n);
+  if (n > 512) return pc_fail(ctx, PC_ERR_SHAPE, "n = %lld > 512 is not supported", (long long)n);
+  PC_HIP(ctx, hipSetDevice(ctx->device));
+  return PC_OK;
+}
+
+extern "C" int pc_tsqr_svd(pc_ctx* ctx, pc_dtype dt, const void* A, int64_t lda, int64_t m, int64_t n, int center,
+                           void* U, int64_t ldu, void* S, void* VT, int64_t ldvt, int precision) {
+  PC_CHECK(ctx, check_common(ctx, dt, m, n));
+  PC_CHECK(ctx, pc_check_mat(ctx, A, lda, m, n, "pc_tsqr_svd A"));
+  PC_CHECK(ctx, pc_check_mat(ctx, U, ldu, m, n, "pc_tsqr_svd U"));
+  PC_CHECK(ctx, pc_check_mat(ctx, VT, ldvt, n, n, "pc_tsqr_svd VT"));
+  if (!S) return pc_fail(ctx, PC_ERR_INVALID, "pc_tsqr_svd: S is NULL");
+  if (center < 0 || center > 2) return pc_fail(ctx, PC_ERR_INVALID, "bad center mode %d", center);
+  if (center && dtype_complex(dt)) return pc_fail(ctx, PC_ERR_UNSUPPORTED, "row centring is for real data");
+  CoreOut o{U, ldu, S, VT, ldvt};
+  return core_dispatch(ctx, dt, A, lda, m, n, center, CORE_SVD, o, precision);
+}
+
+extern "C" int pc_tsqr(pc_ctx* ctx, pc_dtype dt, const void* A, int64_t lda, int64_t m, int64_t n, void* Q,
+                       int64_t ldq, void* R, int64_t ldr, int precision) {
+  PC_CHECK(ctx, check_common(ctx, dt, m, n));
+  PC_CHECK(ctx, pc_check_mat(ctx, A, lda, m, n, "pc_tsqr A"));
+  PC_CHECK(ctx, pc_check_mat(ctx, Q, ldq, m, n, "pc_tsqr Q"));
+  PC_CHECK(ctx, pc_check_mat(ctx, R, ldr, n, n, "pc_tsqr R"));
+  CoreOut o{Q, ldq, nullptr, R, ldr};
+  return core_dispatch(ctx, dt, A, lda, m, n, 0, CORE_QR, o, precision);
+}
+
+#include "ops.inc"

@@ -1,0 +1,310 @@
+"""``pyLOM.math`` counterparts of the POD core, running on the MI355X.
+
+Same names, argument meaning and return values as the reference's
+pyLOM/vmmath (svd.py, maths.py, averaging.py, truncation.py, stats.py); each
+function cites the reference function it replaces.  Inputs may be numpy arrays
+(results come back as numpy) or torch tensors (results stay on the device).
+One extension keyword, ``precision``, selects the arithmetic of the streaming
+passes: ``"fp64"`` (default, FP64 MFMA whatever the storage) or ``"mixed"``
+(float32/complex64 storage: fp16 hi/lo split MFMA, FP64 small steps).
+
+Errors raise ``pyloworder_amd.LinAlgError`` / ``PodCoreError`` instead of the
+reference's MPI_Abort (pyLOM/utils/errors.py:22-28).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import time
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._tensor import Arg, empty, pc_dtype, real_dtype
+from .cr import cr
+
+__all__ = [
+    "qr", "svd", "tsqr", "tsqr_svd", "randomized_qr", "randomized_svd", "matmul", "matmulp",
+    "vecmat", "transpose", "temporal_mean", "subtract_mean", "temporal_variance", "norm_variance",
+    "compute_truncation_residual", "RMSE", "energy", "reference_omega",
+]
+
+_PREC = {"fp64": _lib.PC_PREC_FP64, "mixed": _lib.PC_PREC_MIXED}
+
+
+def _prec(precision):
+    try:
+        return _PREC[precision]
+    except KeyError:
+        raise ValueError(f"precision must be 'fp64' or 'mixed', not {precision!r}") from None
+
+
+def _ctx():
+    return _lib.context()
+
+
+# ---------------------------------------------------------------------------
+# SVD / QR  (pyLOM/vmmath/svd.py)
+# ---------------------------------------------------------------------------
+@cr("math.tsqr_svd")
+def tsqr_svd(Ai, precision: str = "fp64", center: int = 0):
+    """U, S, V of a tall-skinny matrix — pyLOM/vmmath/svd.py:227-252.
+
+    Ui (m, n) left singular vectors, S (n,) descending, V (n, n) = V^H (rows are
+    the right singular vectors).  ``center`` = 1 removes the row mean (2 also
+    divides by the row variance) inside the passes, as POD.run does before
+    calling tsqr_svd (pyLOM/POD/wrapper.py:33-45).
+    """
+    a = Arg(Ai, "Ai")
+    m, n = a.t.shape
+    U = empty((m, n), a.t.dtype, a.t)
+    S = empty((n,), real_dtype(a.t.dtype), a.t)
+    V = empty((n, n), a.t.dtype, a.t)
+    ctx = _ctx()
+    ctx.check(ctx.lib.pc_tsqr_svd(ctx.h, pc_dtype(a.t), a.ptr(), a.ld, m, n, int(center), U.data_ptr(), U.stride(0),
+                                  S.data_ptr(), V.data_ptr(), V.stride(0), _prec(precision)))
+    return a.give(U), a.give(S), a.give(V)
+
+
+@cr("math.tsqr")
+def tsqr(Ai, precision: str = "fp64"):
+    """Qi (m, n), R (n, n) — pyLOM/vmmath/svd.py:49-118.  R is unique up to row signs."""
+    a = Arg(Ai, "Ai")
+    m, n = a.t.shape
+    Q = empty((m, n), a.t.dtype, a.t)
+    R = empty((n, n), a.t.dtype, a.t)
+    ctx = _ctx()
+    ctx.check(ctx.lib.pc_tsqr(ctx.h, pc_dtype(a.t), a.ptr(), a.ld, m, n, Q.data_ptr(), Q.stride(0), R.data_ptr(),
+                              R.stride(0), _prec(precision)))
+    return a.give(Q), a.give(R)
+
+
+@cr("math.qr")
+def qr(A, precision: str = "fp64"):
+    """Reduced QR — pyLOM/vmmath/svd.py:28-36 (tall or square matrices)."""
+    a = Arg(A, "A")
+    m, n = a.t.shape
+    if m < n:
+        raise ValueError("qr: this build factors tall or square matrices (m >= n)")
+    return tsqr(A, precision=precision)
+
+
+@cr("math.svd")
+def svd(A, method: str = "gesdd"):
+    """Thin SVD (U, S, V=V^H) — pyLOM/vmmath/svd.py:38-47.  ``method`` is accepted
+    for signature compatibility; one-sided Jacobi is used for every matrix."""
+    a = Arg(A, "A")
+    m, n = a.t.shape
+    k = min(m, n)
+    U = empty((m, k), a.t.dtype, a.t)
+    S = empty((k,), real_dtype(a.t.dtype), a.t)
+    V = empty((k, n), a.t.dtype, a.t)
+    ctx = _ctx()
+    ctx.check(ctx.lib.pc_svd(ctx.h, pc_dtype(a.t), a.ptr(), a.ld, m, n, U.data_ptr(), U.stride(0), S.data_ptr(),
+                             V.data_ptr(), V.stride(0)))
+    return a.give(U), a.give(S), a.give(V)
+
+
+def reference_omega(n: int, r: int, seed: int, dtype) -> np.ndarray:
+    """The reference's sketch matrix: np.random.seed(seed); rand(n, r) —
+    pyLOM/vmmath/svd.py:131-133 (numpy legacy MT19937, frozen by numpy)."""
+    seed = int(time.time()) if seed < 0 else seed
+    rs = np.random.RandomState(seed)
+    return rs.random_sample((n, r)).astype(dtype)
+
+
+def _omega_tensor(a: Arg, r: int, seed: int, omega):
+    n = a.t.shape[1]
+    if omega is None:
+        np_dt = {torch.float32: np.float32, torch.float64: np.float64,
+                 torch.complex64: np.complex64, torch.complex128: np.complex128}[a.t.dtype]
+        om = reference_omega(n, r, seed, np_dt)
+        return torch.from_numpy(om).to(a.t.device)
+    o = Arg(omega, "omega", dtype=a.t.dtype).t
+    if tuple(o.shape) != (n, r):
+        raise ValueError(f"omega must be ({n}, {r})")
+    return o.contiguous()
+
+
+@cr("math.randomized_qr")
+def randomized_qr(Ai, r: int, q: int, seed: int = -1, omega=None, precision: str = "fp64"):
+    """Qi (m, r), B (r, n) — pyLOM/vmmath/svd.py:120-144 (same Omega stream)."""
+    a = Arg(Ai, "Ai")
+    m, n = a.t.shape
+    om = _omega_tensor(a, r, seed, omega)
+    Q = empty((m, r), a.t.dtype, a.t)
+    B = empty((r, n), a.t.dtype, a.t)
+    ctx = _ctx()
+    ctx.check(ctx.lib.pc_randomized_qr(ctx.h, pc_dtype(a.t), a.ptr(), a.ld, m, n, r, q, max(seed, 0), om.data_ptr(),
+                                       Q.data_ptr(), Q.stride(0), B.data_ptr(), B.stride(0), _prec(precision)))
+    return a.give(Q), a.give(B)
+
+
+@cr("math.randomized_svd")
+def randomized_svd(Ai, r: int, q: int, seed: int = -1, omega=None, precision: str = "fp64"):
+    """Ui (m, r), S (r,), V (r, n) — pyLOM/vmmath/svd.py:254-273."""
+    a = Arg(Ai, "Ai")
+    m, n = a.t.shape
+    om = _omega_tensor(a, r, seed, omega)
+    U = empty((m, r), a.t.dtype, a.t)
+    S = empty((r,), real_dtype(a.t.dtype), a.t)
+    V = empty((r, n), a.t.dtype, a.t)
+    ctx = _ctx()
+    ctx.check(ctx.lib.pc_randomized_svd(ctx.h, pc_dtype(a.t), a.ptr(), a.ld, m, n, r, q, max(seed, 0),
+                                        om.data_ptr(), U.data_ptr(), U.stride(0), S.data_ptr(), V.data_ptr(),
+                                        V.stride(0), _prec(precision)))
+    return a.give(U), a.give(S), a.give(V)
+
+
+# ---------------------------------------------------------------------------
+# products  (pyLOM/vmmath/maths.py)
+# ---------------------------------------------------------------------------
+def _view_op(x):
+    """(tensor, op) with op 0/1/2 for plain / transposed / conj-transposed views."""
+    if isinstance(x, torch.Tensor) and x.dim() == 2 and x.stride(0) == 1 and x.stride(1) >= x.shape[0] and x.shape[0] > 1:
+        return x.t(), 1
+    return x, 0
+
+
+@cr("math.matmul")
+def matmul(A, B, precision: str = "fp64"):
+    """C = A B — pyLOM/vmmath/maths.py:76-90."""
+    was_np = isinstance(A, np.ndarray)
+    ta, opa = _view_op(A if not was_np else A)
+    tb, opb = _view_op(B if not isinstance(B, np.ndarray) else B)
+    a = Arg(ta, "A")
+    b = Arg(tb, "B", dtype=a.t.dtype)
+    am, ak = a.t.shape if opa == 0 else a.t.shape[::-1]
+    bk, bn = b.t.shape if opb == 0 else b.t.shape[::-1]
+    if ak != bk:
+        raise ValueError(f"matmul: inner dimensions differ ({ak} vs {bk})")
+    Cm = empty((am, bn), a.t.dtype, a.t)
+    ctx = _ctx()
+    ctx.check(ctx.lib.pc_gemm(ctx.h, pc_dtype(a.t), opa, opb, am, bn, ak, 1.0, a.ptr(), a.ld, b.ptr(), b.ld, 0.0,
+                              Cm.data_ptr(), Cm.stride(0), _prec(precision)))
+    return a.give(Cm)
+
+
+@cr("math.matmulp")
+def matmulp(A, B, precision: str = "fp64"):
+    """pyLOM/vmmath/maths.py:92-110.  On one GPU the rank sum is the product."""
+    return matmul(A, B, precision=precision)
+
+
+@cr("math.vecmat")
+def vecmat(v, A):
+    """C[i,:] = v[i] A[i,:] — pyLOM/vmmath/maths.py:112-129 (one launch, no row loop)."""
+    a = Arg(A, "A")
+    vv = Arg(v, "v", dtype=a.t.dtype if a.t.is_complex() else real_dtype(a.t.dtype)).t.contiguous().reshape(-1)
+    m, n = a.t.shape
+    if vv.shape[0] != m:
+        raise ValueError("vecmat: len(v) must equal A.shape[0]")
+    if a.t.is_complex() and not vv.is_complex():
+        vv = vv.to(a.t.dtype)
+    Cm = empty((m, n), a.t.dtype, a.t)
+    ctx = _ctx()
+    ctx.check(ctx.lib.pc_vecmat(ctx.h, pc_dtype(a.t), vv.data_ptr(), a.ptr(), a.ld, m, n, Cm.data_ptr(), Cm.stride(0)))
+    return a.give(Cm)
+
+
+@cr("math.transpose")
+def transpose(A):
+    """A^T — pyLOM/vmmath/maths.py:17-29 (materialised)."""
+    a = Arg(A, "A")
+    m, n = a.t.shape
+    B = empty((n, m), a.t.dtype, a.t)
+    ctx = _ctx()
+    ctx.check(ctx.lib.pc_transpose(ctx.h, pc_dtype(a.t), a.ptr(), a.ld, m, n, B.data_ptr(), B.stride(0)))
+    return a.give(B)
+
+
+# ---------------------------------------------------------------------------
+# averaging  (pyLOM/vmmath/averaging.py)
+# ---------------------------------------------------------------------------
+@cr("math.temporal_mean")
+def temporal_mean(X):
+    """Row mean over snapshots, shape (m,) — averaging.py:17-29."""
+    a = Arg(X, "X")
+    m, n = a.t.shape
+    mean = empty((m,), a.t.dtype, a.t)
+    ctx = _ctx()
+    ctx.check(ctx.lib.pc_row_stats(ctx.h, pc_dtype(a.t), a.ptr(), a.ld, m, n, mean.data_ptr(), None))
+    return a.give(mean)
+
+
+@cr("math.subtract_mean")
+def subtract_mean(X, X_mean):
+    """X - X_mean[:, None] — averaging.py:31-44 (no tiled temporary)."""
+    a = Arg(X, "X")
+    mu = Arg(X_mean, "v", dtype=a.t.dtype).t.contiguous().reshape(-1)
+    m, n = a.t.shape
+    Y = empty((m, n), a.t.dtype, a.t)
+    ctx = _ctx()
+    ctx.check(ctx.lib.pc_row_center(ctx.h, pc_dtype(a.t), a.ptr(), a.ld, m, n, mu.data_ptr(), None, Y.data_ptr(),
+                                    Y.stride(0)))
+    return a.give(Y)
+
+
+@cr("math.temporal_variance")
+def temporal_variance(X, X_mean=None):
+    """Population variance over snapshots, shape (m, 1) — averaging.py:46-59."""
+    a = Arg(X, "X")
+    m, n = a.t.shape
+    mean = empty((m,), a.t.dtype, a.t)
+    var = empty((m, 1), a.t.dtype, a.t)
+    ctx = _ctx()
+    ctx.check(ctx.lib.pc_row_stats(ctx.h, pc_dtype(a.t), a.ptr(), a.ld, m, n, mean.data_ptr(), var.data_ptr()))
+    return a.give(var)
+
+
+@cr("math.temporal_variance")
+def norm_variance(X, X_mean, X_var):
+    """(X - mean) / var (VARIANCE, not std) — averaging.py:61-75."""
+    a = Arg(X, "X")
+    mu = Arg(X_mean, "v", dtype=a.t.dtype).t.contiguous().reshape(-1)
+    var = Arg(X_var, "v", dtype=a.t.dtype).t.contiguous().reshape(-1)
+    m, n = a.t.shape
+    Y = empty((m, n), a.t.dtype, a.t)
+    ctx = _ctx()
+    ctx.check(ctx.lib.pc_row_center(ctx.h, pc_dtype(a.t), a.ptr(), a.ld, m, n, mu.data_ptr(), var.data_ptr(),
+                                    Y.data_ptr(), Y.stride(0)))
+    return a.give(Y)
+
+
+# ---------------------------------------------------------------------------
+# truncation / statistics  (truncation.py, stats.py)
+# ---------------------------------------------------------------------------
+def compute_truncation_residual(S, r):
+    """Number of modes kept — pyLOM/vmmath/truncation.py:17-39 (host, n <= 512)."""
+    s = S.detach().double().cpu().numpy() if isinstance(S, torch.Tensor) else np.asarray(S, dtype=np.float64)
+    s = np.ascontiguousarray(s.reshape(-1))
+    N = C.c_int64(0)
+    lib = _lib.load()
+    rc = lib.pc_truncation_residual(s.ctypes.data_as(C.POINTER(C.c_double)), s.shape[0], float(r), C.byref(N))
+    if rc != 0:
+        raise _lib.PodCoreError(rc, lib.pc_last_error(None).decode())
+    return int(N.value)
+
+
+def _diff_norms(A, B):
+    a = Arg(A, "A")
+    b = Arg(B, "B", dtype=a.t.dtype)
+    m, n = a.t.shape if a.t.dim() == 2 else (a.t.shape[0], 1)
+    out = (C.c_double * 2)()
+    ctx = _ctx()
+    ctx.check(ctx.lib.pc_diff_norms(ctx.h, pc_dtype(a.t), a.ptr(), a.ld, b.ptr(), b.ld, m, n, out))
+    return out[0], out[1], m * n
+
+
+@cr("math.RMSE")
+def RMSE(A, B, relative: bool = True) -> float:
+    """Root mean square error — pyLOM/vmmath/stats.py:16-34."""
+    s1, s2, cnt = _diff_norms(A, B)
+    return float(np.sqrt(s1 / (s2 if relative else cnt)))
+
+
+@cr("math.energy")
+def energy(original, rec) -> float:
+    """1 - sum (o - r)^2 / sum o^2 — pyLOM/vmmath/truncation.py:41-60."""
+    s1, s2, _ = _diff_norms(original, rec)
+    return float(1.0 - s1 / s2)

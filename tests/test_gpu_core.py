@@ -1,0 +1,215 @@
+"""GPU parity of the POD core (FP64 path) against the CPU oracle.
+
+Tolerances (FP64 arithmetic on the device): singular values 1e-10 relative on
+the gated modes (s >= 1e-3 s_1) and 1e-8 on every non-zero mode, largest
+principal angle of the gated left subspace 1e-8, orthogonality 1e-10,
+reconstruction RMSE 1e-12.  Singular vectors are compared as subspaces
+(signs are arbitrary, as between the reference's 1-rank and 4-rank runs).
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from oracle import metrics, workloads
+
+pytestmark = pytest.mark.gpu
+
+pl = pytest.importorskip("pyloworder_amd")
+
+
+def _np(x):
+    return x.detach().cpu().numpy() if isinstance(x, torch.Tensor) else np.asarray(x)
+
+
+def check_svd(Y, U, S, V, gated=1e-10, allrel=1e-8, ang=1e-8, orth=1e-10, rmse=1e-12, nranks=1):
+    Y = np.asarray(Y)
+    Uo, So, Vo = oracle.tsqr_svd(oracle.split_rows(Y.astype(np.complex128 if np.iscomplexobj(Y) else np.float64), nranks))
+    Uo = np.concatenate(Uo)
+    U, S, V = _np(U), _np(S), _np(V)
+    e = metrics.sigma_errors(S, So)
+    assert e["maxrel_gated"] <= gated, e
+    assert e["maxrel_all"] <= allrel, e
+    k = e["n_gated"]
+    assert metrics.max_principal_angle(U, Uo, k) <= ang
+    assert metrics.orthogonality(U) <= orth
+    assert metrics.orthogonality(np.conj(V).T) <= orth
+    assert metrics.recon_rmse(Y, U, S, V) <= rmse
+    return e
+
+
+@pytest.mark.parametrize("m,n", [(64, 16), (1000, 7), (4099, 33), (5000, 64), (3001, 100), (2048, 256)])
+def test_tsqr_svd_fp64_shapes(m, n, rng):
+    Y = rng.standard_normal((m, n)) @ np.diag(np.logspace(0, -6, n))
+    U, S, V = pl.math.tsqr_svd(Y)
+    check_svd(Y, U, S, V)
+
+
+def test_tsqr_svd_fp64_torch_in_torch_out(rng):
+    Y = rng.standard_normal((3000, 40))
+    t = torch.from_numpy(Y).cuda()
+    U, S, V = pl.math.tsqr_svd(t)
+    assert isinstance(U, torch.Tensor) and U.is_cuda
+    check_svd(Y, U, S, V)
+
+
+def test_tsqr_svd_known_answer():
+    # Examples/example_SVD.py:17 — sigma = 14.2690955, 0.6268282
+    A = np.array([[1, 2], [3, 4], [5, 6], [7, 8]], dtype=np.double)
+    U, S, V = pl.math.tsqr_svd(A)
+    assert np.allclose(S, [14.2690955, 0.6268282], rtol=1e-7)
+    assert np.allclose(np.abs(U), np.abs(np.linalg.svd(A, full_matrices=False)[0]), atol=1e-12)
+
+
+def test_pod_run_sines_config():
+    # ten travelling sines, 20,000 x 200, remove_mean=True (rank-deficient: Y 1 = 0)
+    X = workloads.sines(20000, 200)
+    U, S, V = pl.POD.run(X, remove_mean=True)
+    Y = oracle.subtract_mean(X, oracle.temporal_mean(X))
+    e = check_svd(Y, U, S, V, allrel=1e-6, rmse=1e-11)
+    assert abs(_np(S)[-1]) <= 1e-10 * _np(S)[0]
+
+
+def test_pod_run_divide_variance(rng):
+    X = rng.standard_normal((3000, 50)) * rng.uniform(0.5, 2, (3000, 1)) + 5.0
+    U, S, V = pl.POD.run(X, remove_mean=True, divide_variance=True)
+    mu = oracle.temporal_mean(X)
+    Y = oracle.norm_variance(X, mu, oracle.temporal_variance(X, mu))
+    check_svd(Y, U, S, V, allrel=1e-7, rmse=1e-11)
+
+
+def test_pod_run_no_mean_matches_multirank_oracle(rng):
+    X = workloads.lowrank(6000, 48, seed=5, decay=0.8, top=10.0, noise=1e-4)
+    U, S, V = pl.POD.run(X, remove_mean=False)
+    check_svd(X, U, S, V, nranks=4)
+
+
+def test_tsqr_svd_float32_storage_fp64_math(rng):
+    Y = rng.standard_normal((5000, 64)).astype(np.float32)
+    U, S, V = pl.math.tsqr_svd(Y)
+    assert U.dtype == np.float32 and S.dtype == np.float32
+    check_svd(Y.astype(np.float64), U.astype(np.float64), S.astype(np.float64), V.astype(np.float64),
+              gated=1e-6, allrel=1e-5, ang=1e-5, orth=1e-6, rmse=1e-6)
+
+
+@pytest.mark.parametrize("dt", [np.complex128, np.complex64])
+def test_tsqr_svd_complex(dt, rng):
+    m, n = 4000, 15
+    Y = (rng.standard_normal((m, n)) + 1j * rng.standard_normal((m, n))) @ np.diag(np.logspace(0, -3, n))
+    Y = Y.astype(dt)
+    U, S, V = pl.math.tsqr_svd(Y)
+    tol = 1e-10 if dt == np.complex128 else 1e-5
+    check_svd(Y.astype(np.complex128), U.astype(np.complex128), S.astype(np.float64), V.astype(np.complex128),
+              gated=tol, allrel=tol * 10, ang=tol * 10, orth=tol, rmse=tol)
+
+
+def test_rank_deficient_exact(rng):
+    # rank 5 in a 3000 x 40 matrix: U still orthonormal (null-space completion)
+    Y = rng.standard_normal((3000, 5)) @ rng.standard_normal((5, 40))
+    U, S, V = pl.math.tsqr_svd(Y)
+    U, S, V = _np(U), _np(S), _np(V)
+    So = np.linalg.svd(Y, compute_uv=False)
+    assert np.allclose(S[:5], So[:5], rtol=1e-11)
+    assert np.all(S[5:] <= 1e-10 * S[0])
+    assert metrics.orthogonality(U) <= 1e-10
+    assert metrics.recon_rmse(Y, U, S, V) <= 1e-12
+
+
+def test_zero_and_constant_columns(rng):
+    Y = rng.standard_normal((2000, 10))
+    Y[:, 3] = 0.0
+    U, S, V = pl.math.tsqr_svd(Y)
+    assert metrics.orthogonality(_np(U)) <= 1e-10
+    assert metrics.recon_rmse(Y, _np(U), _np(S), _np(V)) <= 1e-12
+
+
+def test_tsqr_qr(rng):
+    A = rng.standard_normal((5000, 30)) @ np.diag(np.logspace(0, -5, 30))
+    Q, R = pl.math.tsqr(A)
+    Q, R = _np(Q), _np(R)
+    assert np.abs(np.tril(R, -1)).max() == 0.0
+    assert np.abs(Q @ R - A).max() <= 1e-12 * np.abs(A).max()
+    assert metrics.orthogonality(Q) <= 1e-10
+    Qo, Ro = oracle.tsqr(oracle.split_rows(A, 3))
+    assert np.allclose(np.abs(R), np.abs(Ro), rtol=1e-9, atol=1e-12 * np.abs(Ro).max())
+
+
+def test_dense_svd_square_and_wide(rng):
+    for shape in [(50, 50), (20, 70), (70, 20)]:
+        A = rng.standard_normal(shape)
+        U, S, V = pl.math.svd(A)
+        So = np.linalg.svd(A, compute_uv=False)
+        assert np.allclose(_np(S), So, rtol=1e-11)
+        assert metrics.recon_rmse(A, _np(U), _np(S), _np(V)) <= 1e-12
+
+
+def test_randomized_matches_oracle_with_same_omega():
+    X = workloads.lowrank(20000, 64, seed=7, decay=0.7, top=100.0, noise=1e-6)
+    U, S, V = pl.math.randomized_svd(X, 12, 3, seed=25)
+    Uo, So, Vo = oracle.randomized_svd(oracle.split_rows(X, 1), 12, 3, seed=25)
+    Uo = np.concatenate(Uo)
+    assert np.allclose(_np(S), So, rtol=1e-9)
+    assert metrics.max_principal_angle(_np(U), Uo, 12) <= 1e-7
+
+
+def test_randomized_qr_shapes():
+    X = workloads.lowrank(5000, 40, seed=8, decay=0.8)
+    Q, B = pl.math.randomized_qr(X, 8, 2, seed=3)
+    Q, B = _np(Q), _np(B)
+    assert Q.shape == (5000, 8) and B.shape == (8, 40)
+    assert metrics.orthogonality(Q) <= 1e-10
+    assert np.allclose(Q.T @ X, B, atol=1e-10 * np.abs(B).max())
+
+
+def test_matmul_variants(rng):
+    A = rng.standard_normal((5000, 40))
+    B = rng.standard_normal((40, 17))
+    assert np.allclose(pl.math.matmul(A, B), A @ B, atol=1e-11)
+    tA = torch.from_numpy(A).cuda()
+    tQ = torch.from_numpy(rng.standard_normal((5000, 9))).cuda()
+    got = pl.math.matmulp(tA.T, tQ)
+    assert np.allclose(_np(got), A.T @ _np(tQ), atol=1e-10)
+    small = pl.math.matmul(rng.standard_normal((5, 3)).astype(np.float64), np.arange(12, dtype=np.float64).reshape(3, 4))
+    assert small.shape == (5, 4)
+    # Examples/example_matmul.py: integer 5x3 . 3x4
+    a = np.arange(15, dtype=np.float64).reshape(5, 3)
+    b = np.arange(12, dtype=np.float64).reshape(3, 4)
+    assert np.array_equal(pl.math.matmul(a, b), a @ b)
+
+
+def test_reconstruct_project_vecmat(rng):
+    X = workloads.lowrank(4000, 30, seed=3, decay=0.8, top=5.0, noise=0.0)
+    U, S, V = pl.POD.run(X, remove_mean=False)
+    Ur, Sr, Vr = pl.POD.truncate(U, S, V, r=10)
+    Xr = pl.POD.reconstruct(Ur, Sr, Vr)
+    ref = oracle.pod_reconstruct(_np(Ur), _np(Sr), _np(Vr))
+    assert np.allclose(Xr, ref, atol=1e-12)
+    Cp = pl.POD.project(Ur, X)
+    assert np.allclose(Cp, _np(Ur).T @ X, atol=1e-10)
+    v = rng.standard_normal(30)
+    assert np.allclose(pl.math.vecmat(v, _np(V)), oracle.vecmat(v, _np(V)))
+
+
+def test_averaging_and_stats(rng):
+    X = rng.standard_normal((1000, 37)) + 3.0
+    mu = pl.math.temporal_mean(X)
+    assert np.allclose(mu, oracle.temporal_mean(X))
+    Y = pl.math.subtract_mean(X, mu)
+    assert np.allclose(Y, oracle.subtract_mean(X, mu))
+    var = pl.math.temporal_variance(X, mu)
+    assert var.shape == (1000, 1) and np.allclose(var, oracle.temporal_variance(X, mu))
+    Z = pl.math.norm_variance(X, mu, var)
+    assert np.allclose(Z, oracle.norm_variance(X, mu, var))
+    B = X + 1e-3 * rng.standard_normal(X.shape)
+    assert np.isclose(pl.math.RMSE(X, B), oracle.RMSE(X, B), rtol=1e-12)
+    assert np.isclose(pl.math.RMSE(X, B, relative=False), oracle.RMSE(X, B, relative=False), rtol=1e-12)
+    assert np.isclose(pl.math.energy(X, B), oracle.energy(X, B), rtol=1e-12)
+    T = pl.math.transpose(X)
+    assert np.array_equal(T, X.T)
+
+
+def test_errors_are_raised_not_aborts():
+    with pytest.raises(pl.PodCoreError):
+        pl.math.tsqr_svd(np.zeros((3, 5)))       # m < n
+    with pytest.raises(pl.LinAlgError):
+        pl.math.tsqr_svd(np.full((100, 4), np.nan))
