@@ -1,0 +1,242 @@
+"""Benchmark of the POD core on one MI355X (prints ONE JSON line).
+
+Workload ("tall-skinny SVD 8M x 256"): X = G1 diag(s) G2^T + N(0, 1e-6),
+G1 ~ N(0,1)/sqrt(m) (8,000,000 x 256), G2 orthonormal, s_k = 1e3 0.93^k,
+generated on the device from a fixed seed (not timed).  One step = one
+POD.run(X, remove_mean=True) = row centring + tall-skinny SVD, with the input
+already resident in HBM.
+
+  * mixed: float32 storage (8.2 GB), fp16 hi/lo split MFMA, FP64 small steps —
+    the headline `value` (algorithmic HBM GB/s = 3 m n 4 B / time per solve);
+  * fp64:  float64 storage (16.4 GB), FP64 MFMA — reported beside it.
+
+Singular-value errors of the mixed result are taken against the FP64 GPU
+path run on the same stored (float32, upcast) data; that path is itself
+checked against the CPU oracle by tests/test_gpu_core.py.  The CPU baseline
+is the oracle's numpy restatement of the reference algorithm (butterfly TSQR +
+svd(R) + Q U_R) on a bounded row sample of the same matrix.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+M_DEFAULT, N_DEFAULT = 8_000_000, 256
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--m", type=int, default=M_DEFAULT)
+    ap.add_argument("--n", type=int, default=N_DEFAULT)
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 path")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-rows", type=int, default=400_000)
+    ap.add_argument("--mode", default="both", choices=["both", "mixed", "fp64"])
+    return ap.parse_args()
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def generate(m, n, seed=2024):
+    """Low-rank-plus-noise matrix in float64 on the device, in row chunks."""
+    import torch
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    G2 = torch.linalg.qr(torch.randn(n, n, generator=g, device="cuda", dtype=torch.float64))[0]
+    s = 1e3 * 0.93 ** torch.arange(n, device="cuda", dtype=torch.float64)
+    B = (s[:, None] * G2.T).contiguous()   # X = G1 diag(s) G2^T
+    X = torch.empty((m, n), device="cuda", dtype=torch.float64)
+    step = 1 << 20
+    inv = 1.0 / np.sqrt(m)
+    for r0 in range(0, m, step):
+        r1 = min(m, r0 + step)
+        G1 = torch.randn(r1 - r0, n, generator=g, device="cuda", dtype=torch.float64) * inv
+        X[r0:r1] = G1 @ B + 1e-6 * torch.randn(r1 - r0, n, generator=g, device="cuda", dtype=torch.float64)
+    torch.cuda.synchronize()
+    return X
+
+
+class Solver:
+    """Direct C-ABI calls (no Python-side allocation inside the timed loop)."""
+
+    def __init__(self, X, precision):
+        import torch
+        from pyloworder_amd import _lib
+        from pyloworder_amd._tensor import pc_dtype, real_dtype
+        self.torch = torch
+        self.ctx = _lib.context()
+        self.X = X
+        m, n = X.shape
+        self.m, self.n = m, n
+        self.U = torch.empty((m, n), dtype=X.dtype, device="cuda")
+        self.S = torch.empty((n,), dtype=real_dtype(X.dtype), device="cuda")
+        self.V = torch.empty((n, n), dtype=X.dtype, device="cuda")
+        self.dt = pc_dtype(X)
+        self.prec = _lib.PC_PREC_MIXED if precision == "mixed" else _lib.PC_PREC_FP64
+
+    def __call__(self):
+        c = self.ctx
+        c.check(c.lib.pc_tsqr_svd(c.h, self.dt, self.X.data_ptr(), self.X.stride(0), self.m, self.n, 1,
+                                  self.U.data_ptr(), self.U.stride(0), self.S.data_ptr(), self.V.data_ptr(),
+                                  self.V.stride(0), self.prec))
+
+    def profile(self, on):
+        self.ctx.check(self.ctx.lib.pc_profile(self.ctx.h, 1 if on else 0))
+
+    def profile_read(self):
+        import ctypes as C
+        ms = (C.c_double * 5)()
+        cnt = (C.c_int64 * 5)()
+        self.ctx.check(self.ctx.lib.pc_profile_read(self.ctx.h, ms, cnt, 5))
+        return list(ms), list(cnt)
+
+
+def time_solves(solver, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        solver()
+    torch.cuda.synchronize()
+    st = torch.cuda.current_stream()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    t0 = time.perf_counter()
+    for a, b in evs:
+        a.record(st)
+        solver()
+        b.record(st)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    lat = sorted(a.elapsed_time(b) for a, b in evs)
+    return wall / steps * 1e3, lat
+
+
+def stage_profile(solver, reps=3):
+    solver.profile(True)
+    for _ in range(reps):
+        solver()
+    ms, cnt = solver.profile_read()
+    solver.profile(False)
+    return {k: (ms[i] / max(cnt[i], 1), cnt[i]) for i, k in enumerate(["gram", "zgram", "gemm"])}
+
+
+def roofline_for(stage_ms, m, n, b):
+    """Dominant streaming kernel: algorithmic bytes per launch / its mean duration."""
+    algo = {"gram": m * n * b, "zgram": m * n * b, "gemm": 2 * m * n * b}
+    name = max(stage_ms, key=lambda k: stage_ms[k][0])
+    t = stage_ms[name][0] * 1e-3
+    ach = algo[name] / t / 1e9
+    return name, {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                  "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None}
+
+
+def cpu_baseline(m_sample, n, seed=7):
+    """The oracle's numpy restatement, threaded BLAS, one simulated rank."""
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    import oracle
+    from oracle import workloads
+    X = workloads.lowrank(m_sample, n, seed=seed, decay=0.93, top=1e3, noise=1e-6, dtype=np.float32)
+    t0 = time.perf_counter()
+    oracle.pod_run(X, remove_mean=True)
+    t = time.perf_counter() - t0
+    gbs = 3 * m_sample * n * 4 / t / 1e9
+    return {"value": round(gbs, 3), "unit": "GB/s", "cores": os.cpu_count(), "kind": "port",
+            "sample": f"oracle POD.run (tsqr_svd) on {m_sample} x {n} float32 rows of the same synthetic "
+                      f"low-rank matrix, numpy/OpenBLAS threaded, 1 rank; {t:.2f} s"}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (see DESIGN.md for sharding)"}))
+        sys.exit(2)
+    import torch
+    import pyloworder_amd  # noqa: F401  (fails loudly if libpodcore.so is missing)
+    m, n = a.m, a.n
+    log(f"[bench] generating {m} x {n} on the device")
+    X64 = generate(m, n)
+    out = {}
+    res64 = None
+    if a.mode in ("both", "fp64") and not a.no_fp64:
+        log("[bench] fp64 path")
+        s64 = Solver(X64, "fp64")
+        ms64, lat64 = time_solves(s64, a.steps, a.warmup)
+        res64 = {"ms_per_solve": round(ms64, 3), "solves_per_s": round(1e3 / ms64, 3),
+                 "hbm_gbs": round(3 * m * n * 8 / (ms64 * 1e-3) / 1e9, 1),
+                 "p50_ms": round(lat64[len(lat64) // 2], 3), "p99_ms": round(lat64[min(len(lat64) - 1, int(0.99 * len(lat64)))], 3),
+                 "stages_ms": {k: round(v[0], 3) for k, v in stage_profile(s64).items()}}
+        log(f"[bench] fp64: {res64}")
+        del s64
+    X32 = X64.float()
+    del X64
+    torch.cuda.empty_cache()
+    log("[bench] mixed path")
+    smx = Solver(X32, "mixed")
+    msmx, latmx = time_solves(smx, a.steps, a.warmup)
+    stages = stage_profile(smx)
+    dom, roof = roofline_for(stages, m, n, 4)
+    # accuracy of the mixed result against FP64 math on the same stored data
+    log("[bench] reference solve (fp64 math on the float32 data)")
+    sref = Solver(X32, "fp64")
+    sref()
+    torch.cuda.synchronize()
+    Sm = smx.S.double().cpu().numpy()
+    Sr = sref.S.double().cpu().numpy()
+    gated = Sr >= 1e-3 * Sr[0]
+    nz = Sr > 1e-10 * Sr[0]
+    rel = np.abs(Sm - Sr) / np.where(nz, Sr, 1.0)
+    k = int(gated.sum())
+    Um = smx.U[:, :k].double()
+    Ur = sref.U[:, :k].double()
+    P = Um - Ur @ (Ur.T @ Um)
+    ang = float(torch.linalg.matrix_norm(P, ord=2).clamp(max=1.0).arcsin())
+    del sref, Um, Ur, P
+    acc = {"sigma_relerr_gated": float(rel[gated].max()), "n_gated": k,
+           "sigma_relerr_all": float(rel[nz].max()), "sigma_normwise": float(np.abs(Sm - Sr).max() / Sr[0]),
+           "max_principal_angle_gated": ang}
+    log(f"[bench] mixed accuracy {acc}")
+    value = 3 * m * n * 4 / (msmx * 1e-3) / 1e9
+    cpu = None
+    if not a.no_cpu:
+        log("[bench] cpu baseline")
+        cpu = cpu_baseline(a.cpu_rows, n)
+    out = {
+        "metric": "TSQR-SVD HBM GB/s (mixed precision POD.run, 8M x 256)",
+        "value": round(value, 1),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(msmx, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 storage, fp16x2 MFMA / FP64 small",
+        "data": "synthetic (device-generated low-rank + noise, seed 2024)",
+        "config": {"workload": "POD.run remove_mean=True on X 8M x 256 float32 (mixed) and float64 (fp64)",
+                   "m": m, "n": n, "parallelism": "single GPU"},
+        "solves_per_s": round(1e3 / msmx, 3),
+        "hbm_roofline_frac_solve": round(value / HBM_PEAK_GBS, 4),
+        "p50_ms": round(latmx[len(latmx) // 2], 3),
+        "p99_ms": round(latmx[min(len(latmx) - 1, int(0.99 * len(latmx)))], 3),
+        "stages_ms": {k_: round(v[0], 3) for k_, v in stages.items()},
+        "dominant_kernel": dom,
+        "roofline": roof,
+        "accuracy_mixed_vs_fp64": acc,
+        "fp64": res64,
+        "mixed_vs_fp64_speedup": round(res64["ms_per_solve"] / msmx, 3) if res64 else None,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -69,6 +69,12 @@ const char* pc_last_error(const pc_ctx* ctx);
 int64_t pc_workspace_bytes(const pc_ctx* ctx);
 int64_t pc_workspace_query(int op, pc_dtype dtype, int64_t m, int64_t n, int64_t r, int precision);
 int pc_synchronize(pc_ctx* ctx);
+/* Per-stage timing (no reference counterpart; diagnostics for measurement):
+ * when enabled, HIP events bracket each streaming pass on the context stream.
+ * pc_profile_read synchronises and returns cumulative milliseconds and launch
+ * counts per stage: 0 Gram pass, 1 preconditioned Gram pass, 2 product pass. */
+int pc_profile(pc_ctx* ctx, int enable);
+int pc_profile_read(pc_ctx* ctx, double* ms, int64_t* counts, int nstages);
 const char* pc_version(void);
 
 /* ---- averaging -----------------------------------------------------------

@@ -10,7 +10,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._tensor import Arg, empty, pc_dtype
+from ._tensor import Arg, empty, pc_dtype, real_dtype
 from . import vmmath
 from .cr import cr, cr_start, cr_stop
 from .vmmath import _ctx, _prec, compute_truncation_residual
@@ -61,7 +61,7 @@ def truncate(U, S, V, r: float = 1e-8):
 def reconstruct(U, S, V, precision: str = "fp64"):
     """X = U diag(S) V — pyLOM/POD/wrapper.py:85-103, one fused pass."""
     u = Arg(U, "U")
-    s = Arg(S, "v", dtype=torch.float64).t.contiguous().reshape(-1)
+    s = Arg(S, "v", dtype=real_dtype(u.t.dtype)).t.contiguous().reshape(-1)
     v = Arg(V, "V", dtype=u.t.dtype)
     m, N = u.t.shape
     N2, n = v.t.shape

@@ -33,6 +33,8 @@ SIGNATURES = {
     "pc_workspace_bytes": (i64, [vp]),
     "pc_workspace_query": (i64, [ip, ip, i64, i64, i64, ip]),
     "pc_synchronize": (ip, [vp]),
+    "pc_profile": (ip, [vp, ip]),
+    "pc_profile_read": (ip, [vp, C.POINTER(C.c_double), C.POINTER(i64), ip]),
     "pc_version": (C.c_char_p, []),
     "pc_row_stats": (ip, [vp, ip, vp, i64, i64, i64, vp, vp]),
     "pc_row_center": (ip, [vp, ip, vp, i64, i64, i64, vp, vp, vp, i64]),

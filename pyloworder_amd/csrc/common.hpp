@@ -47,6 +47,23 @@ PC_HD double cmul_re(double a, double b) { return a * b; }
 // fma-style accumulate conj(a)*b
 PC_HD double dotc_(double a, double b) { return a * b; }
 PC_HD cdouble dotc_(cdouble a, cdouble b) { return cdouble(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x); }
+// 1/a without forming |a|^2 (which underflows for |a| < 1e-154): Smith's division
+PC_HD double inv_(double a) { return 1.0 / a; }
+PC_HD cdouble inv_(cdouble a) {
+  if (fabs(a.x) >= fabs(a.y)) {
+    double r = a.y / a.x, d = a.x + a.y * r;
+    return cdouble(1.0 / d, -r / d);
+  }
+  double r = a.x / a.y, d = a.x * r + a.y;
+  return cdouble(r / d, -1.0 / d);
+}
+// unit-modulus phase of a non-zero scalar, computed by division (no reciprocal overflow)
+PC_HD double phase_(double c, double g) { return c / g; }
+PC_HD cdouble phase_(cdouble c, double g) { return cdouble(c.x / g, c.y / g); }
+// total order key for ranking: NaN sorts as -inf (descending ranks) ...
+PC_HD double key_desc(double v) { return v == v ? v : -__builtin_huge_val(); }
+// ... and as +inf (ascending ranks)
+PC_HD double key_asc(double v) { return v == v ? v : __builtin_huge_val(); }
 template <class T> PC_HD T mk_(double re, double im);
 template <> PC_HD double mk_<double>(double re, double) { return re; }
 template <> PC_HD cdouble mk_<cdouble>(double re, double im) { return cdouble(re, im); }
@@ -65,7 +82,17 @@ struct pc_ctx {
   int* dflag = nullptr;   // device status words (error flags, ranks)
   int* hflag = nullptr;   // pinned host mirror
   std::string err;
+  // optional per-stage timing (pc_profile): events around each streaming pass
+  int prof = 0;
+  double prof_ms[8] = {0};
+  int64_t prof_n[8] = {0};
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> prof_pending;
 };
+
+// stage ids of pc_profile_read
+enum { PCP_GRAM = 0, PCP_ZGRAM = 1, PCP_GEMM = 2, PCP_SMALL = 3, PCP_OTHER = 4 };
+int pc_prof_begin(pc_ctx* ctx, hipEvent_t* ev);
+void pc_prof_end(pc_ctx* ctx, int stage, hipEvent_t ev0);
 
 // status helpers --------------------------------------------------------------
 int pc_fail(pc_ctx* ctx, int code, const char* fmt, ...);

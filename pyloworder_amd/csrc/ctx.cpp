@@ -118,3 +118,47 @@ extern "C" int pc_truncation_residual(const double* S, int64_t n, double r, int6
   *N = cnt;
   return PC_OK;
 }
+
+// ---------------------------------------------------------------------------
+// per-stage timing: events are recorded on the context stream around each
+// streaming pass; pc_profile_read synchronises and folds them in.
+// ---------------------------------------------------------------------------
+int pc_prof_begin(pc_ctx* ctx, hipEvent_t* ev) {
+  *ev = nullptr;
+  if (!ctx->prof) return 0;
+  if (hipEventCreate(ev) != hipSuccess) { *ev = nullptr; return 0; }
+  (void)hipEventRecord(*ev, ctx->stream);
+  return 1;
+}
+
+void pc_prof_end(pc_ctx* ctx, int stage, hipEvent_t ev0) {
+  if (!ctx->prof || !ev0) return;
+  hipEvent_t ev1;
+  if (hipEventCreate(&ev1) != hipSuccess) return;
+  (void)hipEventRecord(ev1, ctx->stream);
+  ctx->prof_pending.push_back({stage, {ev0, ev1}});
+}
+
+extern "C" int pc_profile(pc_ctx* ctx, int enable) {
+  if (!ctx) return pc_fail(nullptr, PC_ERR_INVALID, "pc_profile: ctx is NULL");
+  ctx->prof = enable ? 1 : 0;
+  for (int i = 0; i < 8; i++) { ctx->prof_ms[i] = 0; ctx->prof_n[i] = 0; }
+  return PC_OK;
+}
+
+extern "C" int pc_profile_read(pc_ctx* ctx, double* ms, int64_t* counts, int nstages) {
+  if (!ctx || !ms || !counts) return pc_fail(ctx, PC_ERR_INVALID, "pc_profile_read: NULL argument");
+  PC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  for (auto& e : ctx->prof_pending) {
+    float t = 0;
+    if (hipEventElapsedTime(&t, e.second.first, e.second.second) == hipSuccess) {
+      ctx->prof_ms[e.first] += t;
+      ctx->prof_n[e.first] += 1;
+    }
+    (void)hipEventDestroy(e.second.first);
+    (void)hipEventDestroy(e.second.second);
+  }
+  ctx->prof_pending.clear();
+  for (int i = 0; i < nstages && i < 8; i++) { ms[i] = ctx->prof_ms[i]; counts[i] = ctx->prof_n[i]; }
+  return PC_OK;
+}

@@ -153,8 +153,7 @@ __global__ __launch_bounds__(NT) void k_trinv_upper(const T* R, int ldr, int n, 
   for (int j = wv; j < n; j += NT / 64) {
     // W[:, j]: W[j][j] = 1/R[j][j]; W[i][j] = -(sum_{k=i+1..j} R[i][k] W[k][j]) / R[i][i]
     T rjj = R[int64_t(j) * ldr + j];
-    double den = abs2_(rjj);
-    T wjj = conj_(rjj) * (1.0 / den);
+    T wjj = inv_(rjj);
     if (lane == 0) W[int64_t(j) * ldw + j] = wjj;
     __threadfence_block();
     for (int i = j - 1; i >= 0; i--) {
@@ -162,7 +161,7 @@ __global__ __launch_bounds__(NT) void k_trinv_upper(const T* R, int ldr, int n, 
       for (int k = i + 1 + lane; k <= j; k += 64) s += R[int64_t(i) * ldr + k] * W[int64_t(k) * ldw + j];
       s = group_sum_t(s, 64);
       T rii = R[int64_t(i) * ldr + i];
-      T v = -(s * conj_(rii)) * (1.0 / abs2_(rii));
+      T v = -(s * inv_(rii));
       if (lane == 0) W[int64_t(i) * ldw + j] = v;
       __threadfence_block();
     }
@@ -171,8 +170,8 @@ __global__ __launch_bounds__(NT) void k_trinv_upper(const T* R, int ldr, int n, 
 
 // ---------------------------------------------------------------------------
 // Pivoted Cholesky with rank detection on a working copy A (n x n, upper used):
-// P^T G P = R^H R, R upper-trapezoidal (rows >= rank are zero except a unit
-// diagonal, which keeps R invertible for the later products).  perm[k] = the
+// P^T G P = R^H R, R upper-trapezoidal with rows >= rank exactly zero (the
+// caller pads a unit diagonal later, before inverting).  perm[k] = the
 // original index in pivot position k.  Stops when the largest remaining
 // Schur diagonal is <= tol * max(diag G).
 // ---------------------------------------------------------------------------
@@ -195,7 +194,6 @@ __global__ __launch_bounds__(NT) void k_pchol(const T* G, int ldg, int n, double
   dmax = block_max(dmax, sh);
   if (!(dmax > 0.0) || !isfinite(dmax)) {
     if (threadIdx.x == 0) { rank_out[0] = 0; if (!isfinite(dmax)) status[0] = 1; }
-    for (int i = threadIdx.x; i < n; i += NT) R[int64_t(i) * ldr + i] = T(1.0);
     return;
   }
   double thr = tol * dmax;
@@ -256,7 +254,6 @@ __global__ __launch_bounds__(NT) void k_pchol(const T* G, int ldg, int n, double
     }
     __syncthreads();
   }
-  for (int i = rank + threadIdx.x; i < n; i += NT) R[int64_t(i) * ldr + i] = T(1.0);
   if (threadIdx.x == 0) rank_out[0] = rank;
 }
 
@@ -340,6 +337,17 @@ __global__ __launch_bounds__(NT) void k_jacobi(T* H, int ldh, int rows, int cols
   while (g > 1 && g * npairs > NT) g >>= 1;
   int gid = threadIdx.x / g, gl = threadIdx.x % g;
   int groups = NT / g;
+  // A column whose squared norm is below (eps sqrt(rows))^2 ||H||_F^2 is zero to
+  // working precision: it takes part in no rotation.  Without this test the
+  // null columns of a rank-deficient H keep rotating among themselves, shrink
+  // towards underflow and the iteration never stops.
+  double fro2 = 0.0;
+  for (int64_t idx = threadIdx.x; idx < int64_t(cols) * rows; idx += NT) {
+    int k = int(idx / rows), i = int(idx % rows);
+    fro2 += abs2_(H[int64_t(k) * ldh + i]);
+  }
+  fro2 = block_sum(fro2, sh);
+  const double negl = 2.220446049250313e-16 * 2.220446049250313e-16 * double(rows) * fro2;
   __syncthreads();
   int sweep = 0;
   for (; sweep < max_sweeps; sweep++) {
@@ -364,9 +372,10 @@ __global__ __launch_bounds__(NT) void k_jacobi(T* H, int ldh, int rows, int cols
         b = group_sum(b, g);
         c = group_sum_t(c, g);
         double gm = sqrt(abs2_(c));
-        if (!(gm > tol * sqrt(a * b)) || gm == 0.0) continue;
-        // phase e = c / |c|; real rotation on (h_p, conj(e) h_q)
-        T e = c * (1.0 / gm);
+        if (!(a > negl) || !(b > negl)) continue;
+        if (!(gm > tol * sqrt(a) * sqrt(b))) continue;
+        // phase e = c / |c| (by division: 1/gm could overflow); real rotation on (h_p, conj(e) h_q)
+        T e = phase_(c, gm);
         double zeta = (b - a) / (2.0 * gm);
         double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
         double cs = 1.0 / sqrt(1.0 + t * t), sn = cs * t;
@@ -398,7 +407,7 @@ __global__ __launch_bounds__(NT) void k_jacobi(T* H, int ldh, int rows, int cols
     __syncthreads();
     if (!any) break;
   }
-  if (sweep >= max_sweeps && threadIdx.x == 0) status[0] |= 2;
+  if (sweep >= max_sweeps && threadIdx.x == 0) atomicOr(status, 2);
   // column norms
   int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   for (int k = wv; k < cols; k += NT / 64) {
@@ -406,20 +415,24 @@ __global__ __launch_bounds__(NT) void k_jacobi(T* H, int ldh, int rows, int cols
     const T* hk = H + int64_t(k) * ldh;
     for (int i = lane; i < rows; i += 64) s += abs2_(hk[i]);
     s = wave_sum(s);
-    if (lane == 0) sig[k] = sqrt(s);
+    if (lane == 0) {
+      sig[k] = sqrt(s);
+      if (!isfinite(s)) atomicOr(status, 8);
+    }
   }
-  if (threadIdx.x == 0) sh[0] = 0;
 }
 
 // ---------------------------------------------------------------------------
 // order[k] = index of the k-th largest sig (stable; ties by index)
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(NT) void k_sort_desc(const double* sig, int n, int* order) {
+  // NaN keys rank as -inf, ties by index: the ranks always form a permutation,
+  // so every order[] entry is written with a valid column index.
   for (int i = threadIdx.x; i < n; i += NT) {
-    double v = sig[i];
+    double v = key_desc(sig[i]);
     int rnk = 0;
     for (int j = 0; j < n; j++) {
-      double u = sig[j];
+      double u = key_desc(sig[j]);
       rnk += (u > v) || (u == v && j < i);
     }
     order[rnk] = i;
@@ -464,7 +477,7 @@ __global__ __launch_bounds__(NT) void k_house_qr(const T* A, int lda, int rows, 
         // tau = (beta - alpha) / beta ; v = x / (alpha - beta), v0 = 1
         tau[k] = (T(beta) - al) * (1.0 / beta);
         T d = al - T(beta);
-        alpha_s = conj_(d) * (1.0 / abs2_(d));  // 1/(alpha - beta)
+        alpha_s = inv_(d);  // 1/(alpha - beta)
         beta_s = beta;
       }
     }
@@ -586,6 +599,7 @@ __global__ __launch_bounds__(256) void k_svd_finish(const T* Hc, int ldh, int ro
   if (idx >= int64_t(n) * n) return;
   int k = int(idx / n), j = int(idx % n);
   int ok = order[k];
+  ok = ok < 0 ? 0 : (ok >= n ? n - 1 : ok);  // order is a permutation; the clamp only guards
   // VT[k][j] = conj(J[j][ok])
   T v = conj_(J[int64_t(ok) * ldj + j]);
   store_vt(VT + int64_t(k) * ldvt + j, v);

@@ -121,7 +121,7 @@ int make_plan(pc_ctx* ctx, pc_dtype dt, int64_t m, int64_t n, int precision, Pla
   p.tpc = ceil_div(ceil_div(m, p.tr), p.chunks);
   p.chunks3 = stream_chunks(m, p.tr3);
   p.tpc3 = ceil_div(ceil_div(m, p.tr3), p.chunks3);
-  p.nparts = p.mixed ? 2 * p.chunks : p.chunks;
+  p.nparts = p.chunks;
   return PC_OK;
 }
 
@@ -232,14 +232,18 @@ __global__ __launch_bounds__(1024) void k_null_build(int64_t m, int n, const int
   __shared__ double sh[32];
   // rank candidates by leverage (ties by index), keep the first d
   for (int c = threadIdx.x; c < ncand; c += 1024) {
-    double v = lev[c];
+    double v = key_asc(lev[c]);
     int r = 0;
-    for (int e = 0; e < ncand; e++) r += (lev[e] < v) || (lev[e] == v && e < c);
+    for (int e = 0; e < ncand; e++) {
+      double u = key_asc(lev[e]);
+      r += (u < v) || (u == v && e < c);
+    }
     if (r < d) pick[r] = c;
   }
   __syncthreads();
   for (int a = threadIdx.x; a < d; a += 1024) {
     int c = pick[a];
+    c = c < 0 ? 0 : (c >= ncand ? ncand - 1 : c);
     jsel[a] = int(ncand >= m ? c : (int64_t(c) * m) / ncand);
   }
   // Md (d x d) in work, row-major
@@ -308,14 +312,16 @@ __global__ __launch_bounds__(1024) void k_null_build(int64_t m, int n, const int
 }
 
 template <class T, class TO>
-__global__ __launch_bounds__(256) void k_null_fix(const int* nnull, int n, const int* jsel, const T* cnull, TO* U,
+__global__ __launch_bounds__(256) void k_null_fix(const int* nnull, int64_t m, int n, const int* jsel, const T* cnull, TO* U,
                                                   int64_t ldu) {
   int d = nnull[0];
   int idx = blockIdx.x * 256 + threadIdx.x;
   if (idx >= d * d) return;
   int a = idx / d, b = idx % d;
   int g = n - d;
-  TO* p = U + int64_t(jsel[a]) * ldu + g + b;
+  int64_t row = jsel[a];
+  if (row < 0 || row >= m || g + b >= n) return;
+  TO* p = U + row * ldu + g + b;
   add_elem(p, cnull[idx]);
 }
 
@@ -381,6 +387,8 @@ static int launch_streams(pc_ctx* ctx, const Plan& p, const void* A, int64_t lda
   int rc = 0;
   pc_dtype sdt = p.cplx ? (p.dt == PC_C64 ? PC_F32 : PC_F64) : p.dt;
   int64_t ldr = p.cplx ? 2 * lda : lda;
+  hipEvent_t ev0;
+  pc_prof_begin(ctx, &ev0);
   if (p.mixed) {
     if (pass == 1) rc = launch_gram_mx(st, (const float*)A, ldr, p.m, p.nr, center, mean, var, p.chunks, p.tpc, part);
     else rc = launch_zgram_mx(st, (const float*)A, ldr, p.m, p.nr, center, mean, var, b.Wh, b.wscale, p.chunks3,
@@ -395,6 +403,7 @@ static int launch_streams(pc_ctx* ctx, const Plan& p, const void* A, int64_t lda
   }
   if (rc != 0) return pc_fail(ctx, PC_ERR_SHAPE, "streaming kernel has no variant for n = %d", p.n);
   PC_HIP(ctx, hipGetLastError());
+  pc_prof_end(ctx, pass == 1 ? PCP_GRAM : PCP_ZGRAM, ev0);
   return PC_OK;
 }
 
@@ -511,6 +520,8 @@ static int core_t(pc_ctx* ctx, const Plan& p, const void* A, int64_t lda, int ce
   to_operand<T>(st, p, b.Wf, nc, b, true);
   {
     int rc;
+    hipEvent_t ev3;
+    pc_prof_begin(ctx, &ev3);
     pc_dtype sdt = p.cplx ? (p.dt == PC_C64 ? PC_F32 : PC_F64) : p.dt;
     int64_t ldr = p.cplx ? 2 * lda : lda;
     int64_t ldu = p.cplx ? 2 * out.ldu : out.ldu;
@@ -520,8 +531,9 @@ static int core_t(pc_ctx* ctx, const Plan& p, const void* A, int64_t lda, int ce
     else
       rc = launch_gemm_f64(st, sdt, A, ldr, p.m, p.nr, center, mean, var, b.Wpad, p.nr, p.chunks3, p.tpc3, out.U, ldu);
     if (rc != 0) return pc_fail(ctx, PC_ERR_SHAPE, "no gemm variant for n = %d", n);
+    pc_prof_end(ctx, PCP_GEMM, ev3);
   }
-  hipLaunchKernelGGL((k_null_fix<T, typename VtType<T, ST>::type>), dim3(nblk(nn)), dim3(256), 0, st, b.nnull, n,
+  hipLaunchKernelGGL((k_null_fix<T, typename VtType<T, ST>::type>), dim3(nblk(nn)), dim3(256), 0, st, b.nnull, p.m, n,
                      b.jsel, b.cnull, (typename VtType<T, ST>::type*)out.U, out.ldu);
   PC_HIP(ctx, hipGetLastError());
   // status check: one small D2H copy, synchronous (errors must surface here)
@@ -531,6 +543,7 @@ static int core_t(pc_ctx* ctx, const Plan& p, const void* A, int64_t lda, int ce
   if (s0 & 1) return pc_fail(ctx, PC_ERR_LINALG, "non-finite values in the input (Gram pivot is not finite)");
   if (s0 & 2) return pc_fail(ctx, PC_ERR_LINALG, "Jacobi SVD did not converge");
   if (s0 & 4) return pc_fail(ctx, PC_ERR_LINALG, "null-space completion failed (rows of high leverage)");
+  if (s0 & 8) return pc_fail(ctx, PC_ERR_LINALG, "non-finite singular values (overflow in the input scale?)");
   return PC_OK;
 }
 

@@ -213,3 +213,20 @@ def test_errors_are_raised_not_aborts():
         pl.math.tsqr_svd(np.zeros((3, 5)))       # m < n
     with pytest.raises(pl.LinAlgError):
         pl.math.tsqr_svd(np.full((100, 4), np.nan))
+
+
+@pytest.mark.parametrize("precision,dt", [("fp64", np.float64), ("mixed", np.float32)])
+def test_centered_full_width_rank_deficient(precision, dt):
+    # n = 256 exactly, remove_mean -> rank n-1: the structure of the 8M x 256
+    # benchmark at a small m (regression: Jacobi on a rank-deficient factor must
+    # stop, every singular value stays finite, U stays orthonormal)
+    X = workloads.lowrank(16384, 256, seed=21, decay=0.93, top=1e3, noise=1e-6 * np.sqrt(8e6 / 16384), dtype=dt)
+    U, S, V = pl.POD.run(X, remove_mean=True, precision=precision)
+    U, S, V = _np(U).astype(np.float64), _np(S).astype(np.float64), _np(V).astype(np.float64)
+    Y = X.astype(np.float64) - X.astype(np.float64).mean(axis=1, keepdims=True)
+    So = np.linalg.svd(Y, compute_uv=False)
+    e = metrics.sigma_errors(S, So)
+    assert np.all(np.isfinite(S))
+    assert e["maxrel_gated"] <= (1e-10 if precision == "fp64" else 1e-5), e
+    assert metrics.orthogonality(U) <= (1e-10 if precision == "fp64" else 1e-3)
+    assert metrics.recon_rmse(Y, U, S, V) <= (1e-12 if precision == "fp64" else 1e-5)
