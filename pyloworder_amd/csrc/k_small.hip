@@ -106,37 +106,58 @@ __global__ __launch_bounds__(NT) void k_chol_shift(const T* G, int ldg, int n, d
                                                    int* status) {
   __shared__ double sh[32];
   __shared__ double piv;
+  __shared__ int restart;
   double dmax = 0.0;
   for (int i = threadIdx.x; i < n; i += NT) dmax = fmax(dmax, real_(G[int64_t(i) * ldg + i]));
   dmax = block_max(dmax, sh);
+  if (!isfinite(dmax)) {
+    if (threadIdx.x == 0) status[0] = 1;
+    dmax = 1.0;
+  }
   double tau = shift_rel * dmax;
-  for (int64_t idx = threadIdx.x; idx < int64_t(n) * n; idx += NT) {
-    int i = int(idx / n), j = int(idx % n);
-    T v = i <= j ? G[int64_t(i) * ldg + j] : T(0.0);
-    if (i == j) v = v + T(tau);
-    R[int64_t(i) * ldr + j] = v;
-  }
-  __syncthreads();
-  for (int k = 0; k < n; k++) {
-    if (threadIdx.x == 0) {
-      double d = real_(R[int64_t(k) * ldr + k]);
-      if (!(d > 0.0) || !isfinite(d)) { status[0] = 1; d = 1.0; }
-      piv = sqrt(d);
+  if (!(tau > 0.0)) tau = 1.0;
+  // For G + tau I >= tau I every Cholesky pivot is >= tau.  A pivot below
+  // tau/2 means rounding made G indefinite at this scale (the Gram of
+  // mean-removed data is singular): restart with a larger shift.  The factor
+  // only preconditions the next pass, whose operand is inverted exactly, so the
+  // shift changes the preconditioner, never the result.
+  for (int attempt = 0; attempt < 20; attempt++) {
+    for (int64_t idx = threadIdx.x; idx < int64_t(n) * n; idx += NT) {
+      int i = int(idx / n), j = int(idx % n);
+      T v = i <= j ? G[int64_t(i) * ldg + j] : T(0.0);
+      if (i == j) v = v + T(tau);
+      R[int64_t(i) * ldr + j] = v;
     }
+    if (threadIdx.x == 0) restart = 0;
     __syncthreads();
-    double p = piv;
-    T* rk = R + int64_t(k) * ldr;
-    for (int j = k + 1 + threadIdx.x; j < n; j += NT) rk[j] = rk[j] * (1.0 / p);
-    if (threadIdx.x == 0) rk[k] = T(p);
-    __syncthreads();
-    int w = n - k - 1;
-    int64_t cnt = int64_t(w) * w;
-    for (int64_t idx = threadIdx.x; idx < cnt; idx += NT) {
-      int i = k + 1 + int(idx / w), j = k + 1 + int(idx % w);
-      if (i <= j) R[int64_t(i) * ldr + j] -= conj_(rk[i]) * rk[j];
+    for (int k = 0; k < n; k++) {
+      if (threadIdx.x == 0) {
+        double d = real_(R[int64_t(k) * ldr + k]);
+        if (!isfinite(d)) { status[0] = 1; d = tau; }
+        else if (d < 0.5 * tau) { restart = 1; d = tau; }
+        piv = sqrt(d);
+      }
+      __syncthreads();
+      if (restart) break;
+      double p = piv;
+      T* rk = R + int64_t(k) * ldr;
+      for (int j = k + 1 + threadIdx.x; j < n; j += NT) rk[j] = rk[j] * (1.0 / p);
+      if (threadIdx.x == 0) rk[k] = T(p);
+      __syncthreads();
+      int w = n - k - 1;
+      int64_t cnt = int64_t(w) * w;
+      for (int64_t idx = threadIdx.x; idx < cnt; idx += NT) {
+        int i = k + 1 + int(idx / w), j = k + 1 + int(idx % w);
+        if (i <= j) R[int64_t(i) * ldr + j] -= conj_(rk[i]) * rk[j];
+      }
+      __syncthreads();
     }
+    int again = restart;
     __syncthreads();
+    if (!again) return;
+    tau *= 8.0;
   }
+  if (threadIdx.x == 0) status[0] = 1;
 }
 
 // ---------------------------------------------------------------------------
@@ -193,7 +214,7 @@ __global__ __launch_bounds__(NT) void k_pchol(const T* G, int ldg, int n, double
   for (int i = threadIdx.x; i < n; i += NT) dmax = fmax(dmax, real_(A[int64_t(i) * n + i]));
   dmax = block_max(dmax, sh);
   if (!(dmax > 0.0) || !isfinite(dmax)) {
-    if (threadIdx.x == 0) { rank_out[0] = 0; if (!isfinite(dmax)) status[0] = 1; }
+    if (threadIdx.x == 0) { rank_out[0] = 0; if (!isfinite(dmax)) atomicOr(status, 16); }
     return;
   }
   double thr = tol * dmax;
@@ -311,104 +332,146 @@ __global__ __launch_bounds__(256) void k_gemm_small(int opA, int opB, int M, int
 }
 
 // ---------------------------------------------------------------------------
-// One-sided Jacobi SVD on the columns of a column-major matrix H (rows x cols,
-// ld = ldh) with accumulation of the unitary J (cols x cols, column-major,
-// ld = ldj): H J -> columns mutually orthogonal.  Round-robin parallel order;
-// a sweep with no rotation ends the iteration.  On exit sig[k] = ||H[:,k]||.
-// status[0] |= 2 when max_sweeps were used up.
+// Block one-sided Jacobi (multi-workgroup).  The columns of H (rows x cols,
+// column-major) and of the accumulated unitary J are cut into nb2 blocks of b
+// columns.  One launch of k_bjac_step processes one round-robin step: each
+// workgroup takes a pair of blocks, stages their 2b columns of H and J in LDS,
+// orthogonalises them there (one wave per column pair) and writes them back.
+// k_bjac_check closes a sweep: when no workgroup rotated, state[0] (done) is
+// set and every later launch returns at once.  No host synchronisation and no
+// inter-workgroup waiting is involved; the stream orders the launches.
+//   state: [0] done, [1 + s] rotated-in-sweep-s flags;  dstate[0] = negligible
+//   squared column norm (eps^2 rows ||H||_F^2).
 // ---------------------------------------------------------------------------
+constexpr int BJ_LDS = 131072;
+
 template <class T>
-__global__ __launch_bounds__(NT) void k_jacobi(T* H, int ldh, int rows, int cols, T* J, int ldj, double* sig,
-                                               int* status, int max_sweeps, double tol, int init_j) {
-  __shared__ int pos[1024];
+__global__ __launch_bounds__(NT) void k_bjac_init(const T* H, int ldh, int rows, int cols, T* J, int ldj, int* state,
+                                                  double* dstate, int nflags) {
   __shared__ double sh[32];
-  __shared__ int rot_flag;
-  int ncp = cols + (cols & 1);          // even number of players (a dummy if odd)
-  int npairs = ncp / 2;
-  if (init_j) {
-    for (int64_t idx = threadIdx.x; idx < int64_t(cols) * cols; idx += NT) {
-      int i = int(idx / cols), j = int(idx % cols);
-      J[int64_t(j) * ldj + i] = T(i == j ? 1.0 : 0.0);
-    }
+  for (int i = threadIdx.x; i < nflags; i += NT) state[i] = 0;
+  for (int64_t idx = threadIdx.x; idx < int64_t(cols) * cols; idx += NT) {
+    int i = int(idx / cols), j = int(idx % cols);
+    J[int64_t(j) * ldj + i] = T(i == j ? 1.0 : 0.0);
   }
-  for (int i = threadIdx.x; i < ncp; i += NT) pos[i] = i;
-  // threads per pair: power of two, <= 64
-  int g = 64;
-  while (g > 1 && g * npairs > NT) g >>= 1;
-  int gid = threadIdx.x / g, gl = threadIdx.x % g;
-  int groups = NT / g;
-  // A column whose squared norm is below (eps sqrt(rows))^2 ||H||_F^2 is zero to
-  // working precision: it takes part in no rotation.  Without this test the
-  // null columns of a rank-deficient H keep rotating among themselves, shrink
-  // towards underflow and the iteration never stops.
-  double fro2 = 0.0;
+  double f = 0.0;
   for (int64_t idx = threadIdx.x; idx < int64_t(cols) * rows; idx += NT) {
     int k = int(idx / rows), i = int(idx % rows);
-    fro2 += abs2_(H[int64_t(k) * ldh + i]);
+    f += abs2_(H[int64_t(k) * ldh + i]);
   }
-  fro2 = block_sum(fro2, sh);
-  const double negl = 2.220446049250313e-16 * 2.220446049250313e-16 * double(rows) * fro2;
+  f = block_sum(f, sh);
+  if (threadIdx.x == 0) dstate[0] = 2.220446049250313e-16 * 2.220446049250313e-16 * double(rows) * f;
+}
+
+// player sitting at slot k of step s in the circle schedule of N players
+PC_DEV int rr_player(int k, int s, int N) { return k == 0 ? 0 : 1 + ((k - 1 + s) % (N - 1)); }
+
+template <class T>
+__global__ __launch_bounds__(NT) void k_bjac_step(T* H, int ldh, int rows, int cols, T* J, int ldj, int b, int nb2,
+                                                  int step, int sweep, int* state, const double* dstate, double tol,
+                                                  int max_inner) {
+  __shared__ __align__(16) char smem[BJ_LDS];
+  __shared__ int rot_any, rot_sweep;
+  if (state[0]) return;
+  const int c2 = 2 * b;
+  T* Hs = reinterpret_cast<T*>(smem);
+  T* Js = Hs + int64_t(c2) * rows;
+  int p = blockIdx.x;
+  int bi = rr_player(p, step, nb2), bj = rr_player(nb2 - 1 - p, step, nb2);
+  double negl = dstate[0];
+  // stage the 2b columns (columns beyond `cols` are empty)
+  for (int64_t idx = threadIdx.x; idx < int64_t(c2) * rows; idx += NT) {
+    int lc = int(idx / rows), i = int(idx % rows);
+    int gc = lc < b ? bi * b + lc : bj * b + (lc - b);
+    Hs[idx] = gc < cols ? H[int64_t(gc) * ldh + i] : T(0.0);
+  }
+  for (int64_t idx = threadIdx.x; idx < int64_t(c2) * cols; idx += NT) {
+    int lc = int(idx / cols), i = int(idx % cols);
+    int gc = lc < b ? bi * b + lc : bj * b + (lc - b);
+    Js[idx] = gc < cols ? J[int64_t(gc) * ldj + i] : T(0.0);
+  }
+  if (threadIdx.x == 0) rot_any = 0;
   __syncthreads();
-  int sweep = 0;
-  for (; sweep < max_sweeps; sweep++) {
-    if (threadIdx.x == 0) rot_flag = 0;
+  int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  int last_rot = 0;
+  for (int inner = 0; inner < max_inner; inner++) {
+    if (threadIdx.x == 0) rot_sweep = 0;
     __syncthreads();
-    for (int step = 0; step < ncp - 1; step++) {
-      for (int pr = gid; pr < npairs; pr += groups) {
-        int p = pos[pr], q = pos[ncp - 1 - pr];
-        if (p >= cols || q >= cols) continue;
-        if (p > q) { int t = p; p = q; q = t; }
-        T* hp = H + int64_t(p) * ldh;
-        T* hq = H + int64_t(q) * ldh;
-        double a = 0, b = 0;
+    for (int s = 0; s < c2 - 1; s++) {
+      for (int pr = wv; pr < b; pr += NT / 64) {
+        int u = rr_player(pr, s, c2), v = rr_player(c2 - 1 - pr, s, c2);
+        if (u > v) { int t = u; u = v; v = t; }
+        int gu = u < b ? bi * b + u : bj * b + (u - b);
+        int gv = v < b ? bi * b + v : bj * b + (v - b);
+        if (gu >= cols || gv >= cols) continue;
+        T* hu = Hs + int64_t(u) * rows;
+        T* hv = Hs + int64_t(v) * rows;
+        double a = 0, bb = 0;
         T c = T(0.0);
-        for (int i = gl; i < rows; i += g) {
-          T x = hp[i], y = hq[i];
+        for (int i = lane; i < rows; i += 64) {
+          T x = hu[i], y = hv[i];
           a += abs2_(x);
-          b += abs2_(y);
+          bb += abs2_(y);
           c += dotc_(x, y);
         }
-        a = group_sum(a, g);
-        b = group_sum(b, g);
-        c = group_sum_t(c, g);
+        a = wave_sum(a);
+        bb = wave_sum(bb);
+        c = group_sum_t(c, 64);
         double gm = sqrt(abs2_(c));
-        if (!(a > negl) || !(b > negl)) continue;
-        if (!(gm > tol * sqrt(a) * sqrt(b))) continue;
-        // phase e = c / |c| (by division: 1/gm could overflow); real rotation on (h_p, conj(e) h_q)
+        if (!(a > negl) || !(bb > negl)) continue;
+        if (!(gm > tol * sqrt(a) * sqrt(bb))) continue;
         T e = phase_(c, gm);
-        double zeta = (b - a) / (2.0 * gm);
+        double zeta = (bb - a) / (2.0 * gm);
         double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
         double cs = 1.0 / sqrt(1.0 + t * t), sn = cs * t;
         T ec = conj_(e);
-        for (int i = gl; i < rows; i += g) {
-          T x = hp[i], y = hq[i] * ec;
-          hp[i] = x * cs - y * sn;
-          hq[i] = x * sn + y * cs;
+        for (int i = lane; i < rows; i += 64) {
+          T x = hu[i], y = hv[i] * ec;
+          hu[i] = x * cs - y * sn;
+          hv[i] = x * sn + y * cs;
         }
-        T* jp = J + int64_t(p) * ldj;
-        T* jq = J + int64_t(q) * ldj;
-        for (int i = gl; i < cols; i += g) {
-          T x = jp[i], y = jq[i] * ec;
-          jp[i] = x * cs - y * sn;
-          jq[i] = x * sn + y * cs;
+        T* ju = Js + int64_t(u) * cols;
+        T* jv = Js + int64_t(v) * cols;
+        for (int i = lane; i < cols; i += 64) {
+          T x = ju[i], y = jv[i] * ec;
+          ju[i] = x * cs - y * sn;
+          jv[i] = x * sn + y * cs;
         }
-        if (gl == 0) rot_flag = 1;
-      }
-      __syncthreads();
-      // round robin: keep pos[0], rotate the rest by one
-      if (threadIdx.x == 0) {
-        int last = pos[ncp - 1];
-        for (int i = ncp - 1; i > 1; i--) pos[i] = pos[i - 1];
-        pos[1] = last;
+        if (lane == 0) { rot_sweep = 1; rot_any = 1; }
       }
       __syncthreads();
     }
-    int any = rot_flag;
+    int any = rot_sweep;
     __syncthreads();
+    last_rot = any;
     if (!any) break;
   }
-  if (sweep >= max_sweeps && threadIdx.x == 0) atomicOr(status, 2);
-  // column norms
+  // write back
+  for (int64_t idx = threadIdx.x; idx < int64_t(c2) * rows; idx += NT) {
+    int lc = int(idx / rows), i = int(idx % rows);
+    int gc = lc < b ? bi * b + lc : bj * b + (lc - b);
+    if (gc < cols) H[int64_t(gc) * ldh + i] = Hs[idx];
+  }
+  for (int64_t idx = threadIdx.x; idx < int64_t(c2) * cols; idx += NT) {
+    int lc = int(idx / cols), i = int(idx % cols);
+    int gc = lc < b ? bi * b + lc : bj * b + (lc - b);
+    if (gc < cols) J[int64_t(gc) * ldj + i] = Js[idx];
+  }
+  // a single block pair reports whether its last inner sweep still rotated;
+  // several pairs report any rotation (the sweep is then not yet converged)
+  int flag = nb2 == 2 ? last_rot : rot_any;
+  if (threadIdx.x == 0 && flag) atomicOr(&state[1 + sweep], 1);
+}
+
+// close sweep `sweep`: no workgroup reported a rotation -> done
+__global__ void k_bjac_check(int* state, int sweep) {
+  if (threadIdx.x == 0 && !state[0] && state[1 + sweep] == 0) state[0] = 1;
+}
+
+// column norms, non-finite and non-convergence flags
+template <class T>
+__global__ __launch_bounds__(NT) void k_bjac_norms(const T* H, int ldh, int rows, int cols, double* sig,
+                                                   const int* state, int* status) {
   int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   for (int k = wv; k < cols; k += NT / 64) {
     double s = 0;
@@ -420,6 +483,15 @@ __global__ __launch_bounds__(NT) void k_jacobi(T* H, int ldh, int rows, int cols
       if (!isfinite(s)) atomicOr(status, 8);
     }
   }
+  if (threadIdx.x == 0 && !state[0]) atomicOr(status, 2);
+}
+
+// block width of the block Jacobi for an n x n problem with element size esz:
+// the 2b staged columns of H and J must fit BJ_LDS; one pair when they all fit
+inline int bjac_width(int n, int esz) {
+  int b = (n + 1) / 2;
+  while (int64_t(2) * b * (2 * n) * esz > BJ_LDS) b = (b + 1) / 2;
+  return b < 1 ? 1 : b;
 }
 
 // ---------------------------------------------------------------------------

@@ -29,6 +29,34 @@ constexpr double kNullMx = 1e-6;
 
 inline unsigned nblk(int64_t count, int per = 256) { return unsigned(ceil_div(count, per)); }
 
+constexpr int kBjacMaxSweeps = 48;
+
+// One-sided block Jacobi on H (rows x cols, column-major) with J accumulated:
+// one launch per round-robin step of a sweep, a check launch per sweep.  After
+// convergence every remaining launch exits immediately.
+template <class T>
+void launch_block_jacobi(pc_ctx* ctx, T* H, int ldh, int rows, int cols, T* J, int ldj, double* sig, int* state,
+                         double* dstate, int* status) {
+  hipStream_t st = ctx->stream;
+  int b = bjac_width(std::max(rows, cols), int(sizeof(T)));
+  int nb2 = int(ceil_div(cols, b));
+  nb2 += nb2 & 1;
+  if (nb2 < 2) nb2 = 2;
+  hipLaunchKernelGGL((k_bjac_init<T>), dim3(1), dim3(1024), 0, st, H, ldh, rows, cols, J, ldj, state, dstate,
+                     kBjacMaxSweeps + 1);
+  int max_inner = nb2 == 2 ? 60 : 3;
+  // relative orthogonality test |h_p^H h_q| <= tol ||h_p|| ||h_q||: the dot
+  // product itself carries ~eps sqrt(rows) rounding, so tol grows with rows
+  double tol = 2.220446049250313e-16 * std::max(rows, 16);
+  for (int s = 0; s < kBjacMaxSweeps; s++) {
+    for (int step = 0; step < nb2 - 1; step++)
+      hipLaunchKernelGGL((k_bjac_step<T>), dim3(nb2 / 2), dim3(1024), 0, st, H, ldh, rows, cols, J, ldj, b, nb2, step,
+                         s, state, dstate, tol, max_inner);
+    hipLaunchKernelGGL(k_bjac_check, dim3(1), dim3(64), 0, st, state, s);
+  }
+  hipLaunchKernelGGL((k_bjac_norms<T>), dim3(1), dim3(1024), 0, st, H, ldh, rows, cols, sig, state, status);
+}
+
 // output element type for (compute type T, real storage type ST)
 template <class T, class ST> struct VtType;
 template <> struct VtType<double, double> { typedef double type; };
@@ -66,6 +94,8 @@ template <class T> struct SmallBufs {
   T *G1, *R1, *W1, *R1h, *G2, *Awk, *R2, *Hc, *J, *UM, *R2i, *tmp, *Wf;
   double* sig;
   int *perm, *order, *rank, *nnull, *status;
+  int* jstate;       // block Jacobi: done flag + per-sweep rotation flags
+  double* jdstate;   // block Jacobi: negligible squared column norm
   double* Wpad;  // NB x NB real (embedded) operand for the FP64 streaming kernels
   // null completion
   double* lev;
@@ -139,6 +169,8 @@ int64_t carve(WsCursor& c, const Plan& p, SmallBufs<T>& b, double** mean, double
   b.sig = c.take<double>(n);
   b.perm = c.take<int>(n); b.order = c.take<int>(n); b.rank = c.take<int>(4); b.nnull = c.take<int>(4);
   b.status = c.take<int>(4);
+  b.jstate = c.take<int>(kBjacMaxSweeps + 8);
+  b.jdstate = c.take<double>(4);
   b.Wpad = c.take<double>(int64_t(p.NB) * p.NB);
   int ncand = int(std::min<int64_t>(p.m, 4 * int64_t(n) + 16));
   b.lev = c.take<double>(ncand);
@@ -493,8 +525,7 @@ static int core_t(pc_ctx* ctx, const Plan& p, const void* A, int64_t lda, int ce
   hipLaunchKernelGGL((k_trinv_upper<T>), dim3(1), dim3(1024), 0, st, b.R2, n, n, b.R2i, n);
   int nc = n;
   if (mode == CORE_SVD) {
-    hipLaunchKernelGGL((k_jacobi<T>), dim3(1), dim3(1024), 0, st, b.Hc, n, n, n, b.J, n, b.sig, b.status, 60,
-                       1e-15, 1);
+    launch_block_jacobi<T>(ctx, b.Hc, n, n, n, b.J, n, b.sig, b.jstate, b.jdstate, b.status);
     hipLaunchKernelGGL(k_sort_desc, dim3(1), dim3(1024), 0, st, b.sig, n, b.order);
     hipLaunchKernelGGL((k_svd_finish<T, ST, typename VtType<T, ST>::type>), dim3(nblk(nn)), dim3(256), 0, st, b.Hc,
                        n, n, n, b.sig, b.order, b.J, n, p.mixed ? kNullMx : kNullF64, (const int*)b.rank,
@@ -540,7 +571,8 @@ static int core_t(pc_ctx* ctx, const Plan& p, const void* A, int64_t lda, int ce
   PC_HIP(ctx, hipMemcpyAsync(ctx->hflag, b.status, 16, hipMemcpyDeviceToHost, st));
   PC_HIP(ctx, hipStreamSynchronize(st));
   int s0 = ctx->hflag[0];
-  if (s0 & 1) return pc_fail(ctx, PC_ERR_LINALG, "non-finite values in the input (Gram pivot is not finite)");
+  if (s0 & 1) return pc_fail(ctx, PC_ERR_LINALG, "non-finite values in the input (first Gram pass is not finite)");
+  if (s0 & 16) return pc_fail(ctx, PC_ERR_LINALG, "second Gram pass is not finite");
   if (s0 & 2) return pc_fail(ctx, PC_ERR_LINALG, "Jacobi SVD did not converge");
   if (s0 & 4) return pc_fail(ctx, PC_ERR_LINALG, "null-space completion failed (rows of high leverage)");
   if (s0 & 8) return pc_fail(ctx, PC_ERR_LINALG, "non-finite singular values (overflow in the input scale?)");

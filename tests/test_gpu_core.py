@@ -228,5 +228,9 @@ def test_centered_full_width_rank_deficient(precision, dt):
     e = metrics.sigma_errors(S, So)
     assert np.all(np.isfinite(S))
     assert e["maxrel_gated"] <= (1e-10 if precision == "fp64" else 1e-5), e
-    assert metrics.orthogonality(U) <= (1e-10 if precision == "fp64" else 1e-3)
+    # FP32 arithmetic resolves a mode to ~u32 s_1 / s_k: the noise-floor columns
+    # (s_k ~ 3e-6 s_1) are orthogonal to ~1e-2 only, the gated ones to ~1e-5
+    k = e["n_gated"]
+    assert metrics.orthogonality(U, k) <= (1e-10 if precision == "fp64" else 1e-4)
+    assert metrics.orthogonality(U) <= (1e-10 if precision == "fp64" else 5e-2)
     assert metrics.recon_rmse(Y, U, S, V) <= (1e-12 if precision == "fp64" else 1e-5)
