@@ -1,0 +1,578 @@
+// Blocked single-workgroup FP64 factorizations for real n x n matrices,
+// n <= 256 (the small dense stage of the POD core): shifted Cholesky,
+// pivoted Cholesky, triangular inverse and Householder QR with explicit Q.
+// Matrices live in global memory (L2-resident at these sizes); 32-column
+// panels are staged in LDS and every rank-32 update runs on
+// v_mfma_f64_16x16x4f64.  One workgroup of 1024 threads (16 waves).
+//
+// f64 MFMA 16x16x4 lane layout (wave64): lane l holds A[l % 16][l / 16] and
+// B[l / 16][l % 16]; accumulator register r holds D[4 r + l / 16][l % 16].
+// Included by solve.hip after k_small.hip (uses NT, block_max, mfma64s, d4s).
+
+namespace dense {
+
+constexpr int PW = 32;           // panel width
+constexpr int PLD = 257;         // LDS leading dimension of panels (odd: no bank conflicts)
+constexpr int NMAX = 256;
+
+// (I, J), I <= J, of the t-th upper tile of an nt x nt tile grid
+PC_DEV void upper_tile(int t, int nt, int& I, int& J) {
+  int i = 0;
+  while (t >= nt - i) { t -= nt - i; i++; }
+  I = i;
+  J = i + t;
+}
+
+}  // namespace dense
+
+// ---------------------------------------------------------------------------
+// Shifted Cholesky, real: R upper with R^T R = G + tau I, tau = shift_rel *
+// max G_ii.  Right-looking with 32-wide panels; a pivot below tau/2 (rounding
+// made G indefinite at that scale) restarts with tau *= 8.
+// status[0] = 1 on a non-finite Gram or when 20 restarts did not suffice.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(NT) void k_chol_fast(const double* __restrict__ G, int ldg, int n, double shift_rel,
+                                                  double* __restrict__ R, int ldr, int* status) {
+  using namespace dense;
+  __shared__ double D[PW * (PW + 1)];
+  __shared__ double P[PW * PLD];
+  __shared__ double sh[32];
+  __shared__ int restart;
+  const int tid = threadIdx.x, w = tid >> 6, l = tid & 63, lr = l & 15, lk = l >> 4;
+  double dmax = 0.0;
+  for (int i = tid; i < n; i += NT) dmax = fmax(dmax, G[int64_t(i) * ldg + i]);
+  dmax = block_max(dmax, sh);
+  if (!isfinite(dmax)) {
+    if (tid == 0) status[0] = 1;
+    dmax = 1.0;
+  }
+  double tau = shift_rel * dmax;
+  if (!(tau > 0.0)) tau = 1.0;
+  for (int attempt = 0; attempt < 20; attempt++) {
+    for (int i = w; i < n; i += NT / 64)
+      for (int j = l; j < n; j += 64) {
+        double v = i <= j ? G[int64_t(i) * ldg + j] : 0.0;
+        if (i == j) v += tau;
+        R[int64_t(i) * ldr + j] = v;
+      }
+    if (tid == 0) restart = 0;
+    __syncthreads();
+    for (int k0 = 0; k0 < n; k0 += PW) {
+      const int bs = min(PW, n - k0);
+      for (int idx = tid; idx < PW * PW; idx += NT) {
+        int i = idx >> 5, j = idx & 31;
+        D[i * (PW + 1) + j] = (i < bs && j < bs && i <= j) ? R[int64_t(k0 + i) * ldr + k0 + j] : 0.0;
+      }
+      __syncthreads();
+      for (int j = 0; j < bs; j++) {
+        if (tid == 0) {
+          double d = D[j * (PW + 1) + j];
+          if (!isfinite(d)) { status[0] = 1; d = tau; }
+          else if (d < 0.5 * tau) { restart = 1; d = tau; }
+          D[j * (PW + 1) + j] = sqrt(d);
+        }
+        __syncthreads();
+        if (restart) break;
+        const double r = D[j * (PW + 1) + j];
+        if (tid > j && tid < bs) D[j * (PW + 1) + tid] /= r;
+        __syncthreads();
+        {
+          int i = tid >> 5, c = tid & 31;
+          if (i > j && i <= c && c < bs) D[i * (PW + 1) + c] -= D[j * (PW + 1) + i] * D[j * (PW + 1) + c];
+        }
+        __syncthreads();
+      }
+      if (restart) break;
+      for (int idx = tid; idx < PW * PW; idx += NT) {
+        int i = idx >> 5, j = idx & 31;
+        if (i < bs && j < bs && i <= j) R[int64_t(k0 + i) * ldr + k0 + j] = D[i * (PW + 1) + j];
+      }
+      // panel: rows k0..k0+bs-1 of R right of the block, D^T X = A
+      const int c1 = k0 + bs, nt = n - c1;
+      // (each thread owns one column of the panel; its column of P is its scratch)
+      for (int c = tid; c < nt; c += NT) {
+        for (int i = 0; i < PW; i++) {
+          double s = 0.0;
+          if (i < bs) {
+            s = R[int64_t(k0 + i) * ldr + c1 + c];
+            for (int t = 0; t < i; t++) s -= D[t * (PW + 1) + i] * P[t * PLD + c];
+            s /= D[i * (PW + 1) + i];
+            R[int64_t(k0 + i) * ldr + c1 + c] = s;
+          }
+          P[i * PLD + c] = s;
+        }
+      }
+      __syncthreads();
+      // trailing update (upper tiles): A[c1:, c1:] -= P^T P
+      const int ntl = (nt + 15) >> 4;
+      for (int t = w; t < ntl * (ntl + 1) / 2; t += NT / 64) {
+        int I, J;
+        upper_tile(t, ntl, I, J);
+        const int col = 16 * J + lr;
+        d4s acc;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          int row = 16 * I + 4 * r + lk;
+          acc[r] = (row < nt && col < nt) ? -R[int64_t(c1 + row) * ldr + c1 + col] : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < PW; k += 4) acc = mfma64s(P[(k + lk) * PLD + 16 * I + lr], P[(k + lk) * PLD + col], acc);
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          int row = 16 * I + 4 * r + lk;
+          if (row < nt && col < nt) R[int64_t(c1 + row) * ldr + c1 + col] = -acc[r];
+        }
+      }
+      __syncthreads();
+    }
+    int again = restart;
+    __syncthreads();
+    if (!again) {
+      // strictly lower part
+      for (int i = w; i < n; i += NT / 64)
+        for (int j = l; j < i; j += 64) R[int64_t(i) * ldr + j] = 0.0;
+      return;
+    }
+    tau *= 8.0;
+  }
+  if (tid == 0) status[0] = 1;
+}
+
+// ---------------------------------------------------------------------------
+// Inverse of a real upper-triangular R (n <= 256): W = R^-1.
+//   1. diagonal 32x32 blocks inverted by back substitution, one thread per
+//      column;
+//   2. block diagonals d = 1, 2, ...:  S_ij = sum_{k=i+1..j} R_ik W_kj and
+//      W_ij = -W_ii S_ij, both on MFMA.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(NT) void k_trinv_fast(const double* __restrict__ R, int ldr, int n, double* __restrict__ W,
+                                                   int ldw) {
+  using namespace dense;
+  __shared__ double S[(NMAX / PW - 1) * PW * (PW + 1)];
+  const int tid = threadIdx.x, w = tid >> 6, l = tid & 63, lr = l & 15, lk = l >> 4;
+  const int nbk = (n + PW - 1) / PW;
+  for (int i = w; i < n; i += NT / 64)
+    for (int j = l; j < n; j += 64)
+      if (i > j || (i / PW) != (j / PW)) W[int64_t(i) * ldw + j] = 0.0;
+  if (tid < nbk * PW) {
+    const int b = tid / PW, c = tid % PW, o = b * PW, bs = min(PW, n - o);
+    if (c < bs) {
+      double x[PW];
+#pragma unroll
+      for (int i = PW - 1; i >= 0; i--) {
+        double s = 0.0;
+        if (i <= c) {
+          s = i == c ? 1.0 : 0.0;
+#pragma unroll
+          for (int t = i + 1; t < PW; t++)
+            if (t <= c) s -= R[int64_t(o + i) * ldr + o + t] * x[t];
+          s /= R[int64_t(o + i) * ldr + o + i];
+          W[int64_t(o + i) * ldw + o + c] = s;
+        }
+        x[i] = s;
+      }
+    }
+  }
+  __syncthreads();
+  for (int d = 1; d < nbk; d++) {
+    const int nbl = nbk - d;
+    for (int task = w; task < nbl * 4; task += NT / 64) {
+      const int bi = task >> 2, tI = (task >> 1) & 1, tJ = task & 1, bj = bi + d;
+      const int row = PW * bi + 16 * tI + lr, col = PW * bj + 16 * tJ + lr;
+      const int kbeg = PW * (bi + 1), kend = min(PW * (bj + 1), n);
+      d4s acc = {0.0, 0.0, 0.0, 0.0};
+      for (int k0 = kbeg; k0 < kend; k0 += 4) {
+        int k = k0 + lk;
+        double a = (k < kend && row < n) ? R[int64_t(row) * ldr + k] : 0.0;
+        double bb = (k < kend && col < n) ? W[int64_t(k) * ldw + col] : 0.0;
+        acc = mfma64s(a, bb, acc);
+      }
+#pragma unroll
+      for (int r = 0; r < 4; r++) S[bi * PW * (PW + 1) + (16 * tI + 4 * r + lk) * (PW + 1) + 16 * tJ + lr] = acc[r];
+    }
+    __syncthreads();
+    // R_ii W_ij = -S_ij by back substitution (not a product with the computed
+    // W_ii: a graded diagonal block makes the product lose accuracy); one thread
+    // per column, in place in its own column of S.  Block bi is full (bi < nbk-1).
+    if (tid < nbl * PW) {
+      const int bi = tid / PW, c = tid % PW, bj = bi + d, o = PW * bi, col = PW * bj + c;
+      double* sc = S + bi * PW * (PW + 1) + c;
+      if (col < n) {
+        for (int i = PW - 1; i >= 0; i--) {
+          double s = -sc[i * (PW + 1)];
+          for (int t = i + 1; t < PW; t++) s -= R[int64_t(o + i) * ldr + o + t] * sc[t * (PW + 1)];
+          s /= R[int64_t(o + i) * ldr + o + i];
+          sc[i * (PW + 1)] = s;
+          W[int64_t(o + i) * ldw + col] = s;
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Pivoted Cholesky with rank detection, real (n <= 256): P^T G P = R^T R.
+// The factorisation runs in physical (unpermuted) column indices, so nothing
+// is swapped: row j of R picks the unpivoted index p with the largest Schur
+// diagonal, R_phys[j][q] = (A[p][q] - sum_{t in panel, t<j} R_phys[t][p]
+// R_phys[t][q]) / sqrt(d_p); earlier panels are folded into A by MFMA rank-32
+// updates.  At the end R[t][c] = R_phys[t][perm[c]].  Stops when the largest
+// remaining diagonal is <= tol * max(diag G) (rank); rows >= rank are zero and
+// perm[rank..] lists the unpivoted indices in increasing order.
+// A: n x n workspace (upper triangle used, physical indices).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(NT) void k_pchol_fast(const double* __restrict__ G, int ldg, int n, double tol,
+                                                   double* __restrict__ A, double* __restrict__ R, int ldr,
+                                                   int* __restrict__ perm, int* __restrict__ rank_out,
+                                                   int* __restrict__ status) {
+  using namespace dense;
+  __shared__ double P[PW * PLD];
+  __shared__ double dg[NMAX];
+  __shared__ double wv_best[NT / 64];
+  __shared__ int wv_idx[NT / 64];
+  __shared__ int pv_idx, stop_s;
+  __shared__ double pv_val;
+  __shared__ double sh[32];
+  const int tid = threadIdx.x, w = tid >> 6, l = tid & 63, lr = l & 15, lk = l >> 4;
+  for (int i = w; i < n; i += NT / 64)
+    for (int j = l; j < n; j += 64) A[int64_t(i) * n + j] = i <= j ? G[int64_t(i) * ldg + j] : 0.0;
+  for (int q = tid; q < n; q += NT) dg[q] = G[int64_t(q) * ldg + q];
+  double dmax = 0.0;
+  for (int q = tid; q < n; q += NT) dmax = fmax(dmax, G[int64_t(q) * ldg + q]);
+  dmax = block_max(dmax, sh);
+  if (!(dmax > 0.0) || !isfinite(dmax)) {
+    for (int i = w; i < n; i += NT / 64)
+      for (int j = l; j < n; j += 64) R[int64_t(i) * ldr + j] = 0.0;
+    for (int q = tid; q < n; q += NT) perm[q] = q;
+    if (tid == 0) { rank_out[0] = 0; if (!isfinite(dmax)) atomicOr(status, 16); }
+    return;
+  }
+  const double thr = tol * dmax;
+  if (tid == 0) stop_s = 0;
+  int rank = n;
+  __syncthreads();
+  for (int k0 = 0; k0 < n; k0 += PW) {
+    const int bs = min(PW, n - k0);
+    int jj = 0;
+    for (; jj < bs; jj++) {
+      const int j = k0 + jj;
+      // argmax of the Schur diagonal over unpivoted indices (ties: lower index)
+      double best = -1.0;
+      int bi = n;
+      for (int q = tid; q < n; q += NT) {
+        double d = dg[q];
+        if (d > best) { best = d; bi = q; }
+      }
+      for (int o = 32; o > 0; o >>= 1) {
+        double ob = __shfl_xor(best, o);
+        int oi = __shfl_xor(bi, o);
+        if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+      }
+      if (l == 0) { wv_best[w] = best; wv_idx[w] = bi; }
+      __syncthreads();
+      if (tid == 0) {
+        double b = -1.0;
+        int ii = n;
+        for (int q = 0; q < NT / 64; q++)
+          if (wv_best[q] > b || (wv_best[q] == b && wv_idx[q] < ii)) { b = wv_best[q]; ii = wv_idx[q]; }
+        pv_idx = ii;
+        pv_val = b;
+        if (!(b > thr) || ii >= n) stop_s = 1;
+        else perm[j] = ii;
+      }
+      __syncthreads();
+      if (stop_s) break;
+      const int p = pv_idx;
+      const double r = sqrt(pv_val);
+      for (int q = tid; q < n; q += NT) {
+        double v;
+        if (q == p) v = r;
+        else if (dg[q] < 0.0) v = 0.0;  // already pivoted
+        else {
+          double s = p <= q ? A[int64_t(p) * n + q] : A[int64_t(q) * n + p];
+          for (int t = 0; t < jj; t++) s -= P[t * PLD + p] * P[t * PLD + q];
+          v = s / r;
+        }
+        P[jj * PLD + q] = v;
+      }
+      __syncthreads();
+      for (int q = tid; q < n; q += NT) {
+        if (q == p) dg[q] = -2.0;
+        else if (dg[q] >= 0.0) dg[q] = fmax(dg[q] - P[jj * PLD + q] * P[jj * PLD + q], 0.0);
+      }
+      __syncthreads();
+    }
+    if (jj < bs) rank = k0 + jj;
+    // rows of this panel (zero beyond the rows computed)
+    for (int t = w; t < PW; t += NT / 64)
+      for (int q = l; q < n; q += 64) {
+        if (t >= jj) P[t * PLD + q] = 0.0;
+        if (t < bs) R[int64_t(k0 + t) * ldr + q] = t < jj ? P[t * PLD + q] : 0.0;
+      }
+    __syncthreads();
+    if (jj < bs) break;
+    // fold this panel into A (upper tiles, physical indices)
+    const int ntl = (n + 15) >> 4;
+    for (int t = w; t < ntl * (ntl + 1) / 2; t += NT / 64) {
+      int I, J;
+      upper_tile(t, ntl, I, J);
+      const int col = 16 * J + lr;
+      d4s acc;
+#pragma unroll
+      for (int r4 = 0; r4 < 4; r4++) {
+        int row = 16 * I + 4 * r4 + lk;
+        acc[r4] = (row < n && col < n) ? -A[int64_t(row) * n + col] : 0.0;
+      }
+#pragma unroll
+      for (int k = 0; k < PW; k += 4) acc = mfma64s(P[(k + lk) * PLD + 16 * I + lr], P[(k + lk) * PLD + col], acc);
+#pragma unroll
+      for (int r4 = 0; r4 < 4; r4++) {
+        int row = 16 * I + 4 * r4 + lk;
+        if (row < n && col < n) A[int64_t(row) * n + col] = -acc[r4];
+      }
+    }
+    __syncthreads();
+  }
+  // rows at and beyond the rank are zero; perm completes with the unpivoted indices
+  for (int t = rank + w; t < n; t += NT / 64)
+    for (int q = l; q < n; q += 64) R[int64_t(t) * ldr + q] = 0.0;
+  if (tid == 0) {
+    int k = rank;
+    for (int q = 0; q < n; q++)
+      if (dg[q] >= 0.0 && k < n) perm[k++] = q;
+    rank_out[0] = rank;
+  }
+  __syncthreads();
+  // R[t][c] = R_phys[t][perm[c]] for c >= t, in chunks of 32 rows through LDS
+  for (int t0 = 0; t0 < rank; t0 += PW) {
+    for (int t = w; t < PW; t += NT / 64)
+      for (int q = l; q < n; q += 64) P[t * PLD + q] = (t0 + t < rank) ? R[int64_t(t0 + t) * ldr + q] : 0.0;
+    __syncthreads();
+    for (int t = w; t < PW && t0 + t < rank; t += NT / 64)
+      for (int c = l; c < n; c += 64) {
+        int pc = perm[c];
+        pc = pc < 0 ? 0 : (pc >= n ? n - 1 : pc);
+        R[int64_t(t0 + t) * ldr + c] = c >= t0 + t ? P[t * PLD + pc] : 0.0;
+      }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Householder QR of a real n x n matrix (n <= 256), A = Q R, Q explicit.
+// LAPACK conventions (beta = -sign(alpha) ||x||, tau = (beta - alpha) / beta).
+// Panels of 32 columns are factored in LDS; the block reflector
+// I - V T V^T (compact WY, T upper) updates the trailing columns with MFMA
+// (X = V^T A_t, X <- T^T X, A_t -= V X); Q is accumulated backwards the
+// same way.  work: n*n (the factored matrix) + n (tau) + 32*32*ceil(n/32) (T).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(NT) void k_house_qr_fast(const double* __restrict__ A, int lda, int n,
+                                                      double* __restrict__ Q, int ldq, double* __restrict__ R,
+                                                      int ldr, double* __restrict__ work) {
+  using namespace dense;
+  __shared__ double Pn[PW * PLD];  // panel columns (column c at Pn[c * PLD], rows relative to the panel start)
+  __shared__ double Xs[PW * PLD];  // X = V^T A_t (row i at Xs[i * PLD]); also V^T V
+  __shared__ double Ts[PW * (PW + 1)];
+  __shared__ double tau_s[PW];
+  const int tid = threadIdx.x, w = tid >> 6, l = tid & 63, lr = l & 15, lk = l >> 4;
+  double* Wm = work;
+  double* tau = work + int64_t(n) * n;
+  double* Tg = tau + n;
+  for (int i = w; i < n; i += NT / 64)
+    for (int j = l; j < n; j += 64) Wm[int64_t(i) * n + j] = A[int64_t(i) * lda + j];
+  __syncthreads();
+  const int npan = (n + PW - 1) / PW;
+  // V element (row i of the panel, column c), unit diagonal, zero above
+  auto vget = [&](int i, int c) -> double { return i < c ? 0.0 : (i == c ? 1.0 : Pn[c * PLD + i]); };
+  for (int pp = 0; pp < npan; pp++) {
+    const int c0 = pp * PW, bs = min(PW, n - c0), mr = n - c0;
+    for (int c = w; c < PW; c += NT / 64)
+      for (int i = l; i < PLD; i += 64) Pn[c * PLD + i] = (c < bs && i < mr) ? Wm[int64_t(c0 + i) * n + c0 + c] : 0.0;
+    __syncthreads();
+    for (int j = 0; j < bs; j++) {
+      // every wave forms the reflector of column j itself (same arithmetic, same result)
+      double s = 0.0;
+      for (int i = j + 1 + l; i < mr; i += 64) s += Pn[j * PLD + i] * Pn[j * PLD + i];
+      for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+      const double al = Pn[j * PLD + j];
+      double tj, sc, beta;
+      if (s == 0.0) {
+        tj = 0.0; sc = 0.0; beta = al;
+      } else {
+        double anorm = sqrt(al * al + s);
+        beta = al >= 0 ? -anorm : anorm;
+        tj = (beta - al) / beta;
+        sc = 1.0 / (al - beta);
+      }
+      // apply H_j to the panel columns c > j
+      for (int c = j + 1 + w; c < bs; c += NT / 64) {
+        double d = 0.0;
+        for (int i = j + l; i < mr; i += 64) {
+          double v = i == j ? 1.0 : Pn[j * PLD + i] * sc;
+          d += v * Pn[c * PLD + i];
+        }
+        for (int o = 32; o > 0; o >>= 1) d += __shfl_xor(d, o);
+        double f = tj * d;
+        for (int i = j + l; i < mr; i += 64) {
+          double v = i == j ? 1.0 : Pn[j * PLD + i] * sc;
+          Pn[c * PLD + i] -= v * f;
+        }
+      }
+      __syncthreads();
+      if (w == 0) {
+        for (int i = j + 1 + l; i < mr; i += 64) Pn[j * PLD + i] *= sc;
+        if (l == 0) { Pn[j * PLD + j] = beta; tau_s[j] = tj; }
+      }
+      __syncthreads();
+    }
+    // V^T V (bs x bs) into Xs, then T (forward, columnwise)
+    for (int idx = tid; idx < PW * PW; idx += NT) {
+      int a = idx >> 5, b = idx & 31;
+      double s = 0.0;
+      if (a < b && b < bs)
+        for (int i = b; i < mr; i++) s += vget(i, a) * vget(i, b);
+      Xs[a * PLD + b] = s;
+    }
+    __syncthreads();
+    if (w == 0) {
+      for (int j = 0; j < bs; j++) {
+        // T[0:j, j] = -tau_j T[0:j, 0:j] (V^T v_j)[0:j]
+        double y = l < j ? Xs[l * PLD + j] : 0.0;
+        double acc = 0.0;
+        for (int t = 0; t < j; t++) {
+          double yt = __shfl(y, t);
+          if (l <= t && l < j) acc += Ts[l * (PW + 1) + t] * yt;
+        }
+        if (l < j) Ts[l * (PW + 1) + j] = -tau_s[j] * acc;
+        if (l == j) Ts[j * (PW + 1) + j] = tau_s[j];
+        if (l > j && l < PW) Ts[l * (PW + 1) + j] = 0.0;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      }
+      for (int j = bs; j < PW; j++)
+        if (l < PW) Ts[l * (PW + 1) + j] = 0.0;
+    }
+    __syncthreads();
+    // store the panel (R part and V) and T
+    for (int c = w; c < bs; c += NT / 64)
+      for (int i = l; i < mr; i += 64) Wm[int64_t(c0 + i) * n + c0 + c] = Pn[c * PLD + i];
+    for (int idx = tid; idx < PW * PW; idx += NT) Tg[int64_t(pp) * PW * PW + idx] = Ts[(idx >> 5) * (PW + 1) + (idx & 31)];
+    if (tid < bs) tau[c0 + tid] = tau_s[tid];
+    // trailing update of columns c1.. (rows c0..)
+    const int c1 = c0 + bs, nt = n - c1;
+    if (nt > 0) {
+      const int ntl = (nt + 15) >> 4, nml = (mr + 15) >> 4;
+      // X = V^T A_t  (2 x ntl tiles, K = mr)
+      for (int t = w; t < 2 * ntl; t += NT / 64) {
+        const int I = t / ntl, J = t % ntl, col = 16 * J + lr;
+        d4s acc = {0.0, 0.0, 0.0, 0.0};
+        for (int k0 = 0; k0 < mr; k0 += 4) {
+          int k = k0 + lk;
+          double a = k < mr ? vget(k, 16 * I + lr) : 0.0;
+          double bb = (k < mr && col < nt) ? Wm[int64_t(c0 + k) * n + c1 + col] : 0.0;
+          acc = mfma64s(a, bb, acc);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; r++) Xs[(16 * I + 4 * r + lk) * PLD + col] = acc[r];
+      }
+      __syncthreads();
+      // X <- T^T X (column by column in registers)
+      // (in place per column, downwards: row i only needs rows <= i)
+      for (int c = tid; c < nt; c += NT) {
+        for (int i = PW - 1; i >= 0; i--) {
+          double s = 0.0;
+          for (int t = 0; t <= i; t++) s += Ts[t * (PW + 1) + i] * Xs[t * PLD + c];
+          Xs[i * PLD + c] = s;
+        }
+      }
+      __syncthreads();
+      // A_t -= V X  (nml x ntl tiles, K = 32)
+      for (int t = w; t < nml * ntl; t += NT / 64) {
+        const int I = t / ntl, J = t % ntl, col = 16 * J + lr;
+        d4s acc;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          int row = 16 * I + 4 * r + lk;
+          acc[r] = (row < mr && col < nt) ? -Wm[int64_t(c0 + row) * n + c1 + col] : 0.0;
+        }
+        const int arow = 16 * I + lr;
+#pragma unroll
+        for (int k0 = 0; k0 < PW; k0 += 4) {
+          double a = arow < mr ? vget(arow, k0 + lk) : 0.0;
+          acc = mfma64s(a, Xs[(k0 + lk) * PLD + col], acc);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          int row = 16 * I + 4 * r + lk;
+          if (row < mr && col < nt) Wm[int64_t(c0 + row) * n + c1 + col] = -acc[r];
+        }
+      }
+    }
+    __syncthreads();
+  }
+  // R
+  for (int i = w; i < n; i += NT / 64)
+    for (int j = l; j < n; j += 64) R[int64_t(i) * ldr + j] = i <= j ? Wm[int64_t(i) * n + j] : 0.0;
+  // Q = H_0 ... H_{n-1} I, accumulated backwards over the panels
+  for (int i = w; i < n; i += NT / 64)
+    for (int j = l; j < n; j += 64) Q[int64_t(i) * ldq + j] = i == j ? 1.0 : 0.0;
+  __syncthreads();
+  for (int pp = npan - 1; pp >= 0; pp--) {
+    const int c0 = pp * PW, bs = min(PW, n - c0), mr = n - c0, nq = n - c0;
+    for (int c = w; c < PW; c += NT / 64)
+      for (int i = l; i < PLD; i += 64)
+        Pn[c * PLD + i] = (c < bs && i < mr && i > c) ? Wm[int64_t(c0 + i) * n + c0 + c] : 0.0;
+    for (int idx = tid; idx < PW * PW; idx += NT) Ts[(idx >> 5) * (PW + 1) + (idx & 31)] = Tg[int64_t(pp) * PW * PW + idx];
+    __syncthreads();
+    const int ntl = (nq + 15) >> 4, nml = (mr + 15) >> 4;
+    // X = V^T Q_sub (K = mr)
+    for (int t = w; t < 2 * ntl; t += NT / 64) {
+      const int I = t / ntl, J = t % ntl, col = 16 * J + lr;
+      d4s acc = {0.0, 0.0, 0.0, 0.0};
+      for (int k0 = 0; k0 < mr; k0 += 4) {
+        int k = k0 + lk;
+        double a = (k < mr && 16 * I + lr < bs) ? vget(k, 16 * I + lr) : 0.0;
+        double bb = (k < mr && col < nq) ? Q[int64_t(c0 + k) * ldq + c0 + col] : 0.0;
+        acc = mfma64s(a, bb, acc);
+      }
+#pragma unroll
+      for (int r = 0; r < 4; r++) Xs[(16 * I + 4 * r + lk) * PLD + col] = acc[r];
+    }
+    __syncthreads();
+    // X <- T X
+    // (in place per column, upwards: row i only needs rows >= i)
+    for (int c = tid; c < nq; c += NT) {
+      for (int i = 0; i < PW; i++) {
+        double s = 0.0;
+        for (int t = i; t < PW; t++) s += Ts[i * (PW + 1) + t] * Xs[t * PLD + c];
+        Xs[i * PLD + c] = s;
+      }
+    }
+    __syncthreads();
+    // Q_sub -= V X
+    for (int t = w; t < nml * ntl; t += NT / 64) {
+      const int I = t / ntl, J = t % ntl, col = 16 * J + lr;
+      d4s acc;
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        int row = 16 * I + 4 * r + lk;
+        acc[r] = (row < mr && col < nq) ? -Q[int64_t(c0 + row) * ldq + c0 + col] : 0.0;
+      }
+      const int arow = 16 * I + lr;
+#pragma unroll
+      for (int k0 = 0; k0 < PW; k0 += 4) {
+        int kc = k0 + lk;
+        double a = (arow < mr && kc < bs) ? vget(arow, kc) : 0.0;
+        acc = mfma64s(a, Xs[(k0 + lk) * PLD + col], acc);
+      }
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        int row = 16 * I + 4 * r + lk;
+        if (row < mr && col < nq) Q[int64_t(c0 + row) * ldq + c0 + col] = -acc[r];
+      }
+    }
+    __syncthreads();
+  }
+}

@@ -468,6 +468,185 @@ __global__ void k_bjac_check(int* state, int sweep) {
   if (threadIdx.x == 0 && !state[0] && state[1 + sweep] == 0) state[0] = 1;
 }
 
+// ---------------------------------------------------------------------------
+// Block step of the real one-sided Jacobi driven by the Gram of the block
+// pair (rows <= 256, blocks of GB = 16 columns).  Per launch and workgroup:
+//   1. stage the pair's 32 columns of H in LDS;
+//   2. G = Hs^T Hs with v_mfma_f64_16x16x4f64 (split over the rows, partial
+//      sums added in a fixed order);
+//   3. if no pair (u, v) of G has |G_uv| > tol sqrt(G_uu G_vv) the columns are
+//      already orthogonal to the test's accuracy: nothing is written and the
+//      sweep flag stays clear;
+//   4. otherwise cyclic two-sided Jacobi sweeps on G (16 disjoint rotations per
+//      round) accumulate the orthogonal Q, and H[:, pair] <- Hs Q and
+//      J[:, pair] <- J[:, pair] Q are formed with MFMA.
+// The rotation of columns (u, v) is the one-sided Jacobi rotation of the
+// columns themselves (same angle formula); the test in 3. always uses a Gram
+// freshly computed from the current columns, so convergence is decided by
+// the one-sided criterion.
+// ---------------------------------------------------------------------------
+constexpr int GB = 16;
+constexpr int GC = 2 * GB;
+constexpr int GLD = GC + 1;
+constexpr int GROWS = 256;
+
+typedef double d4s __attribute__((ext_vector_type(4)));
+PC_DEV d4s mfma64s(double a, double b, d4s c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+
+__global__ __launch_bounds__(NT) void k_bjac_gstep(double* H, int ldh, int rows, int cols, double* J, int ldj, int nb2,
+                                                   int step, int sweep, int* state, const double* dstate, double tol,
+                                                   int max_inner) {
+  __shared__ double Hs[GC * (GROWS + 1)];
+  __shared__ double G[GC * GLD];
+  __shared__ double Q[GC * GLD];
+  __shared__ double Pp[3 * 4 * 256];
+  __shared__ double rcs[GB], rsn[GB];
+  __shared__ int ru[GB], rv[GB], ract[GB];
+  __shared__ int need, rot_s;
+  if (state[0]) return;
+  const int RP = (rows + 15) & ~15;
+  const int ld = RP + 1;
+  const int p = blockIdx.x;
+  const int bi = rr_player(p, step, nb2), bj = rr_player(nb2 - 1 - p, step, nb2);
+  const double negl = dstate[0];
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63, lr = l & 15, lk = l >> 4;
+  auto gcol = [&](int lc) { return lc < GB ? bi * GB + lc : bj * GB + (lc - GB); };
+  for (int idx = threadIdx.x; idx < GC * RP; idx += NT) {
+    int lc = idx / RP, i = idx - lc * RP;
+    int gc = gcol(lc);
+    Hs[lc * ld + i] = (gc < cols && i < rows) ? H[int64_t(gc) * ldh + i] : 0.0;
+  }
+  if (threadIdx.x == 0) need = 0;
+  __syncthreads();
+  // G = Hs^T Hs: wave w computes output block (bI, bJ) = ((w&3)>>1, w&1) over row quarter q = w>>2
+  {
+    const int blk = w & 3, q = w >> 2, bI = blk >> 1, bJ = blk & 1;
+    const int kq = RP / 4;
+    const double* ca = Hs + (16 * bI + lr) * ld;
+    const double* cb = Hs + (16 * bJ + lr) * ld;
+    d4s acc = {0.0, 0.0, 0.0, 0.0};
+    for (int k0 = q * kq; k0 < (q + 1) * kq; k0 += 4) acc = mfma64s(ca[k0 + lk], cb[k0 + lk], acc);
+    if (q > 0)
+      for (int r = 0; r < 4; r++) Pp[((q - 1) * 4 + blk) * 256 + r * 64 + l] = acc[r];
+    __syncthreads();
+    if (q == 0) {
+      for (int q2 = 0; q2 < 3; q2++)
+        for (int r = 0; r < 4; r++) acc[r] += Pp[(q2 * 4 + blk) * 256 + r * 64 + l];
+      for (int r = 0; r < 4; r++) G[(16 * bI + 4 * r + lk) * GLD + 16 * bJ + lr] = acc[r];
+    }
+  }
+  __syncthreads();
+  for (int idx = threadIdx.x; idx < GC * GC; idx += NT) {
+    int u = idx / GC, v = idx - u * GC;
+    Q[u * GLD + v] = u == v ? 1.0 : 0.0;
+    if (u < v) {
+      double a = G[u * GLD + u], bb = G[v * GLD + v], c = G[u * GLD + v];
+      if (a > negl && bb > negl && fabs(c) > tol * sqrt(a) * sqrt(bb)) need = 1;
+    }
+  }
+  __syncthreads();
+  if (!need) return;
+  if (threadIdx.x == 0) atomicOr(&state[1 + sweep], 1);
+  for (int inner = 0; inner < max_inner; inner++) {
+    if (threadIdx.x == 0) rot_s = 0;
+    __syncthreads();
+    for (int s = 0; s < GC - 1; s++) {
+      if (threadIdx.x < GB) {
+        int pr = threadIdx.x;
+        int u = rr_player(pr, s, GC), v = rr_player(GC - 1 - pr, s, GC);
+        if (u > v) { int t = u; u = v; v = t; }
+        double a = G[u * GLD + u], bb = G[v * GLD + v], c = G[u * GLD + v], gm = fabs(c);
+        int act = 0;
+        double cs = 1.0, sn = 0.0;
+        if (a > negl && bb > negl && gm > tol * sqrt(a) * sqrt(bb)) {
+          double zeta = (bb - a) / (2.0 * gm);
+          double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+          cs = 1.0 / sqrt(1.0 + t * t);
+          sn = cs * t;
+          if (c < 0) sn = -sn;  // fold the sign of c (real phase) into the rotation
+          act = 1;
+          rot_s = 1;
+        }
+        ru[pr] = u; rv[pr] = v; rcs[pr] = cs; rsn[pr] = sn; ract[pr] = act;
+      }
+      __syncthreads();
+      // G <- G R and Q <- Q R (columns u, v of each active pair)
+      for (int t = threadIdx.x; t < 2 * GC * GB; t += NT) {
+        int mat = t / (GC * GB), tt = t - mat * (GC * GB);
+        int i = tt / GB, pr = tt - i * GB;
+        if (!ract[pr]) continue;
+        double* Mx = mat ? Q : G;
+        int u = ru[pr], v = rv[pr];
+        double cs = rcs[pr], sn = rsn[pr];
+        double x = Mx[i * GLD + u], y = Mx[i * GLD + v];
+        Mx[i * GLD + u] = cs * x - sn * y;
+        Mx[i * GLD + v] = sn * x + cs * y;
+      }
+      __syncthreads();
+      // G <- R^T G (rows u, v)
+      for (int t = threadIdx.x; t < GC * GB; t += NT) {
+        int j = t / GB, pr = t - j * GB;
+        if (!ract[pr]) continue;
+        int u = ru[pr], v = rv[pr];
+        double cs = rcs[pr], sn = rsn[pr];
+        double x = G[u * GLD + j], y = G[v * GLD + j];
+        G[u * GLD + j] = cs * x - sn * y;
+        G[v * GLD + j] = sn * x + cs * y;
+      }
+      __syncthreads();
+    }
+    int any = rot_s;
+    __syncthreads();
+    if (!any) break;
+  }
+  // H[:, pair] = Hs Q  (row blocks of 16 rows, one wave each)
+  for (int rb = w; rb < RP / 16; rb += NT / 64) {
+    for (int cB = 0; cB < 2; cB++) {
+      d4s acc = {0.0, 0.0, 0.0, 0.0};
+      for (int k0 = 0; k0 < GC; k0 += 4)
+        acc = mfma64s(Hs[(k0 + lk) * ld + 16 * rb + lr], Q[(k0 + lk) * GLD + 16 * cB + lr], acc);
+      int gc = gcol(16 * cB + lr);
+      for (int r = 0; r < 4; r++) {
+        int row = 16 * rb + 4 * r + lk;
+        if (row < rows && gc < cols) H[int64_t(gc) * ldh + row] = acc[r];
+      }
+    }
+  }
+  // J[:, pair] = J[:, pair] Q  (J is cols x cols, column-major)
+  const int CP = (cols + 15) & ~15;
+  for (int rb = w; rb < CP / 16; rb += NT / 64) {
+    double av[GC / 4];
+    int row = 16 * rb + lr;
+    for (int kk = 0; kk < GC / 4; kk++) {
+      int gc = gcol(4 * kk + lk);
+      av[kk] = (gc < cols && row < cols) ? J[int64_t(gc) * ldj + row] : 0.0;
+    }
+    d4s acc[2];
+    for (int cB = 0; cB < 2; cB++) {
+      acc[cB] = d4s{0.0, 0.0, 0.0, 0.0};
+      for (int kk = 0; kk < GC / 4; kk++) acc[cB] = mfma64s(av[kk], Q[(4 * kk + lk) * GLD + 16 * cB + lr], acc[cB]);
+    }
+    for (int cB = 0; cB < 2; cB++) {
+      int gc = gcol(16 * cB + lr);
+      for (int r = 0; r < 4; r++) {
+        int orow = 16 * rb + 4 * r + lk;
+        if (orow < cols && gc < cols) J[int64_t(gc) * ldj + orow] = acc[cB][r];
+      }
+    }
+  }
+}
+
+// rows of the orthogonal preconditioner: V[perm[k]][:] = Q[k][:]
+template <class T>
+__global__ __launch_bounds__(256) void k_permute_rows(const T* Qm, int ldq, int n, const int* perm, T* V, int ldv) {
+  int64_t idx = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (idx >= int64_t(n) * n) return;
+  int k = int(idx / n), j = int(idx % n);
+  int pk = perm[k];
+  pk = pk < 0 ? 0 : (pk >= n ? n - 1 : pk);  // perm is a permutation; the clamp only guards
+  V[int64_t(pk) * ldv + j] = Qm[int64_t(k) * ldq + j];
+}
+
 // column norms, non-finite and non-convergence flags
 template <class T>
 __global__ __launch_bounds__(NT) void k_bjac_norms(const T* H, int ldh, int rows, int cols, double* sig,
@@ -616,6 +795,7 @@ __global__ __launch_bounds__(256) void k_copy2d(const T* A, int lda, int rows, i
   if (idx >= int64_t(rows) * cols) return;
   int i = int(idx / cols), j = int(idx % cols);
   T v = A[int64_t(i) * lda + j];
+  if (trans == 2) v = conj_(v);  // 2: conjugate transpose
   if (trans) B[int64_t(j) * ldb + i] = v;
   else B[int64_t(i) * ldb + j] = v;
 }

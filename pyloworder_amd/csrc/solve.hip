@@ -15,6 +15,7 @@
 #include "common.hpp"
 #include "kernels.hpp"
 #include "k_small.hip"
+#include "k_dense.hip"
 
 int pc_check_mat(pc_ctx* ctx, const void* p, int64_t ld, int64_t m, int64_t n, const char* what);
 
@@ -24,7 +25,8 @@ constexpr double kShiftF64 = 1e-8;   // tau / trace(G1), FP64 path
 constexpr double kShiftMx = 1e-6;    // tau / trace(G1), mixed path
 constexpr double kTolF64 = 1e-15;    // pivoted Cholesky rank tolerance (relative to max diag)
 constexpr double kTolMx = 1e-7;
-constexpr double kNullF64 = 1e-13;   // sigma <= tol * sigma_max counts as null
+constexpr double kNullF64 = 1e-11;   // sigma <= tol * sigma_max counts as null: the FP64 path resolves
+                                     // a null direction of Y to ~1e-13 sigma_max (rounding through R1)
 constexpr double kNullMx = 1e-6;
 
 inline unsigned nblk(int64_t count, int per = 256) { return unsigned(ceil_div(count, per)); }
@@ -38,24 +40,78 @@ template <class T>
 void launch_block_jacobi(pc_ctx* ctx, T* H, int ldh, int rows, int cols, T* J, int ldj, double* sig, int* state,
                          double* dstate, int* status) {
   hipStream_t st = ctx->stream;
-  int b = bjac_width(std::max(rows, cols), int(sizeof(T)));
+  // real FP64 with rows <= 256: Gram-driven MFMA steps on 16-column blocks;
+  // otherwise column-pair rotations staged in LDS
+  const bool gram_steps = std::is_same<T, double>::value && rows <= GROWS;
+  int b = gram_steps ? GB : bjac_width(std::max(rows, cols), int(sizeof(T)));
   int nb2 = int(ceil_div(cols, b));
   nb2 += nb2 & 1;
   if (nb2 < 2) nb2 = 2;
   hipLaunchKernelGGL((k_bjac_init<T>), dim3(1), dim3(1024), 0, st, H, ldh, rows, cols, J, ldj, state, dstate,
                      kBjacMaxSweeps + 1);
-  int max_inner = nb2 == 2 ? 60 : 3;
+  int max_inner = gram_steps ? (nb2 == 2 ? 8 : 1) : (nb2 == 2 ? 60 : 3);
   // relative orthogonality test |h_p^H h_q| <= tol ||h_p|| ||h_q||: the dot
   // product itself carries ~eps sqrt(rows) rounding, so tol grows with rows
   double tol = 2.220446049250313e-16 * std::max(rows, 16);
   for (int s = 0; s < kBjacMaxSweeps; s++) {
-    for (int step = 0; step < nb2 - 1; step++)
+    for (int step = 0; step < nb2 - 1; step++) {
+      if constexpr (std::is_same<T, double>::value) {
+        if (gram_steps) {
+          hipLaunchKernelGGL(k_bjac_gstep, dim3(nb2 / 2), dim3(1024), 0, st, H, ldh, rows, cols, J, ldj, nb2, step, s,
+                             state, dstate, tol, max_inner);
+          continue;
+        }
+      }
       hipLaunchKernelGGL((k_bjac_step<T>), dim3(nb2 / 2), dim3(1024), 0, st, H, ldh, rows, cols, J, ldj, b, nb2, step,
                          s, state, dstate, tol, max_inner);
+    }
     hipLaunchKernelGGL(k_bjac_check, dim3(1), dim3(64), 0, st, state, s);
   }
   hipLaunchKernelGGL((k_bjac_norms<T>), dim3(1), dim3(1024), 0, st, H, ldh, rows, cols, sig, state, status);
 }
+
+
+// Small dense factorisations: blocked MFMA kernels for real n <= 256, the
+// generic single-workgroup kernels otherwise (complex, larger n).
+template <class T>
+void run_chol_shift(hipStream_t st, const T* G, int n, double shift, T* R, int* status) {
+  if constexpr (std::is_same<T, double>::value)
+    if (n <= dense::NMAX) {
+      hipLaunchKernelGGL(k_chol_fast, dim3(1), dim3(NT), 0, st, G, n, n, shift, R, n, status);
+      return;
+    }
+  hipLaunchKernelGGL((k_chol_shift<T>), dim3(1), dim3(NT), 0, st, G, n, n, shift, R, n, status);
+}
+template <class T>
+void run_trinv(hipStream_t st, const T* R, int n, T* W) {
+  if constexpr (std::is_same<T, double>::value)
+    if (n <= dense::NMAX) {
+      hipLaunchKernelGGL(k_trinv_fast, dim3(1), dim3(NT), 0, st, R, n, n, W, n);
+      return;
+    }
+  hipLaunchKernelGGL((k_trinv_upper<T>), dim3(1), dim3(NT), 0, st, R, n, n, W, n);
+}
+template <class T>
+void run_pchol(hipStream_t st, const T* G, int n, double tol, T* A, T* R, int* perm, int* rank, int* status) {
+  if constexpr (std::is_same<T, double>::value)
+    if (n <= dense::NMAX) {
+      hipLaunchKernelGGL(k_pchol_fast, dim3(1), dim3(NT), 0, st, G, n, n, tol, A, R, n, perm, rank, status);
+      return;
+    }
+  hipLaunchKernelGGL((k_pchol<T>), dim3(1), dim3(NT), 0, st, G, n, n, tol, A, R, n, perm, rank, status);
+}
+// square Householder QR (n x n), Q and R explicit
+template <class T>
+void run_house_qr_sq(hipStream_t st, const T* A, int n, T* Q, T* R, T* work) {
+  if constexpr (std::is_same<T, double>::value)
+    if (n <= dense::NMAX) {
+      hipLaunchKernelGGL(k_house_qr_fast, dim3(1), dim3(NT), 0, st, A, n, n, Q, n, R, n, work);
+      return;
+    }
+  hipLaunchKernelGGL((k_house_qr<T>), dim3(1), dim3(NT), 0, st, A, n, n, n, Q, n, R, n, work);
+}
+// workspace of the small dense stage (T elements): house QR and null completion
+inline int64_t dense_work(int n) { return 3 * int64_t(n) * n + 64 + n + 1024 * int64_t((n + 31) / 32); }
 
 // output element type for (compute type T, real storage type ST)
 template <class T, class ST> struct VtType;
@@ -94,6 +150,7 @@ template <class T> struct SmallBufs {
   T *G1, *R1, *W1, *R1h, *G2, *Awk, *R2, *Hc, *J, *UM, *R2i, *tmp, *Wf;
   double* sig;
   int *perm, *order, *rank, *nnull, *status;
+  int *perm3, *rank3;  // pivoted Cholesky of M^H M (Jacobi preconditioner)
   int* jstate;       // block Jacobi: done flag + per-sweep rotation flags
   double* jdstate;   // block Jacobi: negligible squared column norm
   double* Wpad;  // NB x NB real (embedded) operand for the FP64 streaming kernels
@@ -169,6 +226,7 @@ int64_t carve(WsCursor& c, const Plan& p, SmallBufs<T>& b, double** mean, double
   b.sig = c.take<double>(n);
   b.perm = c.take<int>(n); b.order = c.take<int>(n); b.rank = c.take<int>(4); b.nnull = c.take<int>(4);
   b.status = c.take<int>(4);
+  b.perm3 = c.take<int>(n); b.rank3 = c.take<int>(4);
   b.jstate = c.take<int>(kBjacMaxSweeps + 8);
   b.jdstate = c.take<double>(4);
   b.Wpad = c.take<double>(int64_t(p.NB) * p.NB);
@@ -495,45 +553,62 @@ static int core_t(pc_ctx* ctx, const Plan& p, const void* A, int64_t lda, int ce
   int64_t base = ctx->ws_skip;
   {
     WsCursor probe{nullptr, 0, 0};
-    int64_t need = carve<T>(probe, p, b, &mean, &var, &part, slabs) + (int64_t(3) * nn + 64) * sizeof(T);
+    int64_t need = carve<T>(probe, p, b, &mean, &var, &part, slabs) + dense_work(n) * int64_t(sizeof(T));
     PC_CHECK(ctx, pc_ws_reserve(ctx, base + need + 4096));
   }
   WsCursor cur{ctx->ws + base, 0, ctx->ws_bytes - base};
   carve<T>(cur, p, b, &mean, &var, &part, slabs);
-  T* nwork = cur.take<T>(3 * nn + 64);
+  T* nwork = cur.take<T>(dense_work(n));
   PC_HIP(ctx, hipMemsetAsync(b.status, 0, 16, st));
 
   // ---- P1
   PC_CHECK(ctx, launch_streams<T>(ctx, p, A, lda, center, mean, var, part, b, 1));
   reduce_gram<T>(st, p, part, p.nparts, b.G1);
-  hipLaunchKernelGGL((k_chol_shift<T>), dim3(1), dim3(1024), 0, st, b.G1, n, n, p.mixed ? kShiftMx : kShiftF64, b.R1,
-                     n, b.status);
-  hipLaunchKernelGGL((k_trinv_upper<T>), dim3(1), dim3(1024), 0, st, b.R1, n, n, b.W1, n);
+  run_chol_shift<T>(st, b.G1, n, p.mixed ? kShiftMx : kShiftF64, b.R1, b.status);
+  run_trinv<T>(st, b.R1, n, b.W1);
   to_operand<T>(st, p, b.W1, n, b, false);
   if (p.mixed) rounded_back<T>(st, p, b, b.W1);  // W1 := the operand actually applied
-  hipLaunchKernelGGL((k_trinv_upper<T>), dim3(1), dim3(1024), 0, st, b.W1, n, n, b.R1h, n);
+  run_trinv<T>(st, b.W1, n, b.R1h);
   // ---- P2
   PC_CHECK(ctx, launch_streams<T>(ctx, p, A, lda, center, mean, var, part, b, 2));
   reduce_gram<T>(st, p, part, p.mixed ? p.chunks3 : p.chunks, b.G2);
-  hipLaunchKernelGGL((k_pchol<T>), dim3(1), dim3(1024), 0, st, b.G2, n, n, p.mixed ? kTolMx : kTolF64, b.Awk, b.R2, n,
-                     b.perm, b.rank, b.status);
+  run_pchol<T>(st, b.G2, n, p.mixed ? kTolMx : kTolF64, b.Awk, b.R2, b.perm, b.rank, b.status);
   // M = R2 (rows < rank) * R1h[perm, :], stored column-major into Hc
   dim3 gg(unsigned(ceil_div(n, 32)), unsigned(ceil_div(n, 32)));
   hipLaunchKernelGGL((k_gemm_small<T>), gg, dim3(256), 0, st, 0, 0, n, n, n, T(1.0), b.R2, n, (const int*)nullptr,
                      b.R1h, n, (const int*)b.perm, T(0.0), b.Hc, n, 1);
   hipLaunchKernelGGL((k_pad_identity<T>), dim3(1), dim3(256), 0, st, b.R2, n, n, b.rank);
-  hipLaunchKernelGGL((k_trinv_upper<T>), dim3(1), dim3(1024), 0, st, b.R2, n, n, b.R2i, n);
+  run_trinv<T>(st, b.R2, n, b.R2i);
   int nc = n;
   if (mode == CORE_SVD) {
+    // Right preconditioner V0 = P3 Q4 (Drmac-Veselic): P3^T (M^H M) P3 = R3^H R3
+    // (pivoted Cholesky), R3^H = Q4 R4 (Householder).  One-sided Jacobi on
+    // M V0 takes the same rotations as on the lower-triangular R4^H and
+    // converges in a few sweeps; V0 only has to be unitary, so R3 coming from
+    // the Gram is fine.  Buffers: tmp = M (row-major), G1 = M^H M then R4,
+    // R1 = R3, Awk = R3^H then V0, UM = Q4.
+    hipLaunchKernelGGL((k_copy2d<T>), dim3(nblk(nn)), dim3(256), 0, st, b.Hc, n, n, n, b.tmp, n, 1);
+    hipLaunchKernelGGL((k_gemm_small<T>), gg, dim3(256), 0, st, 2, 0, n, n, n, T(1.0), b.tmp, n, (const int*)nullptr,
+                       b.tmp, n, (const int*)nullptr, T(0.0), b.G1, n, 0);
+    run_pchol<T>(st, b.G1, n, 1e-30, b.Awk, b.R1, b.perm3, b.rank3, b.status);
+    hipLaunchKernelGGL((k_copy2d<T>), dim3(nblk(nn)), dim3(256), 0, st, b.R1, n, n, n, b.Awk, n, 2);
+    run_house_qr_sq<T>(st, b.Awk, n, b.UM, b.G1, nwork);
+    hipLaunchKernelGGL((k_permute_rows<T>), dim3(nblk(nn)), dim3(256), 0, st, b.UM, n, n, (const int*)b.perm3, b.Awk,
+                       n);
+    hipLaunchKernelGGL((k_gemm_small<T>), gg, dim3(256), 0, st, 0, 0, n, n, n, T(1.0), b.tmp, n, (const int*)nullptr,
+                       b.Awk, n, (const int*)nullptr, T(0.0), b.Hc, n, 1);
     launch_block_jacobi<T>(ctx, b.Hc, n, n, n, b.J, n, b.sig, b.jstate, b.jdstate, b.status);
+    // V = V0 J (column-major like J)
+    hipLaunchKernelGGL((k_gemm_small<T>), gg, dim3(256), 0, st, 0, 1, n, n, n, T(1.0), b.Awk, n, (const int*)nullptr,
+                       b.J, n, (const int*)nullptr, T(0.0), b.tmp, n, 1);
     hipLaunchKernelGGL(k_sort_desc, dim3(1), dim3(1024), 0, st, b.sig, n, b.order);
     hipLaunchKernelGGL((k_svd_finish<T, ST, typename VtType<T, ST>::type>), dim3(nblk(nn)), dim3(256), 0, st, b.Hc,
-                       n, n, n, b.sig, b.order, b.J, n, p.mixed ? kNullMx : kNullF64, (const int*)b.rank,
+                       n, n, n, b.sig, b.order, b.tmp, n, p.mixed ? kNullMx : kNullF64, (const int*)b.rank,
                        (ST*)out.S, (typename VtType<T, ST>::type*)out.VT, out.ldvt, b.UM, n, b.nnull);
   } else {
     // QR of M (n x n, rows >= rank zero): M = Q_M R_M;  R output, UM := Q_M
     hipLaunchKernelGGL((k_copy2d<T>), dim3(nblk(nn)), dim3(256), 0, st, b.Hc, n, n, n, b.tmp, n, 1);
-    hipLaunchKernelGGL((k_house_qr<T>), dim3(1), dim3(1024), 0, st, b.tmp, n, n, n, b.UM, n, b.J, n, nwork);
+    run_house_qr_sq<T>(st, b.tmp, n, b.UM, b.J, nwork);
     hipLaunchKernelGGL((k_store_r<T, typename VtType<T, ST>::type>), dim3(nblk(nn)), dim3(256), 0, st, b.J, n, n,
                        (typename VtType<T, ST>::type*)out.VT, out.ldvt, b.rank, b.nnull);
   }
