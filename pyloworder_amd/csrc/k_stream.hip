@@ -8,9 +8,9 @@ template <class T> PC_DEV double to_d(T v) { return double(v); }
 
 // One wave per row: temporal mean (pyLOM/vmmath/averaging.py:17-29) and
 // population variance (averaging.py:46-59), both accumulated in FP64.
-template <class T>
+template <class T, class TO = T>
 __global__ __launch_bounds__(256) void k_row_stats(const T* __restrict__ X, int64_t ldx, int64_t m,
-                                                   int64_t n, T* __restrict__ mean, T* __restrict__ var) {
+                                                   int64_t n, TO* __restrict__ mean, TO* __restrict__ var) {
   int lane = threadIdx.x & 63;
   int64_t row = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
   if (row >= m) return;
@@ -23,9 +23,9 @@ __global__ __launch_bounds__(256) void k_row_stats(const T* __restrict__ X, int6
     double v = 0.0;
     for (int64_t j = lane; j < n; j += 64) { double d = to_d(x[j]) - mu; v += d * d; }
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    if (lane == 0) var[row] = T(v / double(n));
+    if (lane == 0) var[row] = TO(v / double(n));
   }
-  if (lane == 0 && mean) mean[row] = T(mu);
+  if (lane == 0 && mean) mean[row] = TO(mu);
 }
 
 // Y = (X - mean) [/ var]   (averaging.py:31-44 and 61-75)
@@ -139,6 +139,13 @@ int check_mat(pc_ctx* ctx, const void* p, int64_t ld, int64_t m, int64_t n, cons
 }
 
 }  // namespace
+
+// FP64 statistics of float32 rows (the variance-normalised mixed path)
+void launch_row_stats_f32(hipStream_t st, const float* X, int64_t ldx, int64_t m, int64_t n, double* mean,
+                          double* var) {
+  dim3 g = dim3(unsigned(ceil_div(m, 4)));
+  hipLaunchKernelGGL((k_row_stats<float, double>), g, dim3(256), 0, st, X, ldx, m, n, mean, var);
+}
 
 int pc_check_mat(pc_ctx* ctx, const void* p, int64_t ld, int64_t m, int64_t n, const char* what) {
   return check_mat(ctx, p, ld, m, n, what);

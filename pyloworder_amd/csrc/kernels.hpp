@@ -26,6 +26,21 @@ int launch_zgram_mx(hipStream_t st, const float* X, int64_t ldx, int64_t m, int 
 int launch_gemm_mx(hipStream_t st, const float* X, int64_t ldx, int64_t m, int n, int center, const double* mean_in,
                    const double* var_in, const _Float16* Wh, const _Float16* Wl, const float* wscale, int nc,
                    int chunks, int64_t tpc, float* C, int64_t ldc);
+// second-generation mixed kernels (k_mfma_mx2.hip): 8 waves, W operand in registers
+int mx2_nb(int n);
+int mx2_gram_rows();
+int mx2_tile_rows(int n);
+int launch_gram_mx2(hipStream_t st, const float* X, int64_t ldx, int64_t m, int n, int center, double* mean_io,
+                    const double* var, int chunks, int64_t tpc, double* part);
+int launch_zgram_mx2(hipStream_t st, const float* X, int64_t ldx, int64_t m, int n, int center, const double* mean,
+                     const double* var, const _Float16* Wf, const float* wscale, int chunks, int64_t tpc,
+                     double* part);
+int launch_gemm_mx2(hipStream_t st, const float* X, int64_t ldx, int64_t m, int n, int center, const double* mean,
+                    const double* var, const _Float16* Wh, const _Float16* Wl, const float* wscale, int nc,
+                    int chunks, int64_t tpc, float* C, int64_t ldc);
+// FP64 row mean / variance of float32 rows (k_stream.hip)
+void launch_row_stats_f32(hipStream_t st, const float* X, int64_t ldx, int64_t m, int64_t n, double* mean,
+                          double* var);
 // pack an FP64 n x nc matrix (ld ldw) into fp16 hi (and lo) fragment order with
 // per-column power-of-two scales; also writes the FP64 value of hi*2^-s (Wr)
 int launch_pack_w_mx(hipStream_t st, const double* W, int ldw, int n, int nc, int NB, int NCB, _Float16* Wh,

@@ -193,9 +193,9 @@ int make_plan(pc_ctx* ctx, pc_dtype dt, int64_t m, int64_t n, int precision, Pla
   p.nr = p.cplx ? 2 * int(n) : int(n);
   p.m = m;
   if (p.mixed) {
-    p.NB = mx_nb(p.nr);
-    p.tr = mx_gram_rows();
-    p.tr3 = mx_gemm_rows();
+    p.NB = mx2_nb(p.nr);
+    p.tr = mx2_gram_rows();
+    p.tr3 = mx2_tile_rows(p.nr);
   } else {
     p.NB = f64_nb(p.nr);
     p.tr = f64_tile_rows();
@@ -443,6 +443,17 @@ __global__ __launch_bounds__(256) void k_embed_pad(const cdouble* W, int ldw, in
   P[idx] = v;
 }
 
+// M <- M C, C = I - 1 1^T / n  (M n x n, column-major: M[i][j] = Hc[j * ld + i])
+template <class T>
+__global__ __launch_bounds__(256) void k_center_right(T* Hc, int ld, int n) {
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  T s = T(0.0);
+  for (int j = 0; j < n; j++) s += Hc[int64_t(j) * ld + i];
+  s = s * (1.0 / n);
+  for (int j = 0; j < n; j++) Hc[int64_t(j) * ld + i] -= s;
+}
+
 // mixed: Wr (real FP64 NB x NB, values of the rounded operand) -> T n x n
 __global__ __launch_bounds__(256) void k_unpad_real(const double* P, int NB, int n, double* W) {
   int64_t idx = int64_t(blockIdx.x) * 256 + threadIdx.x;
@@ -480,9 +491,12 @@ static int launch_streams(pc_ctx* ctx, const Plan& p, const void* A, int64_t lda
   hipEvent_t ev0;
   pc_prof_begin(ctx, &ev0);
   if (p.mixed) {
-    if (pass == 1) rc = launch_gram_mx(st, (const float*)A, ldr, p.m, p.nr, center, mean, var, p.chunks, p.tpc, part);
-    else rc = launch_zgram_mx(st, (const float*)A, ldr, p.m, p.nr, center, mean, var, b.Wh, b.wscale, p.chunks3,
-                              p.tpc3, part);
+    // center 1: the kernels shift each row by its first element and P1 records
+    // that shift in `mean`; center 2: exact FP64 statistics first
+    if (pass == 1 && center == 2) launch_row_stats_f32(st, (const float*)A, ldr, p.m, p.nr, mean, var);
+    if (pass == 1) rc = launch_gram_mx2(st, (const float*)A, ldr, p.m, p.nr, center, mean, var, p.chunks, p.tpc, part);
+    else rc = launch_zgram_mx2(st, (const float*)A, ldr, p.m, p.nr, center, mean, var, b.Wh, b.wscale, p.chunks3,
+                               p.tpc3, part);
   } else {
     if (pass == 1)
       rc = launch_gram_f64(st, sdt, A, ldr, p.m, p.nr, center, 0, nullptr, nullptr, center ? mean : nullptr,
@@ -577,6 +591,10 @@ static int core_t(pc_ctx* ctx, const Plan& p, const void* A, int64_t lda, int ce
   dim3 gg(unsigned(ceil_div(n, 32)), unsigned(ceil_div(n, 32)));
   hipLaunchKernelGGL((k_gemm_small<T>), gg, dim3(256), 0, st, 0, 0, n, n, n, T(1.0), b.R2, n, (const int*)nullptr,
                      b.R1h, n, (const int*)b.perm, T(0.0), b.Hc, n, 1);
+  // mixed path: the passes saw Y - a 1^T (a = row shift); Y C = (Y - a 1^T) C, so the
+  // exact centring is M <- M C
+  if (p.mixed && center)
+    hipLaunchKernelGGL((k_center_right<T>), dim3(nblk(n)), dim3(256), 0, st, b.Hc, n, n);
   hipLaunchKernelGGL((k_pad_identity<T>), dim3(1), dim3(256), 0, st, b.R2, n, n, b.rank);
   run_trinv<T>(st, b.R2, n, b.R2i);
   int nc = n;
@@ -632,8 +650,8 @@ static int core_t(pc_ctx* ctx, const Plan& p, const void* A, int64_t lda, int ce
     int64_t ldr = p.cplx ? 2 * lda : lda;
     int64_t ldu = p.cplx ? 2 * out.ldu : out.ldu;
     if (p.mixed)
-      rc = launch_gemm_mx(st, (const float*)A, ldr, p.m, p.nr, center, mean, var, b.Wh, b.Wl, b.wscale, p.nr,
-                          p.chunks3, p.tpc3, (float*)out.U, ldu);
+      rc = launch_gemm_mx2(st, (const float*)A, ldr, p.m, p.nr, center, mean, var, b.Wh, b.Wl, b.wscale, p.nr,
+                           p.chunks3, p.tpc3, (float*)out.U, ldu);
     else
       rc = launch_gemm_f64(st, sdt, A, ldr, p.m, p.nr, center, mean, var, b.Wpad, p.nr, p.chunks3, p.tpc3, out.U, ldu);
     if (rc != 0) return pc_fail(ctx, PC_ERR_SHAPE, "no gemm variant for n = %d", n);
