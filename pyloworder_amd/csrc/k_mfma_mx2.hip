@@ -45,6 +45,21 @@ constexpr int EUN = 1 << 20;  // running exponent not set yet
 
 PC_DEV f16v mfma(h8 a, h8 b, f16v c) { return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0); }
 
+// Workgroup barrier for LDS traffic only.  __syncthreads() also fences global
+// memory, which on this ISA waits for every outstanding global load and store
+// (vmcnt(0)) and would serialise the prefetch of the next tile with the
+// products of the current one.  The asm "memory" clobber keeps the compiler
+// from moving LDS accesses across the barrier.
+PC_DEV void lds_sync() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+// Raw buffer access (range-checked by the hardware: loads beyond num_records
+// return 0, stores beyond it are dropped), so edge tiles need no branches.
+// 0x00020000: dword 3 of the gfx9 buffer descriptor (32-bit data format).
+typedef __amdgpu_buffer_rsrc_t rsrc_t;
+PC_DEV rsrc_t make_rsrc(const void* base, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, 0x00020000);
+}
+
 // e with |x| 2^e < 2^14 for |x| <= mx (mx > 0)
 PC_DEV int scale_exp(float mx) {
   int E;
@@ -93,11 +108,13 @@ PC_DEV int update_scale(float tmax, int e_run, f16v* acc) {
   return e_run;
 }
 
-// shifted value of x in row `row` (see the file comment); a is x_i0 for center 1
-PC_DEV float shifted(float x, float a, int center, int64_t row, const double* mean, const double* var) {
-  if (center == 1) return x - a;
-  if (center == 2) return float((double(x) - mean[row]) * (1.0 / var[row]));
-  return x;
+// shifted value of x (see the file comment): a = x_i0 for CENTER 1; for
+// CENTER 2, mu / iv are the row mean and 1 / variance (FP64)
+template <int CENTER>
+PC_DEV float shifted(float x, float a, double mu, double iv) {
+  if constexpr (CENTER == 1) return x - a;
+  else if constexpr (CENTER == 2) return float((double(x) - mu) * iv);
+  else return x;
 }
 
 // Gram accumulation of one k-step from fragments F[(blk * 2 + part) * 64 + lane]
@@ -147,9 +164,9 @@ PC_DEV void write_upper(const f16v* acc, const int* sI, const int* sJ, const boo
 // ---------------------------------------------------------------------------
 // P1
 // ---------------------------------------------------------------------------
-template <int NB, bool VEC>
+template <int NB, bool VEC, int CENTER>
 __global__ __launch_bounds__(NT2, 1) void k_gram_mx2(const float* __restrict__ X, int64_t ldx, int64_t m, int n,
-                                                     int center, double* __restrict__ mean_io,
+                                                     double* __restrict__ mean_io,
                                                      const double* __restrict__ var, int64_t tpc,
                                                      double* __restrict__ part) {
   constexpr int V = NB >= 128 ? 4 : NB / 32;
@@ -173,6 +190,9 @@ __global__ __launch_bounds__(NT2, 1) void k_gram_mx2(const float* __restrict__ X
   const int64_t t0 = int64_t(blockIdx.x) * tpc;
   const int64_t t1 = t0 + tpc < ntiles ? t0 + tpc : ntiles;
   float v[NG][8][V];
+  // branch-free loads: out-of-range lanes read a clamped, valid address; the
+  // value is masked where it is used, so no instruction waits on the load
+  // until the tile is processed (a branch or a select here would)
   auto load = [&](int64_t tile) {
 #pragma unroll
     for (int t = 0; t < 8; t++) {
@@ -182,17 +202,14 @@ __global__ __launch_bounds__(NT2, 1) void k_gram_mx2(const float* __restrict__ X
 #pragma unroll
       for (int g = 0; g < NG; g++) {
         const int col = g * 32 * V + V * c;
-        if (VEC && rv && col + V <= n) {
-          if constexpr (V == 4) {
-            f4 q = *reinterpret_cast<const f4*>(xr + col);
-            v[g][t][0] = q[0]; v[g][t][1] = q[1]; v[g][t][2] = q[2]; v[g][t][3] = q[3];
-          } else {
+        if constexpr (VEC && V == 4) {
+          const bool ok = rv && col < n;  // n % 4 == 0 on this path
+          f4 q = *reinterpret_cast<const f4*>(xr + (ok ? col : 0));
 #pragma unroll
-            for (int u = 0; u < V; u++) v[g][t][u] = xr[col + u];
-          }
+          for (int u = 0; u < 4; u++) v[g][t][u] = q[u];
         } else {
 #pragma unroll
-          for (int u = 0; u < V; u++) v[g][t][u] = (rv && col + u < n) ? xr[col + u] : 0.0f;
+          for (int u = 0; u < V; u++) v[g][t][u] = xr[(rv && col + u < n) ? col + u : 0];
         }
       }
     }
@@ -206,20 +223,24 @@ __global__ __launch_bounds__(NT2, 1) void k_gram_mx2(const float* __restrict__ X
       const int64_t row = tile * TR + 16 * w + 8 * hl + t;
       const bool rv = row < m;
       const float a = __shfl(v[0][t][0], hl * 32);  // x_i0 sits in lane c = 0 of this half
-      if (center == 1 && rv && c == 0 && mean_io) mean_io[row] = double(a);
+      double mu = 0.0, iv = 1.0;
+      if constexpr (CENTER == 2) {
+        mu = mean_io[rv ? row : 0];
+        iv = 1.0 / var[rv ? row : 0];
+      }
 #pragma unroll
       for (int g = 0; g < NG; g++)
 #pragma unroll
         for (int u = 0; u < V; u++) {
           const int col = g * 32 * V + V * c + u;
-          float y = (rv && col < n) ? shifted(v[g][t][u], a, center, rv ? row : 0, mean_io, var) : 0.0f;
+          float y = (rv && col < n) ? shifted<CENTER>(v[g][t][u], a, mu, iv) : 0.0f;
           v[g][t][u] = y;
           mx = fmaxf(mx, fabsf(y));
         }
     }
     mx = group_max(mx, 64);
     if (l == 0) red[w] = mx;
-    __syncthreads();
+    lds_sync();
     float tmax = red[0];
 #pragma unroll
     for (int q = 1; q < NW2; q++) tmax = fmaxf(tmax, red[q]);
@@ -241,11 +262,11 @@ __global__ __launch_bounds__(NT2, 1) void k_gram_mx2(const float* __restrict__ X
         F[((w * NBLK + blk) * 2 + 0) * 64 + l] = hi;
         F[((w * NBLK + blk) * 2 + 1) * 64 + l] = lo;
       }
-    __syncthreads();
+    lds_sync();
     if (tile + 1 < t1) load(tile + 1);  // in flight during the products
 #pragma unroll
     for (int ks = 0; ks < NW2; ks++) gram_step<S>(F + ks * NBLK * 2 * 64, sI, sJ, sv, acc, l);
-    __syncthreads();
+    lds_sync();
   }
   const double inv = ldexp(1.0, e_run == EUN ? 0 : -2 * e_run);
   write_upper<NB, S>(acc, sI, sJ, sv, inv, part + int64_t(blockIdx.x) * NB * NB, l,
@@ -269,29 +290,37 @@ struct RowTile {
   static constexpr int FBYTES = NR * KSW * 2 * FRAGB;
 };
 
+// rows of one tile through a per-tile buffer resource: rows >= m read 0; the
+// columns >= n are masked in rows_split
 template <int NB, bool VEC>
 PC_DEV void rows_load(const float* __restrict__ X, int64_t ldx, int64_t m, int n, int64_t tile, f4* v) {
   using RT = RowTile<NB>;
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
   const int col = 4 * (l % RT::LPR);
+  const int64_t row0 = tile * RT::TR;
+  const int64_t nrow = m - row0 < RT::TR ? m - row0 : RT::TR;
+  const rsrc_t rs = make_rsrc(X + row0 * ldx, uint32_t(((nrow - 1) * ldx + n) * 4));
 #pragma unroll
   for (int i = 0; i < RT::NLD; i++) {
-    const int64_t row = tile * RT::TR + w * RT::RPW + i * RT::RPI + l / RT::LPR;
-    const bool rv = row < m;
-    const float* xr = X + (rv ? row : 0) * ldx;
-    if (VEC && rv && col + 4 <= n) {
-      v[i] = *reinterpret_cast<const f4*>(xr + col);
+    const int rt = w * RT::RPW + i * RT::RPI + l / RT::LPR;
+    const uint32_t off = uint32_t((rt * ldx + (col < n ? col : 0)) * 4);
+    if constexpr (VEC) {
+      typedef int i4 __attribute__((ext_vector_type(4)));
+      i4 q = __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0);
+      v[i] = __builtin_bit_cast(f4, q);
     } else {
 #pragma unroll
-      for (int u = 0; u < 4; u++) v[i][u] = (rv && col + u < n) ? xr[col + u] : 0.0f;
+      for (int u = 0; u < 4; u++)
+        v[i][u] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(
+                                                rs, uint32_t((rt * ldx + (col + u < n ? col + u : 0)) * 4), 0, 0));
     }
   }
 }
 
 // shift, per-row scale (rexp), split, store A fragments: Fs[((R * KSW + ks) * 2 + part)]
-template <int NB>
-PC_DEV void rows_split(f4* v, int64_t m, int n, int center, const double* mean, const double* var, int64_t tile,
-                       char* Fs, int* rexp) {
+template <int NB, int CENTER>
+PC_DEV void rows_split(f4* v, int64_t m, int n, const double* mean, const double* var, int64_t tile, char* Fs,
+                       int* rexp) {
   using RT = RowTile<NB>;
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
   const int j = l % RT::LPR, col = 4 * j;
@@ -301,10 +330,15 @@ PC_DEV void rows_split(f4* v, int64_t m, int n, int center, const double* mean, 
     const int64_t row = tile * RT::TR + rt;
     const bool rv = row < m;
     const float a = __shfl(v[i][0], (l / RT::LPR) * RT::LPR);  // x_i0 (lane holding column 0 of this row)
+    double mu = 0.0, iv = 1.0;
+    if constexpr (CENTER == 2) {
+      mu = mean[rv ? row : 0];
+      iv = 1.0 / var[rv ? row : 0];
+    }
     float mx = 0.0f;
 #pragma unroll
     for (int u = 0; u < 4; u++) {
-      float y = (rv && col + u < n) ? shifted(v[i][u], a, center, rv ? row : 0, mean, var) : 0.0f;
+      float y = (rv && col + u < n) ? shifted<CENTER>(v[i][u], a, mu, iv) : 0.0f;
       v[i][u] = y;
       mx = fmaxf(mx, fabsf(y));
     }
@@ -330,21 +364,23 @@ PC_DEV h8 frag(const char* Fs, int idx, int fragb, int l) { return *reinterpret_
 // ---------------------------------------------------------------------------
 // P3: C = Ys W
 // ---------------------------------------------------------------------------
-template <int NB, int NCB, bool VEC>
+template <int NB, bool VEC, int CENTER>
 __global__ __launch_bounds__(NT2, 1) void k_gemm_mx2(const float* __restrict__ X, int64_t ldx, int64_t m, int n,
-                                                     int center, const double* __restrict__ mean,
+                                                     const double* __restrict__ mean,
                                                      const double* __restrict__ var, const h8* __restrict__ Wh,
                                                      const h8* __restrict__ Wl, const float* __restrict__ wscale,
                                                      int nc, int64_t tpc, float* __restrict__ C, int64_t ldc) {
   using RT = RowTile<NB>;
-  constexpr int KSW = RT::KSW, NCBLK = NCB / 32;
+  constexpr int KSW = RT::KSW, NCBLK = NB / 32;  // output columns padded like the input
   __shared__ __align__(16) char Fs[RT::FBYTES];
   __shared__ int rexp[RT::TR];
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63, hl = l >> 5, c = l & 31;
   const int J = w;
+  // every wave owns a column block when NB = 256: keep that test out of the code
+  const bool own = NCBLK >= NW2 || J < NCBLK;
   h8 bh[KSW], bl[KSW];
   float isc = 1.0f;
-  if (J < NCBLK) {
+  if (own) {
 #pragma unroll
     for (int ks = 0; ks < KSW; ks++) {
       bh[ks] = Wh[int64_t(ks * NCBLK + J) * 64 + l];
@@ -357,12 +393,22 @@ __global__ __launch_bounds__(NT2, 1) void k_gemm_mx2(const float* __restrict__ X
   const int64_t t1 = t0 + tpc < ntiles ? t0 + tpc : ntiles;
   f4 v[RT::NLD];
   if (t0 < t1) rows_load<NB, VEC>(X, ldx, m, n, t0, v);
+  // the W fragments are loop invariant: settle every prologue load here so that
+  // no wait for them is placed inside the loop (it would also wait for the
+  // prefetch of the next tile)
+  __builtin_amdgcn_s_waitcnt(0);
   for (int64_t tile = t0; tile < t1; tile++) {
-    rows_split<NB>(v, m, n, center, mean, var, tile, Fs, rexp);
-    __syncthreads();
+    rows_split<NB, CENTER>(v, m, n, mean, var, tile, Fs, rexp);
+    lds_sync();
     if (tile + 1 < t1) rows_load<NB, VEC>(X, ldx, m, n, tile + 1, v);
-    if (J < NCBLK) {
+    if (own) {
       const int col = J * 32 + c;
+      // stores of this tile: rows >= m fall beyond num_records, columns >= nc get
+      // an offset beyond it; the hardware drops both (no branch, so the wait for
+      // the next tile's loads never has to cover these stores)
+      const int64_t row0 = tile * RT::TR;
+      const int64_t nrow = m - row0 < RT::TR ? m - row0 : RT::TR;
+      const rsrc_t cs = make_rsrc(C + row0 * ldc, uint32_t(((nrow - 1) * ldc + nc) * 4));
 #pragma unroll
       for (int R = 0; R < RT::NR; R++) {
         f16v acc = {};
@@ -377,21 +423,23 @@ __global__ __launch_bounds__(NT2, 1) void k_gemm_mx2(const float* __restrict__ X
 #pragma unroll
         for (int r = 0; r < 16; r++) {
           const int rt = 32 * R + 8 * (r >> 2) + 4 * hl + (r & 3);
-          const int64_t row = tile * RT::TR + rt;
-          if (row < m && col < nc) C[row * ldc + col] = ldexpf(acc[r], -rexp[rt]) * isc;
+          const uint32_t off = col < nc ? uint32_t((rt * ldc + col) * 4) : 0xFFFFFFFFu;
+          // (the builtin's data operand is a 32-bit integer: pass the float's bits)
+          __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, ldexpf(acc[r], -rexp[rt]) * isc), cs, off, 0,
+                                                0);
         }
       }
     }
-    __syncthreads();
+    lds_sync();
   }
 }
 
 // ---------------------------------------------------------------------------
 // P2: Z = Ys W1 (fp16 W1, per-column scale), G2 = Z^T Z
 // ---------------------------------------------------------------------------
-template <int NB, bool VEC>
+template <int NB, bool VEC, int CENTER>
 __global__ __launch_bounds__(NT2, 1) void k_zgram_mx2(const float* __restrict__ X, int64_t ldx, int64_t m, int n,
-                                                      int center, const double* __restrict__ mean,
+                                                      const double* __restrict__ mean,
                                                       const double* __restrict__ var, const h8* __restrict__ Wf,
                                                       const float* __restrict__ wscale, int64_t tpc,
                                                       double* __restrict__ part) {
@@ -406,9 +454,10 @@ __global__ __launch_bounds__(NT2, 1) void k_zgram_mx2(const float* __restrict__ 
   __shared__ float red[NW2];
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63, hl = l >> 5, c = l & 31;
   const int J = w;
+  const bool own = NBLK >= NW2 || J < NBLK;  // see k_gemm_mx2
   h8 bw[KSW];
   float isc = 1.0f;
-  if (J < NBLK) {
+  if (own) {
 #pragma unroll
     for (int ks = 0; ks < KSW; ks++) bw[ks] = Wf[int64_t(ks * NBLK + J) * 64 + l];
     isc = 1.0f / wscale[J * 32 + c];
@@ -426,16 +475,20 @@ __global__ __launch_bounds__(NT2, 1) void k_zgram_mx2(const float* __restrict__ 
   const int64_t t1 = t0 + tpc < ntiles ? t0 + tpc : ntiles;
   f4 v[RT::NLD];
   if (t0 < t1) rows_load<NB, VEC>(X, ldx, m, n, t0, v);
+  // the W fragments are loop invariant: settle every prologue load here so that
+  // no wait for them is placed inside the loop (it would also wait for the
+  // prefetch of the next tile)
+  __builtin_amdgcn_s_waitcnt(0);
   for (int64_t tile = t0; tile < t1; tile++) {
-    rows_split<NB>(v, m, n, center, mean, var, tile, Fs, rexp);
-    __syncthreads();
+    rows_split<NB, CENTER>(v, m, n, mean, var, tile, Fs, rexp);
+    lds_sync();
     if (tile + 1 < t1) rows_load<NB, VEC>(X, ldx, m, n, tile + 1, v);
     f16v z[RT::NR];
     float zmx = 0.0f;
 #pragma unroll
     for (int R = 0; R < RT::NR; R++) {
       z[R] = f16v{};
-      if (J < NBLK) {
+      if (own) {
 #pragma unroll
         for (int ks = 0; ks < KSW; ks++) {
           z[R] = mfma(frag(Fs, (R * KSW + ks) * 2 + 0, RT::FRAGB, l), bw[ks], z[R]);
@@ -452,13 +505,13 @@ __global__ __launch_bounds__(NT2, 1) void k_zgram_mx2(const float* __restrict__ 
     }
     zmx = group_max(zmx, 64);
     if (l == 0) red[w] = zmx;
-    __syncthreads();
+    lds_sync();
     float tmax = red[0];
 #pragma unroll
     for (int q = 1; q < NW2; q++) tmax = fmaxf(tmax, red[q]);
     e_run = update_scale<S>(tmax, e_run, acc);
     const float sc = ldexpf(1.0f, e_run == EUN ? 0 : e_run);
-    if (J < NBLK) {
+    if (own) {
       // D registers 0..7 / 8..15 of a 32-row block are two 16-row Gram k-steps
 #pragma unroll
       for (int R = 0; R < RT::NR; R++)
@@ -477,10 +530,10 @@ __global__ __launch_bounds__(NT2, 1) void k_zgram_mx2(const float* __restrict__ 
           F2[((ks * NBLK + J) * 2 + 1) * 64 + l] = lo;
         }
     }
-    __syncthreads();
+    lds_sync();
 #pragma unroll
     for (int ks = 0; ks < KSG; ks++) gram_step<S>(F2 + ks * NBLK * 2 * 64, sI, sJ, sv, acc, l);
-    __syncthreads();
+    lds_sync();
   }
   const double inv = ldexp(1.0, e_run == EUN ? 0 : -2 * e_run);
   write_upper<NB, S>(acc, sI, sJ, sv, inv, part + int64_t(blockIdx.x) * NB * NB, l,
@@ -495,8 +548,23 @@ int dispatch_nb(int n, F&& f) {
   if (n <= 256) return f(std::integral_constant<int, 256>{});
   return -1;
 }
+// (VEC, CENTER) as compile-time constants
+template <class F>
+int dispatch_vc(bool vec, int center, F&& f) {
+  if (center < 0 || center > 2) return -1;
+  auto c = [&](auto vv) {
+    if (center == 0) return f(vv, std::integral_constant<int, 0>{});
+    if (center == 1) return f(vv, std::integral_constant<int, 1>{});
+    return f(vv, std::integral_constant<int, 2>{});
+  };
+  return vec ? c(std::true_type{}) : c(std::false_type{});
+}
 
-bool vec_ok(const float* X, int64_t ldx) { return (reinterpret_cast<uintptr_t>(X) & 15) == 0 && (ldx & 3) == 0; }
+// float4 path: 16-byte aligned rows and n a multiple of 4 (a row's float4 is
+// then either wholly inside or wholly outside the n valid columns)
+bool vec_ok(const float* X, int64_t ldx, int n) {
+  return (reinterpret_cast<uintptr_t>(X) & 15) == 0 && (ldx & 3) == 0 && (n & 3) == 0;
+}
 
 }  // namespace
 
@@ -508,51 +576,39 @@ int mx2_tile_rows(int n) {
 
 int launch_gram_mx2(hipStream_t st, const float* X, int64_t ldx, int64_t m, int n, int center, double* mean_io,
                     const double* var, int chunks, int64_t tpc, double* part) {
-  const bool vec = vec_ok(X, ldx);
   return dispatch_nb(n, [&](auto nbc) {
     constexpr int NB = decltype(nbc)::value;
-    if (vec)
-      hipLaunchKernelGGL((k_gram_mx2<NB, true>), dim3(chunks), dim3(NT2), 0, st, X, ldx, m, n, center, mean_io, var,
-                         tpc, part);
-    else
-      hipLaunchKernelGGL((k_gram_mx2<NB, false>), dim3(chunks), dim3(NT2), 0, st, X, ldx, m, n, center, mean_io,
-                         var, tpc, part);
-    return 0;
+    return dispatch_vc(vec_ok(X, ldx, n), center, [&](auto vv, auto cc) {
+      hipLaunchKernelGGL((k_gram_mx2<NB, decltype(vv)::value, decltype(cc)::value>), dim3(chunks), dim3(NT2), 0, st,
+                         X, ldx, m, n, mean_io, var, tpc, part);
+      return 0;
+    });
   });
 }
 
 int launch_zgram_mx2(hipStream_t st, const float* X, int64_t ldx, int64_t m, int n, int center, const double* mean,
                      const double* var, const _Float16* Wf, const float* wscale, int chunks, int64_t tpc,
                      double* part) {
-  const bool vec = vec_ok(X, ldx);
   return dispatch_nb(n, [&](auto nbc) {
     constexpr int NB = decltype(nbc)::value;
-    if (vec)
-      hipLaunchKernelGGL((k_zgram_mx2<NB, true>), dim3(chunks), dim3(NT2), 0, st, X, ldx, m, n, center, mean, var,
-                         (const h8*)Wf, wscale, tpc, part);
-    else
-      hipLaunchKernelGGL((k_zgram_mx2<NB, false>), dim3(chunks), dim3(NT2), 0, st, X, ldx, m, n, center, mean, var,
-                         (const h8*)Wf, wscale, tpc, part);
-    return 0;
+    return dispatch_vc(vec_ok(X, ldx, n), center, [&](auto vv, auto cc) {
+      hipLaunchKernelGGL((k_zgram_mx2<NB, decltype(vv)::value, decltype(cc)::value>), dim3(chunks), dim3(NT2), 0, st,
+                         X, ldx, m, n, mean, var, (const h8*)Wf, wscale, tpc, part);
+      return 0;
+    });
   });
 }
 
+// the W operand is packed NB x NB (nc <= n output columns, padded like the input)
 int launch_gemm_mx2(hipStream_t st, const float* X, int64_t ldx, int64_t m, int n, int center, const double* mean,
                     const double* var, const _Float16* Wh, const _Float16* Wl, const float* wscale, int nc,
                     int chunks, int64_t tpc, float* C, int64_t ldc) {
-  const int ncb = mx2_nb(nc);
-  if (ncb < 0) return -1;
-  const bool vec = vec_ok(X, ldx);
+  if (mx2_nb(nc) < 0 || mx2_nb(nc) > mx2_nb(n)) return -1;
   return dispatch_nb(n, [&](auto nbc) {
     constexpr int NB = decltype(nbc)::value;
-    return dispatch_nb(ncb, [&](auto ncc) {
-      constexpr int NCB = decltype(ncc)::value;
-      if (vec)
-        hipLaunchKernelGGL((k_gemm_mx2<NB, NCB, true>), dim3(chunks), dim3(NT2), 0, st, X, ldx, m, n, center, mean,
-                           var, (const h8*)Wh, (const h8*)Wl, wscale, nc, tpc, C, ldc);
-      else
-        hipLaunchKernelGGL((k_gemm_mx2<NB, NCB, false>), dim3(chunks), dim3(NT2), 0, st, X, ldx, m, n, center, mean,
-                           var, (const h8*)Wh, (const h8*)Wl, wscale, nc, tpc, C, ldc);
+    return dispatch_vc(vec_ok(X, ldx, n), center, [&](auto vv, auto cc) {
+      hipLaunchKernelGGL((k_gemm_mx2<NB, decltype(vv)::value, decltype(cc)::value>), dim3(chunks), dim3(NT2), 0, st,
+                         X, ldx, m, n, mean, var, (const h8*)Wh, (const h8*)Wl, wscale, nc, tpc, C, ldc);
       return 0;
     });
   });

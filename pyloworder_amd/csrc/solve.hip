@@ -257,7 +257,9 @@ __global__ __launch_bounds__(256) void k_null_lev(const TS* X, int64_t ldx, int6
   __shared__ T yrow[512];
   for (int j = threadIdx.x; j < n; j += 256) {
     T v = load_elem<T, TS>(X, row * ldx, j);
-    if (center) {
+    if (center == 3) {
+      v = v - load_elem<T, TS>(X, row * ldx, 0);  // mixed path: rows shifted by their first element
+    } else if (center) {
       v = v - T(mean[row]);
       if (center == 2) v = v * (1.0 / var[row]);
     }
@@ -637,7 +639,7 @@ static int core_t(pc_ctx* ctx, const Plan& p, const void* A, int64_t lda, int ce
                      b.tmp, n, (const int*)nullptr, T(0.0), b.Wf, n, 0);
   // null completion (kernels return at once when nnull == 0)
   int ncand = int(std::min<int64_t>(p.m, 4 * int64_t(n) + 16));
-  launch_null_lev<T>(st, p, A, lda, center, mean, var, b, ncand);
+  launch_null_lev<T>(st, p, A, lda, (p.mixed && center == 1) ? 3 : center, mean, var, b, ncand);
   hipLaunchKernelGGL((k_null_build<T>), dim3(1), dim3(1024), 0, st, p.m, n, b.nnull, ncand, b.lev, b.urows, b.Wf, n,
                      b.jsel, b.cnull, nwork, b.status);
   // ---- P3
