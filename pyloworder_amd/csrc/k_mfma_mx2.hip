@@ -170,14 +170,16 @@ __global__ __launch_bounds__(NT2, 1) void k_gram_mx2(const float* __restrict__ X
                                                      const double* __restrict__ var, int64_t tpc,
                                                      double* __restrict__ part) {
   constexpr int V = NB >= 128 ? 4 : NB / 32;
-  constexpr int NG = NB / (32 * V);
+  constexpr int NG = NB / (32 * V);  // column groups of 32 V columns
   constexpr int NBLK = NB / 32;
-  constexpr int TR = 16 * NW2;
+  constexpr int KS = NW2 / NG;       // k-steps per tile: wave w loads k-step w % KS of group w / KS
+  constexpr int TR = 16 * KS;
   constexpr int NU = NBLK * (NBLK + 1) / 2;
   constexpr int S = (NU + NW2 - 1) / NW2;
-  __shared__ h8 F[NW2 * NBLK * 2 * 64];
+  __shared__ h8 F[KS * NBLK * 2 * 64];
   __shared__ float red[NW2];
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63, hl = l >> 5, c = l & 31;
+  const int wk = w % KS, wg = w / KS;
   int sI[S], sJ[S];
   bool sv[S];
 #pragma unroll
@@ -189,29 +191,32 @@ __global__ __launch_bounds__(NT2, 1) void k_gram_mx2(const float* __restrict__ X
   const int64_t ntiles = ceil_div(m, TR);
   const int64_t t0 = int64_t(blockIdx.x) * tpc;
   const int64_t t1 = t0 + tpc < ntiles ? t0 + tpc : ntiles;
-  float v[NG][8][V];
-  // branch-free loads: out-of-range lanes read a clamped, valid address; the
-  // value is masked where it is used, so no instruction waits on the load
-  // until the tile is processed (a branch or a select here would)
+  float v[8][V];
+  float a0[8];  // x_i0 of the 8 rows (CENTER 1)
+  // branch-free loads through a per-tile buffer resource (rows >= m read 0; the
+  // columns >= n are masked where the values are used, so no instruction waits
+  // on the loads until the tile is processed)
   auto load = [&](int64_t tile) {
+    const int64_t row0 = tile * TR;
+    const int64_t nrow = m - row0 < TR ? m - row0 : TR;
+    const rsrc_t rs = make_rsrc(X + row0 * ldx, uint32_t(((nrow - 1) * ldx + n) * 4));
+    const int col = wg * 32 * V + V * c;
 #pragma unroll
     for (int t = 0; t < 8; t++) {
-      const int64_t row = tile * TR + 16 * w + 8 * hl + t;
-      const bool rv = row < m;
-      const float* xr = X + (rv ? row : 0) * ldx;
+      const int rt = 16 * wk + 8 * hl + t;
+      if constexpr (VEC && V == 4) {
+        const uint32_t off = uint32_t((rt * ldx + (col < n ? col : 0)) * 4);
+        f4 q = __builtin_bit_cast(f4, __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0));
 #pragma unroll
-      for (int g = 0; g < NG; g++) {
-        const int col = g * 32 * V + V * c;
-        if constexpr (VEC && V == 4) {
-          const bool ok = rv && col < n;  // n % 4 == 0 on this path
-          f4 q = *reinterpret_cast<const f4*>(xr + (ok ? col : 0));
+        for (int u = 0; u < 4; u++) v[t][u] = q[u];
+      } else {
 #pragma unroll
-          for (int u = 0; u < 4; u++) v[g][t][u] = q[u];
-        } else {
-#pragma unroll
-          for (int u = 0; u < V; u++) v[g][t][u] = xr[(rv && col + u < n) ? col + u : 0];
-        }
+        for (int u = 0; u < V; u++)
+          v[t][u] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(
+                                                  rs, uint32_t((rt * ldx + (col + u < n ? col + u : 0)) * 4), 0, 0));
       }
+      if constexpr (CENTER == 1)
+        a0[t] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, uint32_t(rt * ldx * 4), 0, 0));
     }
   };
   if (t0 < t1) load(t0);
@@ -220,23 +225,21 @@ __global__ __launch_bounds__(NT2, 1) void k_gram_mx2(const float* __restrict__ X
     float mx = 0.0f;
 #pragma unroll
     for (int t = 0; t < 8; t++) {
-      const int64_t row = tile * TR + 16 * w + 8 * hl + t;
+      const int64_t row = tile * TR + 16 * wk + 8 * hl + t;
       const bool rv = row < m;
-      const float a = __shfl(v[0][t][0], hl * 32);  // x_i0 sits in lane c = 0 of this half
       double mu = 0.0, iv = 1.0;
       if constexpr (CENTER == 2) {
         mu = mean_io[rv ? row : 0];
         iv = 1.0 / var[rv ? row : 0];
       }
+      const float a = CENTER == 1 ? a0[t] : 0.0f;
 #pragma unroll
-      for (int g = 0; g < NG; g++)
-#pragma unroll
-        for (int u = 0; u < V; u++) {
-          const int col = g * 32 * V + V * c + u;
-          float y = (rv && col < n) ? shifted<CENTER>(v[g][t][u], a, mu, iv) : 0.0f;
-          v[g][t][u] = y;
-          mx = fmaxf(mx, fabsf(y));
-        }
+      for (int u = 0; u < V; u++) {
+        const int col = wg * 32 * V + V * c + u;
+        float y = (rv && col < n) ? shifted<CENTER>(v[t][u], a, mu, iv) : 0.0f;
+        v[t][u] = y;
+        mx = fmaxf(mx, fabsf(y));
+      }
     }
     mx = group_max(mx, 64);
     if (l == 0) red[w] = mx;
@@ -247,25 +250,23 @@ __global__ __launch_bounds__(NT2, 1) void k_gram_mx2(const float* __restrict__ X
     e_run = update_scale<S>(tmax, e_run, acc);
     const float sc = ldexpf(1.0f, e_run == EUN ? 0 : e_run);
 #pragma unroll
-    for (int g = 0; g < NG; g++)
+    for (int u = 0; u < V; u++) {
+      h8 hi, lo;
 #pragma unroll
-      for (int u = 0; u < V; u++) {
-        h8 hi, lo;
-#pragma unroll
-        for (int t = 0; t < 8; t++) {
-          _Float16 h, o;
-          split(v[g][t][u] * sc, h, o);
-          hi[t] = h;
-          lo[t] = o;
-        }
-        const int blk = g * V + u;
-        F[((w * NBLK + blk) * 2 + 0) * 64 + l] = hi;
-        F[((w * NBLK + blk) * 2 + 1) * 64 + l] = lo;
+      for (int t = 0; t < 8; t++) {
+        _Float16 h, o;
+        split(v[t][u] * sc, h, o);
+        hi[t] = h;
+        lo[t] = o;
       }
+      const int blk = wg * V + u;
+      F[((wk * NBLK + blk) * 2 + 0) * 64 + l] = hi;
+      F[((wk * NBLK + blk) * 2 + 1) * 64 + l] = lo;
+    }
     lds_sync();
-    if (tile + 1 < t1) load(tile + 1);  // in flight during the products
+    load(tile + 1 < t1 ? tile + 1 : t1 - 1);  // in flight during the products (clamped: straight-line code)
 #pragma unroll
-    for (int ks = 0; ks < NW2; ks++) gram_step<S>(F + ks * NBLK * 2 * 64, sI, sJ, sv, acc, l);
+    for (int ks = 0; ks < KS; ks++) gram_step<S>(F + ks * NBLK * 2 * 64, sI, sJ, sv, acc, l);
     lds_sync();
   }
   const double inv = ldexp(1.0, e_run == EUN ? 0 : -2 * e_run);
@@ -391,16 +392,22 @@ __global__ __launch_bounds__(NT2, 1) void k_gemm_mx2(const float* __restrict__ X
   const int64_t ntiles = ceil_div(m, RT::TR);
   const int64_t t0 = int64_t(blockIdx.x) * tpc;
   const int64_t t1 = t0 + tpc < ntiles ? t0 + tpc : ntiles;
-  f4 v[RT::NLD];
-  if (t0 < t1) rows_load<NB, VEC>(X, ldx, m, n, t0, v);
+  // two tiles in flight per wave: buffer va / vb alternate, and the load issued
+  // while a tile is processed is for the tile after next (always issued,
+  // clamped to the chunk's last tile, so the code stays straight-line)
+  f4 va[RT::NLD], vb[RT::NLD];
+  if (t0 < t1) {
+    rows_load<NB, VEC>(X, ldx, m, n, t0, va);
+    rows_load<NB, VEC>(X, ldx, m, n, t0 + 1 < t1 ? t0 + 1 : t0, vb);
+  }
   // the W fragments are loop invariant: settle every prologue load here so that
   // no wait for them is placed inside the loop (it would also wait for the
-  // prefetch of the next tile)
+  // prefetch of the next tiles)
   __builtin_amdgcn_s_waitcnt(0);
-  for (int64_t tile = t0; tile < t1; tile++) {
+  auto body = [&](int64_t tile, f4* v) {
     rows_split<NB, CENTER>(v, m, n, mean, var, tile, Fs, rexp);
     lds_sync();
-    if (tile + 1 < t1) rows_load<NB, VEC>(X, ldx, m, n, tile + 1, v);
+    rows_load<NB, VEC>(X, ldx, m, n, tile + 2 < t1 ? tile + 2 : t1 - 1, v);
     if (own) {
       const int col = J * 32 + c;
       // stores of this tile: rows >= m fall beyond num_records, columns >= nc get
@@ -431,7 +438,13 @@ __global__ __launch_bounds__(NT2, 1) void k_gemm_mx2(const float* __restrict__ X
       }
     }
     lds_sync();
+  };
+  int64_t tile = t0;
+  for (; tile + 1 < t1; tile += 2) {
+    body(tile, va);
+    body(tile + 1, vb);
   }
+  if (tile < t1) body(tile, va);
 }
 
 // ---------------------------------------------------------------------------
@@ -473,16 +486,18 @@ __global__ __launch_bounds__(NT2, 1) void k_zgram_mx2(const float* __restrict__ 
   const int64_t ntiles = ceil_div(m, RT::TR);
   const int64_t t0 = int64_t(blockIdx.x) * tpc;
   const int64_t t1 = t0 + tpc < ntiles ? t0 + tpc : ntiles;
-  f4 v[RT::NLD];
-  if (t0 < t1) rows_load<NB, VEC>(X, ldx, m, n, t0, v);
+  // one tile in flight (registers are taken by W1 and the Gram accumulators);
+  // the load is always issued, clamped to the chunk's last tile
+  f4 va[RT::NLD];
+  if (t0 < t1) rows_load<NB, VEC>(X, ldx, m, n, t0, va);
   // the W fragments are loop invariant: settle every prologue load here so that
   // no wait for them is placed inside the loop (it would also wait for the
-  // prefetch of the next tile)
+  // prefetch of the next tiles)
   __builtin_amdgcn_s_waitcnt(0);
-  for (int64_t tile = t0; tile < t1; tile++) {
+  auto body = [&](int64_t tile, f4* v) {
     rows_split<NB, CENTER>(v, m, n, mean, var, tile, Fs, rexp);
     lds_sync();
-    if (tile + 1 < t1) rows_load<NB, VEC>(X, ldx, m, n, tile + 1, v);
+    rows_load<NB, VEC>(X, ldx, m, n, tile + 1 < t1 ? tile + 1 : t1 - 1, v);
     f16v z[RT::NR];
     float zmx = 0.0f;
 #pragma unroll
@@ -534,7 +549,8 @@ __global__ __launch_bounds__(NT2, 1) void k_zgram_mx2(const float* __restrict__ 
 #pragma unroll
     for (int ks = 0; ks < KSG; ks++) gram_step<S>(F2 + ks * NBLK * 2 * 64, sI, sJ, sv, acc, l);
     lds_sync();
-  }
+  };
+  for (int64_t tile = t0; tile < t1; tile++) body(tile, va);
   const double inv = ldexp(1.0, e_run == EUN ? 0 : -2 * e_run);
   write_upper<NB, S>(acc, sI, sJ, sv, inv, part + int64_t(blockIdx.x) * NB * NB, l,
                      [](int blk, int pos) { return blk * 32 + pos; });
@@ -569,7 +585,12 @@ bool vec_ok(const float* X, int64_t ldx, int n) {
 }  // namespace
 
 int mx2_nb(int n) { return n <= 32 ? 32 : n <= 64 ? 64 : n <= 128 ? 128 : n <= 256 ? 256 : -1; }
-int mx2_gram_rows() { return 16 * NW2; }
+int mx2_gram_rows(int n) {
+  return dispatch_nb(n, [](auto nbc) {
+    constexpr int NB = decltype(nbc)::value, V = NB >= 128 ? 4 : NB / 32;
+    return 16 * (NW2 / (NB / (32 * V)));
+  });
+}
 int mx2_tile_rows(int n) {
   return dispatch_nb(n, [](auto nbc) { return RowTile<decltype(nbc)::value>::TR; });
 }

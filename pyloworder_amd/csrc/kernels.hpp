@@ -28,7 +28,7 @@ int launch_gemm_mx(hipStream_t st, const float* X, int64_t ldx, int64_t m, int n
                    int chunks, int64_t tpc, float* C, int64_t ldc);
 // second-generation mixed kernels (k_mfma_mx2.hip): 8 waves, W operand in registers
 int mx2_nb(int n);
-int mx2_gram_rows();
+int mx2_gram_rows(int n);
 int mx2_tile_rows(int n);
 int launch_gram_mx2(hipStream_t st, const float* X, int64_t ldx, int64_t m, int n, int center, double* mean_io,
                     const double* var, int chunks, int64_t tpc, double* part);

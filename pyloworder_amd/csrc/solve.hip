@@ -53,7 +53,22 @@ void launch_block_jacobi(pc_ctx* ctx, T* H, int ldh, int rows, int cols, T* J, i
   // relative orthogonality test |h_p^H h_q| <= tol ||h_p|| ||h_q||: the dot
   // product itself carries ~eps sqrt(rows) rounding, so tol grows with rows
   double tol = 2.220446049250313e-16 * std::max(rows, 16);
+  // Sweeps are queued in batches: after the first batch the host reads the
+  // done flag and stops queueing once the iteration has converged (typically
+  // after ~10 sweeps at n = 256).  Under stream capture no host read is
+  // possible, so every sweep is queued; converged sweeps then return at once.
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  (void)hipStreamIsCapturing(st, &cap);
+  const bool host_checks = cap == hipStreamCaptureStatusNone;
+  constexpr int kFirstBatch = 8, kNextBatch = 2;
+  int next_check = kFirstBatch;
   for (int s = 0; s < kBjacMaxSweeps; s++) {
+    if (host_checks && s == next_check) {
+      next_check += kNextBatch;
+      if (hipMemcpyAsync(ctx->hflag + 8, state, sizeof(int), hipMemcpyDeviceToHost, st) == hipSuccess &&
+          hipStreamSynchronize(st) == hipSuccess && ctx->hflag[8])
+        break;
+    }
     for (int step = 0; step < nb2 - 1; step++) {
       if constexpr (std::is_same<T, double>::value) {
         if (gram_steps) {
@@ -194,7 +209,7 @@ int make_plan(pc_ctx* ctx, pc_dtype dt, int64_t m, int64_t n, int precision, Pla
   p.m = m;
   if (p.mixed) {
     p.NB = mx2_nb(p.nr);
-    p.tr = mx2_gram_rows();
+    p.tr = mx2_gram_rows(p.nr);
     p.tr3 = mx2_tile_rows(p.nr);
   } else {
     p.NB = f64_nb(p.nr);
