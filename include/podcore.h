@@ -168,6 +168,18 @@ typedef struct {
 int pc_grouped_tsqr_svd(pc_ctx* ctx, pc_dtype dtype, const pc_problem* problems, int64_t count,
                         int precision);
 
+/* ---- SPOD front end --------------------------------------------------------
+ * Welch blocks of pyLOM/SPOD/wrapper.py:39-82: per-row mean removal (FP64
+ * means), Hamming window (pyLOM/vmmath/fft.py hammwin), real DFT of each block
+ * of nDFT samples with step nDFT-nolap, scaled by winWeight/nDFT (x2 for the
+ * interior frequencies) and 1/sqrt(nBlks).  X: M x N float32, rows are points,
+ * columns time samples.  Q receives nf = ceil(nDFT/2)+1 complex64 matrices of
+ * M x nBlks, frequency f at Q + f*M*nBlks, row-major (lda = nBlks): the
+ * operands of the per-frequency SVD (pc_grouped_tsqr_svd with PC_C64).
+ * Requires 2 <= nDFT <= min(N, 4096), 0 <= nolap < nDFT.                    */
+int pc_spod_blocks(pc_ctx* ctx, const float* X, int64_t ldx, int64_t M, int64_t N, int64_t nDFT,
+                   int64_t nolap, int remove_mean, void* Q);
+
 #ifdef __cplusplus
 }
 #endif

@@ -51,6 +51,7 @@ SIGNATURES = {
     "pc_diff_norms": (ip, [vp, ip, vp, i64, vp, i64, i64, i64, C.POINTER(C.c_double)]),
     "pc_truncation_residual": (ip, [C.POINTER(C.c_double), i64, C.c_double, C.POINTER(i64)]),
     "pc_grouped_tsqr_svd": (ip, [vp, ip, vp, i64, ip]),
+    "pc_spod_blocks": (ip, [vp, vp, i64, i64, i64, i64, i64, ip, vp]),
 }
 
 

@@ -81,6 +81,7 @@ struct pc_ctx {
   int64_t ws_skip = 0;    // bytes at the front of ws held by the calling entry point
   int* dflag = nullptr;   // device status words (error flags, ranks)
   int* hflag = nullptr;   // pinned host mirror
+  int* gstat = nullptr;   // grouped solve: where the current problem leaves its status words
   std::string err;
   // optional per-stage timing (pc_profile): events around each streaming pass
   int prof = 0;

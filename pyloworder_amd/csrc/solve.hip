@@ -492,6 +492,16 @@ __global__ __launch_bounds__(256) void k_unembed(const double* P, int NB, int n,
 // ---------------------------------------------------------------------------
 enum CoreMode { CORE_SVD = 0, CORE_QR = 1 };
 
+// status word of a solve -> error code and message
+static int status_error(pc_ctx* ctx, int s0) {
+  if (s0 & 1) return pc_fail(ctx, PC_ERR_LINALG, "non-finite values in the input (first Gram pass is not finite)");
+  if (s0 & 16) return pc_fail(ctx, PC_ERR_LINALG, "second Gram pass is not finite");
+  if (s0 & 2) return pc_fail(ctx, PC_ERR_LINALG, "Jacobi SVD did not converge");
+  if (s0 & 4) return pc_fail(ctx, PC_ERR_LINALG, "null-space completion failed (rows of high leverage)");
+  if (s0 & 8) return pc_fail(ctx, PC_ERR_LINALG, "non-finite singular values (overflow in the input scale?)");
+  return PC_OK;
+}
+
 struct CoreOut {
   void* U; int64_t ldu;    // SVD: U ; QR: Q
   void* S;                 // SVD
@@ -585,6 +595,10 @@ static int core_t(pc_ctx* ctx, const Plan& p, const void* A, int64_t lda, int ce
   {
     WsCursor probe{nullptr, 0, 0};
     int64_t need = carve<T>(probe, p, b, &mean, &var, &part, slabs) + dense_work(n) * int64_t(sizeof(T));
+    // inside a grouped call the status words live in the workspace: it must
+    // not move, so the group reserves the largest problem's need up front
+    if (ctx->gstat && base + need + 4096 > ctx->ws_bytes)
+      return pc_fail(ctx, PC_ERR_INVALID, "grouped workspace smaller than a member's need");
     PC_CHECK(ctx, pc_ws_reserve(ctx, base + need + 4096));
   }
   WsCursor cur{ctx->ws + base, 0, ctx->ws_bytes - base};
@@ -677,16 +691,14 @@ static int core_t(pc_ctx* ctx, const Plan& p, const void* A, int64_t lda, int ce
   hipLaunchKernelGGL((k_null_fix<T, typename VtType<T, ST>::type>), dim3(nblk(nn)), dim3(256), 0, st, b.nnull, p.m, n,
                      b.jsel, b.cnull, (typename VtType<T, ST>::type*)out.U, out.ldu);
   PC_HIP(ctx, hipGetLastError());
+  if (ctx->gstat) {  // grouped solve: statuses are read once for the whole group
+    PC_HIP(ctx, hipMemcpyAsync(ctx->gstat, b.status, 16, hipMemcpyDeviceToDevice, st));
+    return PC_OK;
+  }
   // status check: one small D2H copy, synchronous (errors must surface here)
   PC_HIP(ctx, hipMemcpyAsync(ctx->hflag, b.status, 16, hipMemcpyDeviceToHost, st));
   PC_HIP(ctx, hipStreamSynchronize(st));
-  int s0 = ctx->hflag[0];
-  if (s0 & 1) return pc_fail(ctx, PC_ERR_LINALG, "non-finite values in the input (first Gram pass is not finite)");
-  if (s0 & 16) return pc_fail(ctx, PC_ERR_LINALG, "second Gram pass is not finite");
-  if (s0 & 2) return pc_fail(ctx, PC_ERR_LINALG, "Jacobi SVD did not converge");
-  if (s0 & 4) return pc_fail(ctx, PC_ERR_LINALG, "null-space completion failed (rows of high leverage)");
-  if (s0 & 8) return pc_fail(ctx, PC_ERR_LINALG, "non-finite singular values (overflow in the input scale?)");
-  return PC_OK;
+  return status_error(ctx, ctx->hflag[0]);
 }
 
 static int core_dispatch(pc_ctx* ctx, pc_dtype dt, const void* A, int64_t lda, int64_t m, int64_t n, int center,

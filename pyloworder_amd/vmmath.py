@@ -24,7 +24,7 @@ from ._tensor import Arg, empty, pc_dtype, real_dtype
 from .cr import cr
 
 __all__ = [
-    "qr", "svd", "tsqr", "tsqr_svd", "randomized_qr", "randomized_svd", "matmul", "matmulp",
+    "qr", "svd", "tsqr", "tsqr_svd", "tsqr_svd_grouped", "randomized_qr", "randomized_svd", "matmul", "matmulp",
     "vecmat", "transpose", "temporal_mean", "subtract_mean", "temporal_variance", "norm_variance",
     "compute_truncation_residual", "RMSE", "energy", "reference_omega",
 ]
@@ -64,6 +64,45 @@ def tsqr_svd(Ai, precision: str = "fp64", center: int = 0):
     ctx.check(ctx.lib.pc_tsqr_svd(ctx.h, pc_dtype(a.t), a.ptr(), a.ld, m, n, int(center), U.data_ptr(), U.stride(0),
                                   S.data_ptr(), V.data_ptr(), V.stride(0), _prec(precision)))
     return a.give(U), a.give(S), a.give(V)
+
+
+@cr("math.tsqr_svd_grouped")
+def tsqr_svd_grouped(As, precision: str = "fp64", center: int = 0):
+    """tsqr_svd of every matrix of ``As`` in one grouped call (pc_grouped_tsqr_svd).
+
+    No single reference function: it replaces the per-frequency loop of
+    pyLOM/SPOD/wrapper.py:80-86 (one tsqr_svd per block) and any loop of
+    independent PODs.  ``As`` is a list of 2-D arrays/tensors of one dtype, or a
+    3-D tensor (count, m, n).  Returns lists (Us, Ss, Vs), each entry identical
+    to ``tsqr_svd`` of that matrix alone.
+    """
+    if isinstance(As, torch.Tensor) and As.dim() == 3:
+        args = [Arg(As[i], "Ai") for i in range(As.shape[0])]
+    else:
+        args = [Arg(A, "Ai") for A in As]
+    if not args:
+        return [], [], []
+    dt = args[0].t.dtype
+    for a in args:
+        if a.t.dtype != dt:
+            raise TypeError("tsqr_svd_grouped: all matrices must share one dtype")
+    Us, Ss, Vs = [], [], []
+    probs = (_lib.PCProblem * len(args))()
+    for i, a in enumerate(args):
+        m, n = a.t.shape
+        U = empty((m, n), dt, a.t)
+        S = empty((n,), real_dtype(dt), a.t)
+        V = empty((n, n), dt, a.t)
+        Us.append(U)
+        Ss.append(S)
+        Vs.append(V)
+        probs[i] = _lib.PCProblem(a.ptr(), a.ld, m, n, int(center), 0, U.data_ptr(), U.stride(0), S.data_ptr(),
+                                  V.data_ptr(), V.stride(0))
+    ctx = _ctx()
+    ctx.check(ctx.lib.pc_grouped_tsqr_svd(ctx.h, pc_dtype(args[0].t), C.cast(probs, C.c_void_p), len(args),
+                                          _prec(precision)))
+    return ([a.give(U) for a, U in zip(args, Us)], [a.give(S) for a, S in zip(args, Ss)],
+            [a.give(V) for a, V in zip(args, Vs)])
 
 
 @cr("math.tsqr")
