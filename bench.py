@@ -41,6 +41,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-rows", type=int, default=400_000)
     ap.add_argument("--mode", default="both", choices=["both", "mixed", "fp64"])
+    ap.add_argument("--workload", default="pod", choices=["pod", "grouped"],
+                    help="pod: tall-skinny SVD 8M x 256 (default); grouped: SPOD 200000 x 2048 "
+                         "(129 complex64 200000 x 15 blocks) + 64 uneven real POD problems")
     return ap.parse_args()
 
 
@@ -153,11 +156,137 @@ def cpu_baseline(m_sample, n, seed=7):
                       f"low-rank matrix, numpy/OpenBLAS threaded, 1 rank; {t:.2f} s"}
 
 
+def waves_device(m, n, seed=4, noise=1e-2):
+    """Six superposed travelling waves + N(0, noise), float32, on the device."""
+    import torch
+    rng = np.random.default_rng(seed)
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    x = torch.arange(m, device="cuda", dtype=torch.float64)[:, None] / m
+    t = torch.arange(n, device="cuda", dtype=torch.float64)[None, :] * 0.01
+    X = torch.zeros((m, n), device="cuda", dtype=torch.float32)
+    amp = [1.0, 0.7, 0.5, 0.35, 0.25, 0.15]
+    kap = [6.0, 11.0, 17.0, 23.0, 31.0, 40.0]
+    fr = [7.8125, 15.625, 23.4375, 31.25, 39.0625, 46.875]
+    for a_, k_, f_ in zip(amp, kap, fr):
+        X += (a_ * torch.sin(2 * np.pi * k_ * x - 2 * np.pi * f_ * t + rng.uniform(0, 2 * np.pi))).float()
+    X += noise * torch.randn((m, n), generator=g, device="cuda", dtype=torch.float32)
+    torch.cuda.synchronize()
+    return X, (np.arange(n) * 0.01)
+
+
+def uneven_problems(count, seed=77):
+    """count real float32 problems: m log-uniform in [5e4, 2e6], n uniform in
+    [16, 256], low-rank + noise with decay U(0.9, 0.97)."""
+    import torch
+    rng = np.random.default_rng(seed)
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    out = []
+    for _ in range(count):
+        m = int(round(np.exp(rng.uniform(np.log(5e4), np.log(2e6)))))
+        n = int(rng.integers(16, 257))
+        d = rng.uniform(0.9, 0.97)
+        G2 = torch.linalg.qr(torch.randn(n, n, generator=g, device="cuda", dtype=torch.float64))[0]
+        s = 1e3 * d ** torch.arange(n, device="cuda", dtype=torch.float64)
+        G1 = torch.randn(m, n, generator=g, device="cuda", dtype=torch.float64) / np.sqrt(m)
+        X = (G1 @ (s[:, None] * G2.T)).float()
+        X += 1e-6 * torch.randn(m, n, generator=g, device="cuda", dtype=torch.float32)
+        out.append(X)
+        del G1
+    torch.cuda.synchronize()
+    return out
+
+
+def run_grouped(a):
+    """SPOD (nDFT 256, 50% overlap) of 200000 x 2048 float32 + 64 uneven PODs."""
+    import ctypes as C
+    import torch
+    import pyloworder_amd as pl
+    from pyloworder_amd import _lib
+    ctx = _lib.context()
+    M, N, nDFT, nolap = 200_000, 2048, 256, 128
+    log(f"[bench] grouped: generating SPOD input {M} x {N} and 64 uneven problems")
+    X, t = waves_device(M, N)
+    probs_X = uneven_problems(64)
+    nf, nB = nDFT // 2 + 1, (N - nolap) // (nDFT - nolap)
+    spod_bytes = nf * 3 * M * nB * 8
+    real_bytes = sum(3 * P.shape[0] * P.shape[1] * 4 for P in probs_X)
+    res = {}
+    sync = torch.cuda.synchronize
+    for prec in ("mixed", "fp64"):
+        # full SPOD.run (blocks + grouped per-frequency SVDs + sort/P layout)
+        for _ in range(a.warmup):
+            pl.SPOD.run(X, t, nDFT, nolap, precision=prec)
+        sync()
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            L, P, f = pl.SPOD.run(X, t, nDFT, nolap, precision=prec)
+        sync()
+        t_spod = (time.perf_counter() - t0) / a.steps
+        del L, P, f
+        # the SVD stage alone: one grouped call over the 129 blocks
+        Q, _, _ = pl.SPOD.blocks(X, t, nDFT, nolap)
+        sync()
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            Us, Ss, Vs = pl.math.tsqr_svd_grouped(Q, precision=prec)
+        sync()
+        t_blocksvd = (time.perf_counter() - t0) / a.steps
+        del Us, Ss, Vs, Q
+        # 64 real problems (row centring fused) in one grouped call, outputs preallocated
+        outs = [(torch.empty_like(P_), torch.empty(P_.shape[1], device="cuda"),
+                 torch.empty((P_.shape[1], P_.shape[1]), device="cuda")) for P_ in probs_X]
+        arr = (_lib.PCProblem * len(probs_X))()
+        for i, (P_, (U_, S_, V_)) in enumerate(zip(probs_X, outs)):
+            arr[i] = _lib.PCProblem(P_.data_ptr(), P_.stride(0), P_.shape[0], P_.shape[1], 1, 0, U_.data_ptr(),
+                                    U_.stride(0), S_.data_ptr(), V_.data_ptr(), V_.stride(0))
+        pc = _lib.PC_PREC_MIXED if prec == "mixed" else _lib.PC_PREC_FP64
+
+        def grouped():
+            ctx.check(ctx.lib.pc_grouped_tsqr_svd(ctx.h, _lib.PC_F32, C.cast(arr, C.c_void_p), len(probs_X), pc))
+        for _ in range(a.warmup):
+            grouped()
+        sync()
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            grouped()
+        sync()
+        t_real = (time.perf_counter() - t0) / a.steps
+        # the same 64 problems as separate calls, for comparison
+        t0 = time.perf_counter()
+        for P_, (U_, S_, V_) in zip(probs_X, outs):
+            ctx.check(ctx.lib.pc_tsqr_svd(ctx.h, _lib.PC_F32, P_.data_ptr(), P_.stride(0), P_.shape[0], P_.shape[1],
+                                          1, U_.data_ptr(), U_.stride(0), S_.data_ptr(), V_.data_ptr(),
+                                          V_.stride(0), pc))
+        sync()
+        t_single = time.perf_counter() - t0
+        del outs
+        tot = t_spod + t_real
+        res[prec] = {"spod_run_ms": round(t_spod * 1e3, 2), "spod_svd_stage_ms": round(t_blocksvd * 1e3, 2),
+                     "spod_svd_stage_gbs": round(spod_bytes / t_blocksvd / 1e9, 1),
+                     "real64_grouped_ms": round(t_real * 1e3, 2), "real64_separate_calls_ms": round(t_single * 1e3, 2),
+                     "real64_gbs": round(real_bytes / t_real / 1e9, 1),
+                     "solves_per_s": round((nf + len(probs_X)) / tot, 1),
+                     "gbs": round((spod_bytes + real_bytes) / tot / 1e9, 1)}
+        log(f"[bench] grouped {prec}: {res[prec]}")
+    out = {"metric": "grouped SPOD + uneven POD solves/s (cfg: 129 x 200000x15 c64 + 64 real)",
+           "value": res["mixed"]["solves_per_s"], "unit": "solves/s", "n_gpus": 1, "steps": a.steps,
+           "warmup": a.warmup, "ms_per_step": round(1e3 * (nf + 64) / res["mixed"]["solves_per_s"], 2),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "c64/f32 storage, fp16x2 MFMA / FP64 small", "data": "synthetic (device-generated)",
+           "config": {"workload": "SPOD.run nDFT=256 nolap=128 on 200000 x 2048 float32 + 64 real POD problems "
+                                  "m log-U[5e4, 2e6], n U[16, 256]", "nf": nf, "nBlks": nB,
+                      "algorithmic_GB": round((spod_bytes + real_bytes) / 1e9, 2)},
+           "mixed": res["mixed"], "fp64": res["fp64"]}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     a = parse()
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (see DESIGN.md for sharding)"}))
         sys.exit(2)
+    if a.workload == "grouped":
+        return run_grouped(a)
     import torch
     import pyloworder_amd  # noqa: F401  (fails loudly if libpodcore.so is missing)
     m, n = a.m, a.n

@@ -82,6 +82,11 @@ struct pc_ctx {
   int* dflag = nullptr;   // device status words (error flags, ranks)
   int* hflag = nullptr;   // pinned host mirror
   int* gstat = nullptr;   // grouped solve: where the current problem leaves its status words
+  // grouped solve: internal streams (default priority) the problems are spread
+  // over, and the fork / join events; created on first use
+  static constexpr int kGroupStreams = 4;
+  hipStream_t gs[kGroupStreams] = {nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t gev[kGroupStreams + 1] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   std::string err;
   // optional per-stage timing (pc_profile): events around each streaming pass
   int prof = 0;

@@ -43,6 +43,14 @@ extern "C" int pc_ctx_destroy(pc_ctx* ctx) {
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   else (void)hipDeviceSynchronize();
+  for (int i = 0; i < pc_ctx::kGroupStreams; i++) {
+    if (ctx->gs[i]) {
+      (void)hipStreamSynchronize(ctx->gs[i]);
+      (void)hipStreamDestroy(ctx->gs[i]);
+    }
+  }
+  for (hipEvent_t e : ctx->gev)
+    if (e) (void)hipEventDestroy(e);
   if (ctx->ws) (void)hipFree(ctx->ws);
   if (ctx->dflag) (void)hipFree(ctx->dflag);
   if (ctx->hflag) (void)hipHostFree(ctx->hflag);

@@ -63,6 +63,42 @@ PC_DEV double block_max(double v, double* sh) {
 //   Re G_ab = P[2a][2b] + P[2a+1][2b+1], Im G_ab = P[2a][2b+1] - P[2a+1][2b].
 // Summation over chunks is in a fixed order (deterministic).
 // ---------------------------------------------------------------------------
+// Chunk sum of the partial slabs, in place into chunk 0, for the entries
+// i <= j < nr of the NB x NB slab.  A workgroup owns 64 consecutive entries
+// and 16 chunk lanes; lane k sums chunks k, k+16, ... in four interleaved
+// accumulators combined in a fixed order, then lane 0 adds the 16 lane sums in
+// order: the summation order depends only on (chunks, NB), so results are
+// deterministic.  Entry e is read and written only by its own workgroup.
+__global__ __launch_bounds__(1024) void k_reduce_slab(double* P, int chunks, int64_t pstride, int NB, int nr) {
+  __shared__ double red[16][65];
+  const int le = threadIdx.x & 63, k = threadIdx.x >> 6;
+  const int e = blockIdx.x * 64 + le;
+  const int i = e / NB, j = e - i * NB;
+  const bool use = e < NB * NB && i <= j && j < nr;
+  double s = 0.0;
+  if (use) {
+    const double* p = P + e;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int c = k;
+    for (; c + 48 < chunks; c += 64) {
+      s0 += p[int64_t(c) * pstride];
+      s1 += p[int64_t(c + 16) * pstride];
+      s2 += p[int64_t(c + 32) * pstride];
+      s3 += p[int64_t(c + 48) * pstride];
+    }
+    for (; c < chunks; c += 16) s0 += p[int64_t(c) * pstride];
+    s = (s0 + s1) + (s2 + s3);
+  }
+  red[k][le] = s;
+  __syncthreads();
+  if (k == 0 && use) {
+    double t = 0.0;
+#pragma unroll
+    for (int q = 0; q < 16; q++) t += red[q][le];
+    P[e] = t;
+  }
+}
+
 __global__ __launch_bounds__(256) void k_reduce_gram_real(const double* __restrict__ P, int chunks, int64_t pstride,
                                                          int NB, int n, double* G, int ldg) {
   int64_t idx = int64_t(blockIdx.x) * 256 + threadIdx.x;
@@ -91,7 +127,7 @@ __global__ __launch_bounds__(256) void k_reduce_gram_cplx(const double* __restri
     const double* p = P + c * pstride;
     rr += p[o_rr]; ii += p[o_ii]; ri += p[o_ri]; ir += p[o_ir];
   }
-  cdouble g(rr + ii, ri - ir);
+  cdouble g(rr + ii, a == b ? 0.0 : ri - ir);  // the diagonal is real
   G[int64_t(a) * ldg + b] = g;
   G[int64_t(b) * ldg + a] = conj_(g);
 }

@@ -1,6 +1,8 @@
 // Orchestration of the POD core: tall-skinny SVD / QR pipelines, randomized
 // SVD, products and grouped solves.  All device work is queued on the
-// context stream; no host synchronisation happens inside a solve.
+// context stream; the only host waits inside a solve are the convergence
+// checks between batches of block-Jacobi sweeps (several block pairs only,
+// skipped under stream capture) and the final status read.
 //
 // Tall-skinny SVD of Y (m x n, optionally row-centred), "two-pass
 // Gram-Cholesky" (replaces Demmel TSQR + svd(R) of pyLOM/vmmath/svd.py:49-252):
@@ -49,7 +51,11 @@ void launch_block_jacobi(pc_ctx* ctx, T* H, int ldh, int rows, int cols, T* J, i
   if (nb2 < 2) nb2 = 2;
   hipLaunchKernelGGL((k_bjac_init<T>), dim3(1), dim3(1024), 0, st, H, ldh, rows, cols, J, ldj, state, dstate,
                      kBjacMaxSweeps + 1);
-  int max_inner = gram_steps ? (nb2 == 2 ? 8 : 1) : (nb2 == 2 ? 60 : 3);
+  // one block pair (all columns in one workgroup): each launch iterates to
+  // convergence on its own, so a short fixed run of sweeps is queued and the
+  // host never waits; several pairs: sweeps in host-checked batches
+  const bool one_pair = nb2 == 2;
+  int max_inner = one_pair ? 60 : gram_steps ? 1 : 3;
   // relative orthogonality test |h_p^H h_q| <= tol ||h_p|| ||h_q||: the dot
   // product itself carries ~eps sqrt(rows) rounding, so tol grows with rows
   double tol = 2.220446049250313e-16 * std::max(rows, 16);
@@ -59,10 +65,11 @@ void launch_block_jacobi(pc_ctx* ctx, T* H, int ldh, int rows, int cols, T* J, i
   // possible, so every sweep is queued; converged sweeps then return at once.
   hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
   (void)hipStreamIsCapturing(st, &cap);
-  const bool host_checks = cap == hipStreamCaptureStatusNone;
-  constexpr int kFirstBatch = 8, kNextBatch = 2;
+  const bool host_checks = cap == hipStreamCaptureStatusNone && !one_pair;
+  constexpr int kFirstBatch = 8, kNextBatch = 2, kOnePairSweeps = 6;
   int next_check = kFirstBatch;
-  for (int s = 0; s < kBjacMaxSweeps; s++) {
+  const int sweeps = one_pair ? kOnePairSweeps : kBjacMaxSweeps;
+  for (int s = 0; s < sweeps; s++) {
     if (host_checks && s == next_check) {
       next_check += kNextBatch;
       if (hipMemcpyAsync(ctx->hflag + 8, state, sizeof(int), hipMemcpyDeviceToHost, st) == hipSuccess &&
@@ -540,14 +547,23 @@ static int launch_streams(pc_ctx* ctx, const Plan& p, const void* A, int64_t lda
 
 template <class T>
 static void reduce_gram(hipStream_t st, const Plan& p, const double* part, int nparts, T* G);
+// the chunk partials are summed in place into chunk 0 (k_reduce_slab), then
+// chunk 0 is unpacked into the symmetric / Hermitian n x n Gram
+static void reduce_slab(hipStream_t st, const Plan& p, const double* part, int nparts) {
+  const int64_t e = int64_t(p.NB) * p.NB;
+  hipLaunchKernelGGL(k_reduce_slab, dim3(unsigned(ceil_div(e, 64))), dim3(1024), 0, st, const_cast<double*>(part),
+                     nparts, e, p.NB, p.nr);
+}
 template <>
 void reduce_gram<double>(hipStream_t st, const Plan& p, const double* part, int nparts, double* G) {
-  hipLaunchKernelGGL(k_reduce_gram_real, dim3(nblk(int64_t(p.n) * p.n)), dim3(256), 0, st, part, nparts,
+  reduce_slab(st, p, part, nparts);
+  hipLaunchKernelGGL(k_reduce_gram_real, dim3(nblk(int64_t(p.n) * p.n)), dim3(256), 0, st, part, 1,
                      int64_t(p.NB) * p.NB, p.NB, p.n, G, p.n);
 }
 template <>
 void reduce_gram<cdouble>(hipStream_t st, const Plan& p, const double* part, int nparts, cdouble* G) {
-  hipLaunchKernelGGL(k_reduce_gram_cplx, dim3(nblk(int64_t(p.n) * p.n)), dim3(256), 0, st, part, nparts,
+  reduce_slab(st, p, part, nparts);
+  hipLaunchKernelGGL(k_reduce_gram_cplx, dim3(nblk(int64_t(p.n) * p.n)), dim3(256), 0, st, part, 1,
                      int64_t(p.NB) * p.NB, p.NB, p.n, G, p.n);
 }
 
