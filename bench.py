@@ -41,9 +41,13 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-rows", type=int, default=400_000)
     ap.add_argument("--mode", default="both", choices=["both", "mixed", "fp64"])
-    ap.add_argument("--workload", default="pod", choices=["pod", "grouped"],
+    ap.add_argument("--rate", type=float, default=200.0, help="serve: Poisson arrival rate, requests/s")
+    ap.add_argument("--duration", type=float, default=20.0, help="serve: arrival window, s")
+    ap.add_argument("--slots", type=int, default=4, help="serve: requests in flight")
+    ap.add_argument("--workload", default="pod", choices=["pod", "grouped", "serve"],
                     help="pod: tall-skinny SVD 8M x 256 (default); grouped: SPOD 200000 x 2048 "
-                         "(129 complex64 200000 x 15 blocks) + 64 uneven real POD problems")
+                         "(129 complex64 200000 x 15 blocks) + 64 uneven real POD problems; serve: "
+                         "Poisson stream of POD / project / reconstruct requests")
     return ap.parse_args()
 
 
@@ -280,6 +284,122 @@ def run_grouped(a):
     print(json.dumps(out), flush=True)
 
 
+def pool_device(m, n, decay, seed):
+    """Low-rank + noise float32 pool (as the 8M x 256 workload, decay given)."""
+    import torch
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    G2 = torch.linalg.qr(torch.randn(n, n, generator=g, device="cuda", dtype=torch.float64))[0]
+    s = 1e3 * decay ** torch.arange(n, device="cuda", dtype=torch.float64)
+    B = (s[:, None] * G2.T).contiguous()
+    X = torch.empty((m, n), device="cuda", dtype=torch.float32)
+    step = 1 << 20
+    for r0 in range(0, m, step):
+        r1 = min(m, r0 + step)
+        G1 = torch.randn(r1 - r0, n, generator=g, device="cuda", dtype=torch.float64) / np.sqrt(m)
+        X[r0:r1] = (G1 @ B).float() + 1e-6 * torch.randn(r1 - r0, n, generator=g, device="cuda", dtype=torch.float32)
+    torch.cuda.synchronize()
+    return X
+
+
+def run_serve(a):
+    """Poisson arrivals of POD solves (50 %), projections (25 %) and
+    reconstructions (25 %) on resident bases; m log-uniform in [1e4, 4e6],
+    n log-uniform in [8, 256]; float32 mixed path; 200 GB admission budget."""
+    import threading
+    import torch
+    from pyloworder_amd import serve
+    from pyloworder_amd.serve import POD, PROJECT, RECONSTRUCT, Request, Server
+    M_MAX, N_MAX = 4_000_000, 256
+    log(f"[bench] serve: generating 3 input pools {M_MAX} x {N_MAX} float32")
+    pools = [pool_device(M_MAX, N_MAX, d, 100 + i) for i, d in enumerate((0.9, 0.935, 0.97))]
+    rng = np.random.default_rng(12)
+    ex = serve.DeviceExecutor("mixed")
+    reserve = ex.workspace_bound(M_MAX, N_MAX)
+    algo = {"bytes": 0}
+    lock = threading.Lock()
+
+    def on_done(r):
+        if r.error is not None:
+            return
+        if r.kind == POD:
+            m, n = r.X.shape
+            b = 3 * m * n * 4
+        elif r.kind == PROJECT:
+            m, n = r.X.shape
+            b = (m * r.modes + m * n + r.modes * n) * 4
+        else:
+            U = r.result
+            m, n = U.shape
+            b = (m * r.modes + r.modes * n + m * n) * 4
+        with lock:
+            algo["bytes"] += b
+    srv = Server(ex, slots=a.slots, budget=200e9, reserve_per_slot=reserve, keep_results=False, on_done=on_done)
+
+    def logu(lo, hi):
+        return float(np.exp(rng.uniform(np.log(lo), np.log(hi))))
+
+    def make(kind):
+        if kind != POD:
+            ids = srv.resident_ids()
+            if ids:
+                bid = ids[int(rng.integers(len(ids)))]
+                b = srv.bases.get(bid)
+                if b is not None:
+                    m, N = b.U.shape
+                    k = int(min(N, rng.integers(4, 65)))
+                    if kind == PROJECT:
+                        n2 = int(round(logu(8, N_MAX)))
+                        return Request(PROJECT, X=pools[int(rng.integers(3))][:m, :n2], basis=bid, modes=k)
+                    return Request(RECONSTRUCT, basis=bid, modes=k)
+        m = int(round(logu(1e4, M_MAX)))
+        n = int(round(logu(8, N_MAX)))
+        n = min(n, m)
+        return Request(POD, X=pools[int(rng.integers(3))][:m, :n])
+
+    # warm-up: allocate every slot's workspace and the caching allocator's pools
+    for _ in range(2 * a.slots):
+        srv.submit(Request(POD, X=pools[0][:M_MAX, :N_MAX]))
+    srv.drain()
+    srv.completed.clear()
+    algo["bytes"] = 0
+    arrivals = []
+    t = 0.0
+    while True:
+        t += rng.exponential(1.0 / a.rate)
+        if t > a.duration:
+            break
+        arrivals.append(t)
+    kinds = rng.choice([POD, PROJECT, RECONSTRUCT], size=len(arrivals), p=[0.5, 0.25, 0.25])
+    log(f"[bench] serve: {len(arrivals)} requests over {a.duration} s at {a.rate}/s, {a.slots} slots")
+    t0 = time.perf_counter()
+    late = 0
+    for ta, kind in zip(arrivals, kinds):
+        now = time.perf_counter() - t0
+        if ta > now:
+            time.sleep(ta - now)
+        elif now - ta > 0.01:
+            late += 1
+        r = make(str(kind))
+        r.arrival = t0 + ta
+        srv.submit(r)
+    srv.drain()
+    wall = max(r.done for r in srv.completed) - t0
+    srv.close()
+    summ = serve.summary(srv, wall)
+    gbs = algo["bytes"] / wall / 1e9
+    out = {"metric": "serving: solves/s and p50/p99 latency (POD / project / reconstruct stream)",
+           "value": summ["solves_per_s"], "unit": "solves/s", "n_gpus": 1, "steps": len(arrivals), "warmup": 2 * a.slots,
+           "ms_per_step": round(wall * 1e3 / max(1, summ["requests"]), 3), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32 storage, fp16x2 MFMA / FP64 small", "data": "synthetic (device-generated)",
+           "config": {"workload": "Poisson arrivals, 50% POD.run / 25% project / 25% reconstruct on resident bases, "
+                                  "m log-U[1e4, 4e6], n log-U[8, 256], decay in {0.9, 0.935, 0.97}, float32 mixed",
+                      "rate_per_s": a.rate, "duration_s": a.duration, "slots": a.slots, "budget_GB": 200,
+                      "workspace_reserve_per_slot_GB": round(reserve / 1e9, 3)},
+           "p50_ms": summ["p50_ms"], "p99_ms": summ["p99_ms"], "serve": summ,
+           "algorithmic_gbs": round(gbs, 1), "late_submissions": late}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -287,6 +407,8 @@ def main():
         sys.exit(2)
     if a.workload == "grouped":
         return run_grouped(a)
+    if a.workload == "serve":
+        return run_serve(a)
     import torch
     import pyloworder_amd  # noqa: F401  (fails loudly if libpodcore.so is missing)
     m, n = a.m, a.n

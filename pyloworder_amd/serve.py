@@ -1,0 +1,349 @@
+"""Serving loop for POD requests on one MI355X: continuous batching under an
+HBM admission budget, with resident bases.
+
+No reference counterpart: pyLOM runs one POD per script.  The requests reuse
+the library's entry points (POD.run = tsqr_svd with fused centring,
+POD.project, POD.reconstruct), so every result is the one those calls give.
+
+  * A request is admitted as soon as a slot is idle and its device bytes fit
+    the budget: requests that arrive while others run join at once (continuous
+    batching), not after the current batch drains.  Admission is FIFO with a
+    bounded skip-ahead so that a large request at the head does not idle the
+    device, and it evicts least-recently-used idle bases to make room.
+  * Each slot is a worker thread with its own HIP stream and library context
+    (own workspace), so admitted requests overlap on the device.  The library
+    calls release the GIL (ctypes) and every kernel is deterministic, so a
+    result does not depend on what else is running.
+  * POD results stay resident as bases (U, S, V) for later project /
+    reconstruct requests, counted against the budget until evicted.
+
+The scheduling logic is independent of the device: ``Server`` takes an
+executor with ``bytes_needed`` / ``run``; ``DeviceExecutor`` is the MI355X
+one.
+"""
+from __future__ import annotations
+
+import collections
+import dataclasses
+import threading
+import time
+from typing import Any, Callable, Dict, List, Optional
+
+POD, PROJECT, RECONSTRUCT = "pod", "project", "reconstruct"
+
+
+@dataclasses.dataclass
+class Request:
+    """One unit of work.  ``X``: input rows x columns (pod, project);
+    ``basis``: id of a resident basis (project, reconstruct); ``modes``: number
+    of leading modes used by project / reconstruct (0 = all)."""
+    kind: str
+    X: Any = None
+    basis: Optional[int] = None
+    modes: int = 0
+    arrival: float = 0.0
+    rid: int = -1
+    # filled by the server
+    admitted: float = 0.0
+    done: float = 0.0
+    result: Any = None
+    error: Optional[BaseException] = None
+    nbytes: int = 0
+    slot: int = -1
+
+    @property
+    def latency(self) -> float:
+        return self.done - self.arrival
+
+
+@dataclasses.dataclass
+class Basis:
+    U: Any
+    S: Any
+    V: Any
+    nbytes: int
+    last_used: float = 0.0
+    pins: int = 0
+
+
+class DeviceExecutor:
+    """Runs requests with the library on the calling thread's current stream."""
+
+    def __init__(self, precision: str = "mixed", remove_mean: bool = True):
+        from . import _lib
+        self.precision = precision
+        self.remove_mean = remove_mean
+        self._lib = _lib
+        self._prec = _lib.PC_PREC_MIXED if precision == "mixed" else _lib.PC_PREC_FP64
+
+    @staticmethod
+    def _esz(X) -> int:
+        return X.element_size()
+
+    def workspace_bound(self, m_max: int, n_max: int, dtype=None) -> int:
+        """Workspace one slot's context may hold: that of the largest solve."""
+        from ._lib import PC_F32
+        lib = self._lib.load()
+        return int(max(lib.pc_workspace_query(0, PC_F32 if dtype is None else dtype, m_max, n_max, 0, self._prec), 0))
+
+    def bytes_needed(self, req: Request, basis: Optional[Basis]) -> int:
+        """Device bytes of the request's outputs (the slot's workspace is
+        reserved separately); a POD solve's outputs stay resident afterwards."""
+        if req.kind == POD:
+            m, n = req.X.shape
+            e = self._esz(req.X)
+            return int(m * n * e + n * n * e + n * 8)
+        U = basis.U
+        m, N = U.shape
+        k = req.modes or N
+        e = U.element_size()
+        if req.kind == PROJECT:
+            n = req.X.shape[1]
+            return int(k * n * e + 2 * k * n * 8 + (1 << 20))
+        n = basis.V.shape[1]
+        return int(m * n * e)
+
+    def _dtype(self, X):
+        from ._tensor import pc_dtype
+        return pc_dtype(X)
+
+    def run(self, req: Request, basis: Optional[Basis]):
+        from . import POD as P
+        if req.kind == POD:
+            return P.run(req.X, remove_mean=self.remove_mean, precision=self.precision)
+        k = req.modes or basis.U.shape[1]
+        if req.kind == PROJECT:
+            return P.project(basis.U[:, :k], req.X, precision=self.precision)
+        return P.reconstruct(basis.U[:, :k], basis.S[:k], basis.V[:k, :], precision=self.precision)
+
+    def make_stream(self):
+        import torch
+        return torch.cuda.Stream()
+
+    def stream_context(self, stream):
+        import torch
+        return torch.cuda.stream(stream)
+
+    def synchronize(self, stream):
+        stream.synchronize()
+
+    def resident_bytes(self, result) -> int:
+        U, S, V = result
+        return int(U.numel() * U.element_size() + S.numel() * S.element_size() + V.numel() * V.element_size())
+
+
+class _NullCtx:
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        return False
+
+
+class Server:
+    """Continuous-batching request loop.
+
+    ``slots``: requests in flight at once (one worker thread and stream each);
+    ``budget``: device bytes for in-flight requests, resident bases and
+    ``reserve_per_slot`` bytes of workspace per slot;
+    ``skip_ahead``: how many queued requests behind a head that does not fit
+    may be admitted before it; ``keep_bases``: keep POD results resident;
+    ``keep_results``: keep each result on its request after ``on_done`` (a
+    callback run on completion) has seen it — off for long runs, so that the
+    device memory of results and evicted bases is released.
+    """
+
+    def __init__(self, executor=None, slots: int = 4, budget: float = 200e9, skip_ahead: int = 8,
+                 keep_bases: bool = True, reserve_per_slot: int = 0, keep_results: bool = True,
+                 on_done: Optional[Callable[[Request], None]] = None,
+                 clock: Callable[[], float] = time.perf_counter):
+        self.ex = executor if executor is not None else DeviceExecutor()
+        self.slots = slots
+        # each slot's context keeps the workspace of the largest request it ran
+        self.budget = int(budget) - slots * int(reserve_per_slot)
+        if self.budget <= 0:
+            raise ValueError("budget does not cover the per-slot workspace reserve")
+        self.skip_ahead = skip_ahead
+        self.keep_bases = keep_bases
+        self.keep_results = keep_results
+        self.on_done = on_done
+        self.clock = clock
+        self.pending: collections.deque = collections.deque()
+        self.bases: Dict[int, Basis] = {}
+        self.inflight_bytes = 0
+        self.used_bases = 0
+        self.completed: List[Request] = []
+        self.rejected: List[Request] = []
+        self._next_basis = 0
+        self._next_rid = 0
+        self._cv = threading.Condition()
+        self._idle = list(range(slots))
+        self._work = [None] * slots
+        self._stop = False
+        self._n_inflight = 0
+        self.evictions = 0
+        self._threads = [threading.Thread(target=self._worker, args=(i,), daemon=True) for i in range(slots)]
+        for t in self._threads:
+            t.start()
+
+    # -- public ---------------------------------------------------------------
+    def submit(self, req: Request) -> Request:
+        with self._cv:
+            if req.arrival == 0.0:
+                req.arrival = self.clock()
+            req.rid = self._next_rid
+            self._next_rid += 1
+            if req.kind != POD:
+                b = self.bases.get(req.basis)
+                if b is None:
+                    req.error = KeyError(f"basis {req.basis} is not resident")
+                    req.done = self.clock()
+                    self.rejected.append(req)
+                    return req
+                b.pins += 1  # a basis with queued or running users is never evicted
+            self.pending.append(req)
+            self._admit_locked()
+            self._cv.notify_all()
+        return req
+
+    def add_basis(self, U, S, V) -> int:
+        """Register an existing (U, S, V) as a resident basis; returns its id."""
+        with self._cv:
+            return self._add_basis_locked((U, S, V))
+
+    def resident_ids(self) -> List[int]:
+        with self._cv:
+            return list(self.bases)
+
+    def drain(self, timeout: Optional[float] = None) -> bool:
+        """Wait until nothing is queued or in flight."""
+        end = None if timeout is None else self.clock() + timeout
+        with self._cv:
+            while self.pending or self._n_inflight:
+                left = None if end is None else end - self.clock()
+                if left is not None and left <= 0:
+                    return False
+                self._cv.wait(timeout=left if left is not None else 0.5)
+                self._admit_locked()
+        return True
+
+    def close(self):
+        with self._cv:
+            self._stop = True
+            self._cv.notify_all()
+        for t in self._threads:
+            t.join(timeout=30)
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+        return False
+
+    # -- scheduling -----------------------------------------------------------
+    def _add_basis_locked(self, res) -> int:
+        bid = self._next_basis
+        self._next_basis += 1
+        nb = self.ex.resident_bytes(res)
+        self.bases[bid] = Basis(res[0], res[1], res[2], nb, self.clock())
+        self.used_bases += nb
+        return bid
+
+    def _evict_for_locked(self, need: int) -> bool:
+        """Evict idle bases, least recently used first, until need fits."""
+        while self.inflight_bytes + self.used_bases + need > self.budget:
+            idle = [(b.last_used, bid) for bid, b in self.bases.items() if b.pins == 0]
+            if not idle:
+                return False
+            _, bid = min(idle)
+            b = self.bases.pop(bid)
+            self.used_bases -= b.nbytes
+            self.evictions += 1
+        return True
+
+    def _admit_locked(self):
+        i = 0
+        while self._idle and i < len(self.pending) and i <= self.skip_ahead:
+            req = self.pending[i]
+            basis = self.bases[req.basis] if req.kind != POD else None
+            need = int(self.ex.bytes_needed(req, basis))
+            if need > self.budget:
+                del self.pending[i]
+                if basis is not None:
+                    basis.pins -= 1
+                req.error = MemoryError(f"request needs {need} bytes, budget is {self.budget}")
+                req.done = self.clock()
+                self.rejected.append(req)
+                continue
+            if not self._evict_for_locked(need):
+                i += 1
+                continue
+            del self.pending[i]
+            req.nbytes = need
+            req.admitted = self.clock()
+            self.inflight_bytes += need
+            slot = self._idle.pop()
+            req.slot = slot
+            self._work[slot] = (req, basis)
+            self._n_inflight += 1
+            self._cv.notify_all()
+
+    def _worker(self, slot: int):
+        stream = self.ex.make_stream()
+        while True:
+            with self._cv:
+                while self._work[slot] is None and not self._stop:
+                    self._cv.wait()
+                if self._stop and self._work[slot] is None:
+                    return
+                req, basis = self._work[slot]
+            try:
+                with self.ex.stream_context(stream) if stream is not None else _NullCtx():
+                    res = self.ex.run(req, basis)
+                self.ex.synchronize(stream)
+                req.result = res
+            except BaseException as e:  # reported on the request, the loop keeps serving
+                req.error = e
+            with self._cv:
+                req.done = self.clock()
+                self.inflight_bytes -= req.nbytes
+                if basis is not None:
+                    basis.pins -= 1
+                    basis.last_used = req.done
+                if req.kind == POD and req.error is None and self.keep_bases:
+                    req.basis = self._add_basis_locked(req.result)
+                if self.on_done is not None:
+                    self.on_done(req)
+                if not self.keep_results:
+                    req.result = None
+                self._work[slot] = None
+                self._idle.append(slot)
+                self._n_inflight -= 1
+                self.completed.append(req)
+                self._admit_locked()
+                self._cv.notify_all()
+
+
+def percentile(xs, q):
+    if not xs:
+        return None
+    s = sorted(xs)
+    return s[min(len(s) - 1, int(round(q / 100.0 * (len(s) - 1))))]
+
+
+def summary(server: Server, wall: float) -> dict:
+    """p50 / p99 latency per kind, solves/s and requests/s over ``wall`` seconds."""
+    out = {"requests": len(server.completed), "rejected": len(server.rejected), "evictions": server.evictions,
+           "wall_s": round(wall, 3)}
+    ok = [r for r in server.completed if r.error is None]
+    out["errors"] = len(server.completed) - len(ok)
+    lat = [r.latency * 1e3 for r in ok]
+    out["p50_ms"] = percentile(lat, 50)
+    out["p99_ms"] = percentile(lat, 99)
+    for kind in (POD, PROJECT, RECONSTRUCT):
+        lk = [r.latency * 1e3 for r in ok if r.kind == kind]
+        out[kind] = {"count": len(lk), "p50_ms": percentile(lk, 50), "p99_ms": percentile(lk, 99)}
+    out["solves_per_s"] = round(sum(1 for r in ok if r.kind == POD) / wall, 2) if wall > 0 else None
+    out["requests_per_s"] = round(len(ok) / wall, 2) if wall > 0 else None
+    return out
