@@ -39,15 +39,19 @@ def parse():
     ap.add_argument("--n", type=int, default=N_DEFAULT)
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 path")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="run one plain device copy of the float32 input (known bytes) to calibrate "
+                         "the FETCH_SIZE / WRITE_SIZE counters")
     ap.add_argument("--cpu-rows", type=int, default=400_000)
     ap.add_argument("--mode", default="both", choices=["both", "mixed", "fp64"])
     ap.add_argument("--rate", type=float, default=200.0, help="serve: Poisson arrival rate, requests/s")
     ap.add_argument("--duration", type=float, default=20.0, help="serve: arrival window, s")
     ap.add_argument("--slots", type=int, default=4, help="serve: requests in flight")
-    ap.add_argument("--workload", default="pod", choices=["pod", "grouped", "serve"],
+    ap.add_argument("--workload", default="pod", choices=["pod", "grouped", "serve", "copy"],
                     help="pod: tall-skinny SVD 8M x 256 (default); grouped: SPOD 200000 x 2048 "
                          "(129 complex64 200000 x 15 blocks) + 64 uneven real POD problems; serve: "
-                         "Poisson stream of POD / project / reconstruct requests")
+                         "Poisson stream of POD / project / reconstruct requests; copy: one plain device "
+                         "copy of an 8M x 256 float32 matrix (counter calibration)")
     return ap.parse_args()
 
 
@@ -135,14 +139,36 @@ def stage_profile(solver, reps=3):
     return {k: (ms[i] / max(cnt[i], 1), cnt[i]) for i, k in enumerate(["gram", "zgram", "gemm"])}
 
 
+TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                            "mixed_passes_8Mx256_traffic.md")
+
+
+def measured_traffic(kernel, m, n):
+    """HBM bytes per launch of a mixed pass, from the committed counter summary
+    (FETCH_SIZE + WRITE_SIZE per dispatch, scaled by the copy calibration);
+    None when the summary does not cover this kernel at this size."""
+    if (m, n) != (M_DEFAULT, N_DEFAULT) or not os.path.exists(TRAFFIC_FILE):
+        return None
+    for line in open(TRAFFIC_FILE):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) >= 5 and cells[0] == f"`{kernel}`":
+            try:
+                return float(cells[4])
+            except ValueError:
+                return None
+    return None
+
+
 def roofline_for(stage_ms, m, n, b):
     """Dominant streaming kernel: algorithmic bytes per launch / its mean duration."""
     algo = {"gram": m * n * b, "zgram": m * n * b, "gemm": 2 * m * n * b}
+    kern = {"gram": "k_gram_mx2", "zgram": "k_zgram_mx2", "gemm": "k_gemm_mx2"}
     name = max(stage_ms, key=lambda k: stage_ms[k][0])
     t = stage_ms[name][0] * 1e-3
     ach = algo[name] / t / 1e9
     return name, {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                  "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None}
+                  "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": measured_traffic(kern[name], m, n),
+                  "kernel": kern[name], "algorithmic_bytes": algo[name], "avg_ms": round(t * 1e3, 3)}
 
 
 def cpu_baseline(m_sample, n, seed=7):
@@ -400,6 +426,23 @@ def run_serve(a):
     print(json.dumps(out), flush=True)
 
 
+def run_copy(a):
+    """One plain copy of an m x n float32 matrix: known bytes (m n 4 read, m n 4
+    written) to calibrate the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    X = torch.empty((a.m, a.n), dtype=torch.float32, device="cuda")
+    X.fill_(1.0)
+    Y = torch.empty_like(X)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    Y.copy_(X)
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    b = X.numel() * 4
+    print(json.dumps({"workload": "copy", "bytes_read": b, "bytes_written": b, "ms": round(t * 1e3, 3),
+                      "gbs": round(2 * b / t / 1e9, 1)}), flush=True)
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -409,6 +452,8 @@ def main():
         return run_grouped(a)
     if a.workload == "serve":
         return run_serve(a)
+    if a.workload == "copy":
+        return run_copy(a)
     import torch
     import pyloworder_amd  # noqa: F401  (fails loudly if libpodcore.so is missing)
     m, n = a.m, a.n
@@ -427,6 +472,12 @@ def main():
         log(f"[bench] fp64: {res64}")
         del s64
     X32 = X64.float()
+    if a.copy_calib:
+        Yc = torch.empty_like(X32)
+        Yc.copy_(X32)
+        torch.cuda.synchronize()
+        log(f"[bench] copy calibration: {X32.numel() * 4} bytes read, {X32.numel() * 4} bytes written")
+        del Yc
     del X64
     torch.cuda.empty_cache()
     log("[bench] mixed path")
