@@ -159,8 +159,11 @@ int pc_truncation_residual(const double* S, int64_t n, double r, int64_t* N);
 
 /* ---- batched (grouped) solves --------------------------------------------
  * The per-frequency loop of pyLOM/SPOD/wrapper.py:81-86 (tsqr_svd per block)
- * and any list of independent POD problems, run as one grouped launch per
- * pass.  Results are identical to pc_tsqr_svd on each problem alone.       */
+ * and any list of independent POD problems in one call: every problem is
+ * validated first, the problems run spread over internal streams (forked
+ * from and joined back to the context stream) with one workspace slice each,
+ * and the statuses are read once at the end; an error names the problem.
+ * Results are identical, bit for bit, to pc_tsqr_svd on each problem alone. */
 typedef struct {
   const void* A; int64_t lda; int64_t m; int64_t n; int32_t center; int32_t pad_;
   void* U; int64_t ldu; void* S; void* VT; int64_t ldvt;
@@ -170,7 +173,7 @@ int pc_grouped_tsqr_svd(pc_ctx* ctx, pc_dtype dtype, const pc_problem* problems,
 
 /* ---- SPOD front end --------------------------------------------------------
  * Welch blocks of pyLOM/SPOD/wrapper.py:39-82: per-row mean removal (FP64
- * means), Hamming window (pyLOM/vmmath/fft.py hammwin), real DFT of each block
+ * means), Hamming window (pyLOM/vmmath/fft.py:17-27), real DFT of each block
  * of nDFT samples with step nDFT-nolap, scaled by winWeight/nDFT (x2 for the
  * interior frequencies) and 1/sqrt(nBlks).  X: M x N float32, rows are points,
  * columns time samples.  Q receives nf = ceil(nDFT/2)+1 complex64 matrices of

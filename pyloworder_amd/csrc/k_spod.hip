@@ -1,7 +1,7 @@
 // SPOD front end: Welch blocks of the spectral POD (pyLOM/SPOD/wrapper.py:39-76).
 // For block b (time samples b*step .. b*step+nDFT-1), frequency f < nf and row i
 //   Q[f][i][b] = s_f * sum_t (x_i(b*step + t) - mu_i) w_t exp(-2 pi i f t / nDFT)
-// with w the Hamming window of pyLOM/vmmath/fft.py (0.54 - 0.46 cos(2 pi t /
+// with w the Hamming window of pyLOM/vmmath/fft.py:17-27 (0.54 - 0.46 cos(2 pi t /
 // (nDFT - 1))), s_f = winWeight / nDFT / sqrt(nBlks), doubled for 0 < f < nf-1
 // (wrapper.py:73-74 and the 1/sqrt(nBlks) of wrapper.py:82).  Q[f] is the
 // M x nBlks complex64 matrix of frequency f with contiguous rows, i.e. exactly
