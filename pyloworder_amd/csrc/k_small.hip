@@ -672,6 +672,102 @@ __global__ __launch_bounds__(NT) void k_bjac_gstep(double* H, int ldh, int rows,
   }
 }
 
+// One-sided Jacobi of a small problem held by a single block pair (2b <= 32
+// columns, rows <= 64), on one wavefront: 4 lanes per column pair, every
+// pair of a round in parallel, 4-lane shuffle reductions, and wave-level
+// ordering between rounds instead of workgroup barriers.  Inner sweeps run
+// until one rotates nothing (at most max_inner); state[1 + sweep] is set when
+// the last inner sweep still rotated, as k_bjac_step does for one pair.
+template <class T>
+__global__ __launch_bounds__(64) void k_bjac_small(T* H, int ldh, int rows, int cols, T* J, int ldj, int b, int sweep,
+                                                   int* state, const double* dstate, double tol, int max_inner) {
+  __shared__ T Hs[32 * 64];
+  __shared__ T Js[32 * 32];
+  if (state[0]) return;
+  auto wave_sync = [] {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  };
+  const int c2 = 2 * b, l = threadIdx.x;
+  const double negl = dstate[0];
+  for (int idx = l; idx < c2 * rows; idx += 64) {
+    int lc = idx / rows, i = idx - lc * rows;
+    Hs[idx] = lc < cols ? H[int64_t(lc) * ldh + i] : T(0.0);
+  }
+  for (int idx = l; idx < c2 * cols; idx += 64) {
+    int lc = idx / cols, i = idx - lc * cols;
+    Js[idx] = lc < cols ? J[int64_t(lc) * ldj + i] : T(0.0);
+  }
+  wave_sync();
+  const int pr = l >> 2, sub = l & 3;
+  int last_rot = 0;
+  for (int inner = 0; inner < max_inner; inner++) {
+    unsigned long long rot = 0;
+    for (int s = 0; s < c2 - 1; s++) {
+      int u = 0, v = 0;
+      if (pr < b) {
+        u = rr_player(pr, s, c2);
+        v = rr_player(c2 - 1 - pr, s, c2);
+        if (u > v) { int t = u; u = v; v = t; }
+      }
+      const bool valid = pr < b && v < cols;  // the same for the 4 lanes of a pair
+      T* hu = Hs + int64_t(u) * rows;
+      T* hv = Hs + int64_t(v) * rows;
+      double a = 0.0, bb = 0.0;
+      T c = T(0.0);
+      if (valid) {
+        for (int i = sub; i < rows; i += 4) {
+          T x = hu[i], y = hv[i];
+          a += abs2_(x);
+          bb += abs2_(y);
+          c += dotc_(x, y);
+        }
+      }
+      a = group_sum(a, 4);
+      bb = group_sum(bb, 4);
+      c = group_sum_t(c, 4);
+      int act = 0;
+      if (valid) {
+        double gm = sqrt(abs2_(c));
+        if (a > negl && bb > negl && gm > tol * sqrt(a) * sqrt(bb)) {
+          T e = phase_(c, gm);
+          double zeta = (bb - a) / (2.0 * gm);
+          double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+          double cs = 1.0 / sqrt(1.0 + t * t), sn = cs * t;
+          T ec = conj_(e);
+          for (int i = sub; i < rows; i += 4) {
+            T x = hu[i], y = hv[i] * ec;
+            hu[i] = x * cs - y * sn;
+            hv[i] = x * sn + y * cs;
+          }
+          T* ju = Js + int64_t(u) * cols;
+          T* jv = Js + int64_t(v) * cols;
+          for (int i = sub; i < cols; i += 4) {
+            T x = ju[i], y = jv[i] * ec;
+            ju[i] = x * cs - y * sn;
+            jv[i] = x * sn + y * cs;
+          }
+          act = 1;
+        }
+      }
+      rot |= __ballot(act);
+      wave_sync();
+    }
+    last_rot = rot != 0;
+    if (!rot) break;
+  }
+  for (int idx = l; idx < c2 * rows; idx += 64) {
+    int lc = idx / rows, i = idx - lc * rows;
+    if (lc < cols) H[int64_t(lc) * ldh + i] = Hs[idx];
+  }
+  for (int idx = l; idx < c2 * cols; idx += 64) {
+    int lc = idx / cols, i = idx - lc * cols;
+    if (lc < cols) J[int64_t(lc) * ldj + i] = Js[idx];
+  }
+  if (l == 0 && last_rot) atomicOr(&state[1 + sweep], 1);
+}
+
 // rows of the orthogonal preconditioner: V[perm[k]][:] = Q[k][:]
 template <class T>
 __global__ __launch_bounds__(256) void k_permute_rows(const T* Qm, int ldq, int n, const int* perm, T* V, int ldv) {

@@ -34,13 +34,14 @@ constexpr double kNullMx = 1e-6;
 inline unsigned nblk(int64_t count, int per = 256) { return unsigned(ceil_div(count, per)); }
 
 constexpr int kBjacMaxSweeps = 48;
+constexpr double kJacTolMx = 1e-10;  // Jacobi orthogonality tolerance, mixed path
 
 // One-sided block Jacobi on H (rows x cols, column-major) with J accumulated:
 // one launch per round-robin step of a sweep, a check launch per sweep.  After
 // convergence every remaining launch exits immediately.
 template <class T>
 void launch_block_jacobi(pc_ctx* ctx, T* H, int ldh, int rows, int cols, T* J, int ldj, double* sig, int* state,
-                         double* dstate, int* status) {
+                         double* dstate, int* status, double min_tol) {
   hipStream_t st = ctx->stream;
   // real FP64 with rows <= 256: Gram-driven MFMA steps on 16-column blocks;
   // otherwise column-pair rotations staged in LDS
@@ -58,7 +59,7 @@ void launch_block_jacobi(pc_ctx* ctx, T* H, int ldh, int rows, int cols, T* J, i
   int max_inner = one_pair ? 60 : gram_steps ? 1 : 3;
   // relative orthogonality test |h_p^H h_q| <= tol ||h_p|| ||h_q||: the dot
   // product itself carries ~eps sqrt(rows) rounding, so tol grows with rows
-  double tol = 2.220446049250313e-16 * std::max(rows, 16);
+  double tol = std::max(2.220446049250313e-16 * std::max(rows, 16), min_tol);
   // Sweeps are queued in batches: after the first batch the host reads the
   // done flag and stops queueing once the iteration has converged (typically
   // after ~10 sweeps at n = 256).  Under stream capture no host read is
@@ -83,6 +84,11 @@ void launch_block_jacobi(pc_ctx* ctx, T* H, int ldh, int rows, int cols, T* J, i
                              state, dstate, tol, max_inner);
           continue;
         }
+      }
+      if (one_pair && rows <= 64 && 2 * b <= 32) {
+        hipLaunchKernelGGL((k_bjac_small<T>), dim3(1), dim3(64), 0, st, H, ldh, rows, cols, J, ldj, b, s, state, dstate,
+                           tol, max_inner);
+        continue;
       }
       hipLaunchKernelGGL((k_bjac_step<T>), dim3(nb2 / 2), dim3(1024), 0, st, H, ldh, rows, cols, J, ldj, b, nb2, step,
                          s, state, dstate, tol, max_inner);
@@ -662,7 +668,10 @@ static int core_t(pc_ctx* ctx, const Plan& p, const void* A, int64_t lda, int ce
                        n);
     hipLaunchKernelGGL((k_gemm_small<T>), gg, dim3(256), 0, st, 0, 0, n, n, n, T(1.0), b.tmp, n, (const int*)nullptr,
                        b.Awk, n, (const int*)nullptr, T(0.0), b.Hc, n, 1);
-    launch_block_jacobi<T>(ctx, b.Hc, n, n, n, b.J, n, b.sig, b.jstate, b.jdstate, b.status);
+    // mixed precision: the factor carries ~1e-7 relative data error, so the
+    // columns need only be orthogonal to 1e-10 (sigma errs by O(tol^2))
+    launch_block_jacobi<T>(ctx, b.Hc, n, n, n, b.J, n, b.sig, b.jstate, b.jdstate, b.status,
+                           p.mixed ? kJacTolMx : 0.0);
     // V = V0 J (column-major like J)
     hipLaunchKernelGGL((k_gemm_small<T>), gg, dim3(256), 0, st, 0, 1, n, n, n, T(1.0), b.Awk, n, (const int*)nullptr,
                        b.J, n, (const int*)nullptr, T(0.0), b.tmp, n, 1);
