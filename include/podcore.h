@@ -171,6 +171,21 @@ typedef struct {
 int pc_grouped_tsqr_svd(pc_ctx* ctx, pc_dtype dtype, const pc_problem* problems, int64_t count,
                         int precision);
 
+/* Captured grouped solves (no reference counterpart: repeated SPOD / POD of
+ * fixed shapes, e.g. new snapshots written into the same buffers).
+ * pc_grouped_graph_create records pc_grouped_tsqr_svd of this problem list
+ * (same pointers, shapes and precision) into a HIP graph that owns its own
+ * workspaces; pc_grouped_graph_launch replays it on the context stream and
+ * returns when the results are ready, naming a failing problem like the
+ * direct call.  Inside a graph every block-Jacobi sweep is recorded (no host
+ * check); converged sweeps return at once.  The buffers of the problems must
+ * outlive the graph.                                                        */
+typedef struct pc_graph pc_graph;
+int pc_grouped_graph_create(pc_ctx* ctx, pc_dtype dtype, const pc_problem* problems, int64_t count,
+                            int precision, pc_graph** out);
+int pc_grouped_graph_launch(pc_ctx* ctx, pc_graph* graph);
+int pc_graph_destroy(pc_graph* graph);
+
 /* ---- SPOD front end --------------------------------------------------------
  * Welch blocks of pyLOM/SPOD/wrapper.py:39-82: per-row mean removal (FP64
  * means), Hamming window (pyLOM/vmmath/fft.py:17-27), real DFT of each block

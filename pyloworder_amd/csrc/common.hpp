@@ -87,6 +87,9 @@ struct pc_ctx {
   static constexpr int kGroupStreams = 4;
   hipStream_t gs[kGroupStreams] = {nullptr, nullptr, nullptr, nullptr};
   hipEvent_t gev[kGroupStreams + 1] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  // one sub-context per internal stream (own workspace and status words),
+  // driven by its own host thread during a grouped solve
+  pc_ctx* sub[kGroupStreams] = {nullptr, nullptr, nullptr, nullptr};
   std::string err;
   // optional per-stage timing (pc_profile): events around each streaming pass
   int prof = 0;

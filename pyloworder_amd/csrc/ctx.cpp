@@ -44,6 +44,12 @@ extern "C" int pc_ctx_destroy(pc_ctx* ctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   else (void)hipDeviceSynchronize();
   for (int i = 0; i < pc_ctx::kGroupStreams; i++) {
+    if (ctx->sub[i]) {
+      (void)pc_ctx_destroy(ctx->sub[i]);
+      ctx->sub[i] = nullptr;
+    }
+  }
+  for (int i = 0; i < pc_ctx::kGroupStreams; i++) {
     if (ctx->gs[i]) {
       (void)hipStreamSynchronize(ctx->gs[i]);
       (void)hipStreamDestroy(ctx->gs[i]);

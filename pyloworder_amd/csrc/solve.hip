@@ -14,6 +14,7 @@
 //       W = W1 P R2^-1 U_M               (U_M = H sigma^-1)
 //   P3  U = Y W                          (MFMA streaming pass)
 // plus an orthonormal completion of U for numerically null singular values.
+#include <thread>
 #include "common.hpp"
 #include "kernels.hpp"
 #include "k_small.hip"
@@ -34,6 +35,11 @@ constexpr double kNullMx = 1e-6;
 inline unsigned nblk(int64_t count, int per = 256) { return unsigned(ceil_div(count, per)); }
 
 constexpr int kBjacMaxSweeps = 48;
+
+// dst[0..3] = src ? src[0..3] : 0 (the four status words of a solve)
+__global__ void k_status_copy(const int* src, int* dst) {
+  if (threadIdx.x < 4) dst[threadIdx.x] = src ? src[threadIdx.x] : 0;
+}
 constexpr double kJacTolMx = 1e-10;  // Jacobi orthogonality tolerance, mixed path
 
 // One-sided block Jacobi on H (rows x cols, column-major) with J accumulated:
@@ -626,7 +632,9 @@ static int core_t(pc_ctx* ctx, const Plan& p, const void* A, int64_t lda, int ce
   WsCursor cur{ctx->ws + base, 0, ctx->ws_bytes - base};
   carve<T>(cur, p, b, &mean, &var, &part, slabs);
   T* nwork = cur.take<T>(dense_work(n));
-  PC_HIP(ctx, hipMemsetAsync(b.status, 0, 16, st));
+  // status words cleared by a kernel (no memset node when the solve is
+  // captured into a graph)
+  hipLaunchKernelGGL(k_status_copy, dim3(1), dim3(64), 0, st, nullptr, b.status);
 
   // ---- P1
   PC_CHECK(ctx, launch_streams<T>(ctx, p, A, lda, center, mean, var, part, b, 1));
@@ -717,7 +725,8 @@ static int core_t(pc_ctx* ctx, const Plan& p, const void* A, int64_t lda, int ce
                      b.jsel, b.cnull, (typename VtType<T, ST>::type*)out.U, out.ldu);
   PC_HIP(ctx, hipGetLastError());
   if (ctx->gstat) {  // grouped solve: statuses are read once for the whole group
-    PC_HIP(ctx, hipMemcpyAsync(ctx->gstat, b.status, 16, hipMemcpyDeviceToDevice, st));
+    hipLaunchKernelGGL(k_status_copy, dim3(1), dim3(64), 0, st, b.status, ctx->gstat);
+    PC_HIP(ctx, hipGetLastError());
     return PC_OK;
   }
   // status check: one small D2H copy, synchronous (errors must surface here)

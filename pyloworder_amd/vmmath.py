@@ -24,7 +24,7 @@ from ._tensor import Arg, empty, pc_dtype, real_dtype
 from .cr import cr
 
 __all__ = [
-    "qr", "svd", "tsqr", "tsqr_svd", "tsqr_svd_grouped", "randomized_qr", "randomized_svd", "matmul", "matmulp",
+    "qr", "svd", "tsqr", "tsqr_svd", "tsqr_svd_grouped", "GroupedGraph", "randomized_qr", "randomized_svd", "matmul", "matmulp",
     "vecmat", "transpose", "temporal_mean", "subtract_mean", "temporal_variance", "norm_variance",
     "compute_truncation_residual", "RMSE", "energy", "reference_omega",
 ]
@@ -103,6 +103,60 @@ def tsqr_svd_grouped(As, precision: str = "fp64", center: int = 0):
                                           _prec(precision)))
     return ([a.give(U) for a, U in zip(args, Us)], [a.give(S) for a, S in zip(args, Ss)],
             [a.give(V) for a, V in zip(args, Vs)])
+
+
+class GroupedGraph:
+    """A grouped tsqr_svd recorded once into a HIP graph and replayed.
+
+    No reference counterpart: for repeated solves of fixed shapes (SPOD on
+    new snapshots written into the same buffers, repeated POD of a stream).
+    ``As``: device tensors (kept referenced; their contents may change
+    between launches).  ``launch()`` replays the recorded solves on the
+    current stream and returns the output lists (Us, Ss, Vs), the same
+    tensors each time, identical to ``tsqr_svd_grouped`` of the current data.
+    """
+
+    def __init__(self, As, precision: str = "fp64", center: int = 0):
+        if isinstance(As, torch.Tensor) and As.dim() == 3:
+            As = [As[i] for i in range(As.shape[0])]
+        self.args = [Arg(A, "Ai") for A in As]
+        if not self.args or any(a.was_numpy for a in self.args):
+            raise TypeError("GroupedGraph records device tensors (torch, on the GPU)")
+        dt = self.args[0].t.dtype
+        self.Us, self.Ss, self.Vs = [], [], []
+        self._probs = (_lib.PCProblem * len(self.args))()
+        for i, a in enumerate(self.args):
+            if a.t.dtype != dt:
+                raise TypeError("GroupedGraph: all matrices must share one dtype")
+            m, n = a.t.shape
+            U, S, V = empty((m, n), dt, a.t), empty((n,), real_dtype(dt), a.t), empty((n, n), dt, a.t)
+            self.Us.append(U)
+            self.Ss.append(S)
+            self.Vs.append(V)
+            self._probs[i] = _lib.PCProblem(a.ptr(), a.ld, m, n, int(center), 0, U.data_ptr(), U.stride(0),
+                                            S.data_ptr(), V.data_ptr(), V.stride(0))
+        ctx = _ctx()
+        self._lib = ctx.lib
+        h = C.c_void_p()
+        ctx.check(ctx.lib.pc_grouped_graph_create(ctx.h, pc_dtype(self.args[0].t), C.cast(self._probs, C.c_void_p),
+                                                  len(self.args), _prec(precision), C.byref(h)))
+        self.h = h
+
+    def launch(self):
+        ctx = _ctx()
+        ctx.check(ctx.lib.pc_grouped_graph_launch(ctx.h, self.h))
+        return self.Us, self.Ss, self.Vs
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._lib.pc_graph_destroy(self.h)
+            self.h = None
+
+    def __del__(self):  # pragma: no cover - interpreter shutdown order varies
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 @cr("math.tsqr")
