@@ -576,3 +576,54 @@ __global__ __launch_bounds__(NT) void k_house_qr_fast(const double* __restrict__
     __syncthreads();
   }
 }
+
+// W = R^-1 for an upper triangular R (n <= 256, row-major), one wavefront per
+// column j of W: the back substitution R w = e_j runs from i = j down to 0
+// with the right-hand side r in registers (lane l owns rows l + 64 q):
+//   w_i = r_i / R_ii,   r_k -= R_ki w_i  (k < i).
+// The columns of R a stretch of 64 steps needs are staged in LDS, rows
+// padded to 65 doubles so that the lanes of a wave read one column without
+// bank conflicts.  The n columns are independent solves, one workgroup each.
+__global__ __launch_bounds__(64) void k_trinv_cols(const double* __restrict__ R, int ldr, int n,
+                                                   double* __restrict__ W, int ldw) {
+  __shared__ double Rs[dense::NMAX * 65];
+  const int j = blockIdx.x, l = threadIdx.x;
+  auto wave_sync = [] {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  };
+  double r[4], w[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    r[q] = (l + 64 * q == j) ? 1.0 : 0.0;
+    w[q] = 0.0;
+  }
+  for (int g = j >> 6; g >= 0; --g) {
+    const int c0 = 64 * g;
+    const int cl = j - c0 < 63 ? j - c0 : 63;  // last column of the stretch
+    const int rows = c0 + cl + 1;              // rows 0 .. c0 + cl hold the upper triangle
+    wave_sync();  // the previous stretch has been read
+    if (l <= cl)
+      for (int k = 0; k < rows; k++) Rs[k * 65 + l] = R[int64_t(k) * ldr + c0 + l];
+    wave_sync();
+    for (int t = cl; t >= 0; --t) {
+      const int i = c0 + t;
+      double rv = r[0];
+#pragma unroll
+      for (int q = 1; q < 4; q++) rv = g == q ? r[q] : rv;
+      const double wi = __shfl(rv, t) / Rs[i * 65 + t];
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        if (g == q && l == t) w[q] = wi;
+        const int k = l + 64 * q;
+        if (k < i) r[q] -= Rs[k * 65 + t] * wi;
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const int k = l + 64 * q;
+    if (k < n) W[int64_t(k) * ldw + j] = k <= j ? w[q] : 0.0;
+  }
+}

@@ -120,7 +120,7 @@ template <class T>
 void run_trinv(hipStream_t st, const T* R, int n, T* W) {
   if constexpr (std::is_same<T, double>::value)
     if (n <= dense::NMAX) {
-      hipLaunchKernelGGL(k_trinv_fast, dim3(1), dim3(NT), 0, st, R, n, n, W, n);
+      hipLaunchKernelGGL(k_trinv_cols, dim3(n), dim3(64), 0, st, R, n, n, W, n);
       return;
     }
   hipLaunchKernelGGL((k_trinv_upper<T>), dim3(1), dim3(NT), 0, st, R, n, n, W, n);
