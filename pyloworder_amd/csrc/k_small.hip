@@ -533,11 +533,11 @@ __global__ __launch_bounds__(NT) void k_bjac_gstep(double* H, int ldh, int rows,
                                                    int step, int sweep, int* state, const double* dstate, double tol,
                                                    int max_inner) {
   __shared__ double Hs[GC * (GROWS + 1)];
-  __shared__ double G[GC * GLD];
-  __shared__ double Q[GC * GLD];
+  __shared__ double G[GC * GLD], G2[GC * GLD];
+  __shared__ double Q[GC * GLD], Q2[GC * GLD];
   __shared__ double Pp[3 * 4 * 256];
-  __shared__ double rcs[GB], rsn[GB];
-  __shared__ int ru[GB], rv[GB], ract[GB];
+  __shared__ double ral[GC], rbe[GC];  // this round's rotation R: column x of R = ral[x] e_x + rbe[x] e_rpt[x]
+  __shared__ int rpt[GC];
   __shared__ int need, rot_s;
   if (state[0]) return;
   const int RP = (rows + 15) & ~15;
@@ -583,6 +583,15 @@ __global__ __launch_bounds__(NT) void k_bjac_gstep(double* H, int ldh, int rows,
   __syncthreads();
   if (!need) return;
   if (threadIdx.x == 0) atomicOr(&state[1 + sweep], 1);
+  // Each round applies 16 disjoint rotations as G <- R^T G R and Q <- Q R in
+  // one pass: thread (i, j) forms its entry from the old G and Q into the
+  // other buffer ((G R)[i][j] and (G R)[i'][j] first, then the row
+  // combination), so a round needs two barriers.
+  static_assert(NT == GC * GC, "one thread per entry of the pair's Gram");
+  double* Gc = G;
+  double* Gn = G2;
+  double* Qc = Q;
+  double* Qn = Q2;
   for (int inner = 0; inner < max_inner; inner++) {
     if (threadIdx.x == 0) rot_s = 0;
     __syncthreads();
@@ -591,8 +600,7 @@ __global__ __launch_bounds__(NT) void k_bjac_gstep(double* H, int ldh, int rows,
         int pr = threadIdx.x;
         int u = rr_player(pr, s, GC), v = rr_player(GC - 1 - pr, s, GC);
         if (u > v) { int t = u; u = v; v = t; }
-        double a = G[u * GLD + u], bb = G[v * GLD + v], c = G[u * GLD + v], gm = fabs(c);
-        int act = 0;
+        double a = Gc[u * GLD + u], bb = Gc[v * GLD + v], c = Gc[u * GLD + v], gm = fabs(c);
         double cs = 1.0, sn = 0.0;
         if (a > negl && bb > negl && gm > tol * sqrt(a) * sqrt(bb)) {
           double zeta = (bb - a) / (2.0 * gm);
@@ -600,36 +608,25 @@ __global__ __launch_bounds__(NT) void k_bjac_gstep(double* H, int ldh, int rows,
           cs = 1.0 / sqrt(1.0 + t * t);
           sn = cs * t;
           if (c < 0) sn = -sn;  // fold the sign of c (real phase) into the rotation
-          act = 1;
           rot_s = 1;
         }
-        ru[pr] = u; rv[pr] = v; rcs[pr] = cs; rsn[pr] = sn; ract[pr] = act;
+        // column u of R: cs e_u - sn e_v; column v: sn e_u + cs e_v (identity when inactive)
+        ral[u] = cs; rbe[u] = -sn; rpt[u] = v;
+        ral[v] = cs; rbe[v] = sn; rpt[v] = u;
       }
       __syncthreads();
-      // G <- G R and Q <- Q R (columns u, v of each active pair)
-      for (int t = threadIdx.x; t < 2 * GC * GB; t += NT) {
-        int mat = t / (GC * GB), tt = t - mat * (GC * GB);
-        int i = tt / GB, pr = tt - i * GB;
-        if (!ract[pr]) continue;
-        double* Mx = mat ? Q : G;
-        int u = ru[pr], v = rv[pr];
-        double cs = rcs[pr], sn = rsn[pr];
-        double x = Mx[i * GLD + u], y = Mx[i * GLD + v];
-        Mx[i * GLD + u] = cs * x - sn * y;
-        Mx[i * GLD + v] = sn * x + cs * y;
+      {
+        const int i = threadIdx.x / GC, j = threadIdx.x - i * GC;
+        const double ai = ral[i], bi = rbe[i], aj = ral[j], bj = rbe[j];
+        const int ip = rpt[i], jp = rpt[j];
+        const double t1 = aj * Gc[i * GLD + j] + bj * Gc[i * GLD + jp];
+        const double t2 = aj * Gc[ip * GLD + j] + bj * Gc[ip * GLD + jp];
+        Gn[i * GLD + j] = ai * t1 + bi * t2;
+        Qn[i * GLD + j] = aj * Qc[i * GLD + j] + bj * Qc[i * GLD + jp];
       }
       __syncthreads();
-      // G <- R^T G (rows u, v)
-      for (int t = threadIdx.x; t < GC * GB; t += NT) {
-        int j = t / GB, pr = t - j * GB;
-        if (!ract[pr]) continue;
-        int u = ru[pr], v = rv[pr];
-        double cs = rcs[pr], sn = rsn[pr];
-        double x = G[u * GLD + j], y = G[v * GLD + j];
-        G[u * GLD + j] = cs * x - sn * y;
-        G[v * GLD + j] = sn * x + cs * y;
-      }
-      __syncthreads();
+      double* t = Gc; Gc = Gn; Gn = t;
+      t = Qc; Qc = Qn; Qn = t;
     }
     int any = rot_s;
     __syncthreads();
@@ -640,7 +637,7 @@ __global__ __launch_bounds__(NT) void k_bjac_gstep(double* H, int ldh, int rows,
     for (int cB = 0; cB < 2; cB++) {
       d4s acc = {0.0, 0.0, 0.0, 0.0};
       for (int k0 = 0; k0 < GC; k0 += 4)
-        acc = mfma64s(Hs[(k0 + lk) * ld + 16 * rb + lr], Q[(k0 + lk) * GLD + 16 * cB + lr], acc);
+        acc = mfma64s(Hs[(k0 + lk) * ld + 16 * rb + lr], Qc[(k0 + lk) * GLD + 16 * cB + lr], acc);
       int gc = gcol(16 * cB + lr);
       for (int r = 0; r < 4; r++) {
         int row = 16 * rb + 4 * r + lk;
@@ -660,7 +657,7 @@ __global__ __launch_bounds__(NT) void k_bjac_gstep(double* H, int ldh, int rows,
     d4s acc[2];
     for (int cB = 0; cB < 2; cB++) {
       acc[cB] = d4s{0.0, 0.0, 0.0, 0.0};
-      for (int kk = 0; kk < GC / 4; kk++) acc[cB] = mfma64s(av[kk], Q[(4 * kk + lk) * GLD + 16 * cB + lr], acc[cB]);
+      for (int kk = 0; kk < GC / 4; kk++) acc[cB] = mfma64s(av[kk], Qc[(4 * kk + lk) * GLD + 16 * cB + lr], acc[cB]);
     }
     for (int cB = 0; cB < 2; cB++) {
       int gc = gcol(16 * cB + lr);
