@@ -1,9 +1,10 @@
-// Blocked single-workgroup FP64 factorizations for real n x n matrices,
-// n <= 256 (the small dense stage of the POD core): shifted Cholesky,
-// pivoted Cholesky, triangular inverse and Householder QR with explicit Q.
-// Matrices live in global memory (L2-resident at these sizes); 32-column
-// panels are staged in LDS and every rank-32 update runs on
-// v_mfma_f64_16x16x4f64.  One workgroup of 1024 threads (16 waves).
+// FP64 dense kernels for real n x n matrices, n <= 256 (the small dense
+// stage of the POD core).  Shifted Cholesky, pivoted Cholesky and Householder
+// QR with explicit Q are blocked single-workgroup factorizations (1024
+// threads): matrices live in global memory (L2-resident at these sizes),
+// 32-column panels are staged in LDS and every rank-32 update runs on
+// v_mfma_f64_16x16x4f64.  The triangular inverse is n independent column
+// back-substitutions, one wavefront each.
 //
 // f64 MFMA 16x16x4 lane layout (wave64): lane l holds A[l % 16][l / 16] and
 // B[l / 16][l % 16]; accumulator register r holds D[4 r + l / 16][l % 16].
@@ -136,79 +137,6 @@ __global__ __launch_bounds__(NT) void k_chol_fast(const double* __restrict__ G, 
     tau *= 8.0;
   }
   if (tid == 0) status[0] = 1;
-}
-
-// ---------------------------------------------------------------------------
-// Inverse of a real upper-triangular R (n <= 256): W = R^-1.
-//   1. diagonal 32x32 blocks inverted by back substitution, one thread per
-//      column;
-//   2. block diagonals d = 1, 2, ...:  S_ij = sum_{k=i+1..j} R_ik W_kj and
-//      W_ij = -W_ii S_ij, both on MFMA.
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(NT) void k_trinv_fast(const double* __restrict__ R, int ldr, int n, double* __restrict__ W,
-                                                   int ldw) {
-  using namespace dense;
-  __shared__ double S[(NMAX / PW - 1) * PW * (PW + 1)];
-  const int tid = threadIdx.x, w = tid >> 6, l = tid & 63, lr = l & 15, lk = l >> 4;
-  const int nbk = (n + PW - 1) / PW;
-  for (int i = w; i < n; i += NT / 64)
-    for (int j = l; j < n; j += 64)
-      if (i > j || (i / PW) != (j / PW)) W[int64_t(i) * ldw + j] = 0.0;
-  if (tid < nbk * PW) {
-    const int b = tid / PW, c = tid % PW, o = b * PW, bs = min(PW, n - o);
-    if (c < bs) {
-      double x[PW];
-#pragma unroll
-      for (int i = PW - 1; i >= 0; i--) {
-        double s = 0.0;
-        if (i <= c) {
-          s = i == c ? 1.0 : 0.0;
-#pragma unroll
-          for (int t = i + 1; t < PW; t++)
-            if (t <= c) s -= R[int64_t(o + i) * ldr + o + t] * x[t];
-          s /= R[int64_t(o + i) * ldr + o + i];
-          W[int64_t(o + i) * ldw + o + c] = s;
-        }
-        x[i] = s;
-      }
-    }
-  }
-  __syncthreads();
-  for (int d = 1; d < nbk; d++) {
-    const int nbl = nbk - d;
-    for (int task = w; task < nbl * 4; task += NT / 64) {
-      const int bi = task >> 2, tI = (task >> 1) & 1, tJ = task & 1, bj = bi + d;
-      const int row = PW * bi + 16 * tI + lr, col = PW * bj + 16 * tJ + lr;
-      const int kbeg = PW * (bi + 1), kend = min(PW * (bj + 1), n);
-      d4s acc = {0.0, 0.0, 0.0, 0.0};
-      for (int k0 = kbeg; k0 < kend; k0 += 4) {
-        int k = k0 + lk;
-        double a = (k < kend && row < n) ? R[int64_t(row) * ldr + k] : 0.0;
-        double bb = (k < kend && col < n) ? W[int64_t(k) * ldw + col] : 0.0;
-        acc = mfma64s(a, bb, acc);
-      }
-#pragma unroll
-      for (int r = 0; r < 4; r++) S[bi * PW * (PW + 1) + (16 * tI + 4 * r + lk) * (PW + 1) + 16 * tJ + lr] = acc[r];
-    }
-    __syncthreads();
-    // R_ii W_ij = -S_ij by back substitution (not a product with the computed
-    // W_ii: a graded diagonal block makes the product lose accuracy); one thread
-    // per column, in place in its own column of S.  Block bi is full (bi < nbk-1).
-    if (tid < nbl * PW) {
-      const int bi = tid / PW, c = tid % PW, bj = bi + d, o = PW * bi, col = PW * bj + c;
-      double* sc = S + bi * PW * (PW + 1) + c;
-      if (col < n) {
-        for (int i = PW - 1; i >= 0; i--) {
-          double s = -sc[i * (PW + 1)];
-          for (int t = i + 1; t < PW; t++) s -= R[int64_t(o + i) * ldr + o + t] * sc[t * (PW + 1)];
-          s /= R[int64_t(o + i) * ldr + o + i];
-          sc[i * (PW + 1)] = s;
-          W[int64_t(o + i) * ldw + col] = s;
-        }
-      }
-    }
-    __syncthreads();
-  }
 }
 
 // ---------------------------------------------------------------------------

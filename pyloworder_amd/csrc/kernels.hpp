@@ -14,19 +14,7 @@ int launch_gemm_f64(hipStream_t st, pc_dtype dt, const void* X, int64_t ldx, int
 int launch_xgram_f64(hipStream_t st, pc_dtype dt, const void* A, int64_t lda, int p, const void* B, int64_t ldb, int q,
                      int64_t m, int chunks, int64_t tpc, double* part);
 
-// mixed (fp16 split) MFMA streaming kernels (k_mfma_mx.hip)
-int mx_nb(int n);
-int mx_gram_rows();
-int mx_gemm_rows();
-int launch_gram_mx(hipStream_t st, const float* X, int64_t ldx, int64_t m, int n, int center, double* mean_out,
-                   double* var_out, int chunks, int64_t tpc, double* part);
-int launch_zgram_mx(hipStream_t st, const float* X, int64_t ldx, int64_t m, int n, int center, const double* mean_in,
-                    const double* var_in, const _Float16* Wf, const float* wscale, int chunks, int64_t tpc,
-                    double* part);
-int launch_gemm_mx(hipStream_t st, const float* X, int64_t ldx, int64_t m, int n, int center, const double* mean_in,
-                   const double* var_in, const _Float16* Wh, const _Float16* Wl, const float* wscale, int nc,
-                   int chunks, int64_t tpc, float* C, int64_t ldc);
-// second-generation mixed kernels (k_mfma_mx2.hip): 8 waves, W operand in registers
+// mixed (fp16 split) MFMA streaming kernels (k_mfma_mx2.hip): 8 waves, W operand in registers
 int mx2_nb(int n);
 int mx2_gram_rows(int n);
 int mx2_tile_rows(int n);
@@ -43,5 +31,6 @@ void launch_row_stats_f32(hipStream_t st, const float* X, int64_t ldx, int64_t m
                           double* var);
 // pack an FP64 n x nc matrix (ld ldw) into fp16 hi (and lo) fragment order with
 // per-column power-of-two scales; also writes the FP64 value of hi*2^-s (Wr)
+// (k_mfma_mx.hip)
 int launch_pack_w_mx(hipStream_t st, const double* W, int ldw, int n, int nc, int NB, int NCB, _Float16* Wh,
                      _Float16* Wl, float* wscale, double* Wr, int ldwr);
