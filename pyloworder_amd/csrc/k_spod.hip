@@ -1,4 +1,6 @@
 // SPOD front end: Welch blocks of the spectral POD (pyLOM/SPOD/wrapper.py:39-76).
+// Power-of-two block lengths use a radix-2 FFT in LDS (k_spod_fft), other
+// lengths the direct windowed DFT (k_spod_blocks).
 // For block b (time samples b*step .. b*step+nDFT-1), frequency f < nf and row i
 //   Q[f][i][b] = s_f * sum_t (x_i(b*step + t) - mu_i) w_t exp(-2 pi i f t / nDFT)
 // with w the Hamming window of pyLOM/vmmath/fft.py:17-27 (0.54 - 0.46 cos(2 pi t /
@@ -7,6 +9,7 @@
 // M x nBlks complex64 matrix of frequency f with contiguous rows, i.e. exactly
 // the operand the per-frequency tall-skinny SVD reads.
 #include <cmath>
+#include <type_traits>
 #include <vector>
 #include "common.hpp"
 #include "kernels.hpp"
@@ -105,6 +108,78 @@ __global__ __launch_bounds__(256) void k_spod_blocks(const float* __restrict__ X
   }
 }
 
+// Power-of-two nDFT: radix-2 Stockham FFT in LDS.  A workgroup takes 2 NF
+// rows of one block; rows 2p and 2p+1 form one complex sequence
+// z = y_a + i y_b, and after the FFT Z the two real spectra are
+//   Y_a[f] = (Z[f] + conj Z[N-f]) / 2,   Y_b[f] = (Z[f] - conj Z[N-f]) / 2i.
+// Stage with half-size m (m = 1, 2, ..., N/2), butterfly j < N/2, k = j mod m:
+//   a = A[j], b = A[j + N/2] w^(k N / 2m);  B[2j - k] = a + b, B[2j - k + m] = a - b.
+// Twiddles w^q = exp(-2 pi i q / N) come from the FP64-accurate tables.
+// Workgroup x runs on XCD x mod 8 (round-robin dispatch); the blocks of one
+// row group are given to consecutive workgroups of the same XCD, so that
+// their writes Q[f][row][0..nBlks) (nBlks consecutive complex values) merge
+// in that XCD's L2 and the overlapping input samples are read once from HBM.
+template <int NF>
+__global__ __launch_bounds__(256) void k_spod_fft(const float* __restrict__ X, int64_t ldx, int64_t M,
+                                                  const double* __restrict__ mean, const float* __restrict__ win,
+                                                  const float* __restrict__ twc, const float* __restrict__ tws,
+                                                  int nDFT, int logN, int step, int nBlks, int nf,
+                                                  const float* __restrict__ fscale, float2* __restrict__ Q) {
+  extern __shared__ float2 fsm[];
+  float2* A = fsm;                 // [NF][N]
+  float2* B = A + NF * nDFT;       // [NF][N]
+  float2* tw = B + NF * nDFT;      // [N / 2]
+  const int N = nDFT, H = nDFT / 2;
+  const int64_t kx = int64_t(blockIdx.x) >> 3;
+  const int64_t rg = (kx / nBlks) * 8 + (blockIdx.x & 7);
+  const int b = int(kx % nBlks);
+  const int64_t row0 = rg * (2 * NF);
+  if (row0 >= M) return;  // padding of the grid to whole XCD rounds (the whole workgroup leaves)
+  const int64_t t0 = int64_t(b) * step;
+  for (int q = threadIdx.x; q < H; q += 256) tw[q] = make_float2(twc[q], -tws[q]);
+  for (int idx = threadIdx.x; idx < NF * N; idx += 256) {
+    const int p = idx / N, t = idx - p * N;
+    const int64_t ra = row0 + 2 * p, rb = ra + 1;
+    float ya = 0.0f, yb = 0.0f;
+    if (ra < M) {
+      double x = double(X[ra * ldx + t0 + t]);
+      if (mean) x -= mean[ra];
+      ya = float(x) * win[t];
+    }
+    if (rb < M) {
+      double x = double(X[rb * ldx + t0 + t]);
+      if (mean) x -= mean[rb];
+      yb = float(x) * win[t];
+    }
+    A[idx] = make_float2(ya, yb);
+  }
+  __syncthreads();
+  for (int s = 0, m = 1; s < logN; s++, m <<= 1) {
+    const int tstride = N >> (s + 1);  // N / 2m
+    for (int idx = threadIdx.x; idx < NF * H; idx += 256) {
+      const int p = idx / H, j = idx - p * H;
+      const int k = j & (m - 1);
+      const float2 a = A[p * N + j], bv = A[p * N + j + H], w = tw[k * tstride];
+      const float2 wb = make_float2(w.x * bv.x - w.y * bv.y, w.x * bv.y + w.y * bv.x);
+      B[p * N + 2 * j - k] = make_float2(a.x + wb.x, a.y + wb.y);
+      B[p * N + 2 * j - k + m] = make_float2(a.x - wb.x, a.y - wb.y);
+    }
+    __syncthreads();
+    float2* t = A; A = B; B = t;
+  }
+  for (int idx = threadIdx.x; idx < NF * nf; idx += 256) {
+    const int p = idx / nf, f = idx - p * nf;
+    const float2 z = A[p * N + f], zr = A[p * N + ((N - f) & (N - 1))];
+    const float sc = fscale[f];
+    // Y_a = (z + conj zr) / 2,  Y_b = (z - conj zr) / 2i
+    const float2 ya = make_float2(0.5f * (z.x + zr.x), 0.5f * (z.y - zr.y));
+    const float2 yb = make_float2(0.5f * (z.y + zr.y), -0.5f * (z.x - zr.x));
+    const int64_t ra = row0 + 2 * p, rb = ra + 1;
+    if (ra < M) Q[(int64_t(f) * M + ra) * nBlks + b] = make_float2(ya.x * sc, ya.y * sc);
+    if (rb < M) Q[(int64_t(f) * M + rb) * nBlks + b] = make_float2(yb.x * sc, yb.y * sc);
+  }
+}
+
 }  // namespace
 
 extern "C" int pc_spod_blocks(pc_ctx* ctx, const float* X, int64_t ldx, int64_t M, int64_t N, int64_t nDFT,
@@ -140,10 +215,27 @@ extern "C" int pc_spod_blocks(pc_ctx* ctx, const float* X, int64_t ldx, int64_t 
   std::vector<float> hs(static_cast<size_t>(nf));
   for (int64_t f = 0; f < nf; f++) hs[size_t(f)] = float(base * ((f > 0 && f < nf - 1) ? 2.0 : 1.0));
   PC_HIP(ctx, hipMemcpyAsync(fsc, hs.data(), size_t(nf) * 4, hipMemcpyHostToDevice, st));
-  const size_t lds = (size_t(SPR) * TC + 2 * size_t(nDFT)) * 4;
-  dim3 g = dim3(unsigned(ceil_div(M, SPR)), unsigned(nBlks));
-  hipLaunchKernelGGL(k_spod_blocks, g, dim3(256), lds, st, X, ldx, M, remove_mean ? mean : nullptr, win, twc, tws,
-                     int(nDFT), int(step), int(nBlks), int(nf), fsc, (float2*)Q);
+  int logN = 0;
+  while ((int64_t(1) << logN) < nDFT) logN++;
+  const bool pow2 = (int64_t(1) << logN) == nDFT;
+  // LDS of the FFT: 2 NF N + N/2 complex, kept within 64 KB
+  auto fft = [&](auto nfc) {
+    constexpr int NF = decltype(nfc)::value;
+    const size_t lds = (size_t(2 * NF) * size_t(nDFT) + size_t(nDFT / 2)) * sizeof(float2);
+    const int64_t groups = round_up(ceil_div(M, 2 * NF), 8);
+    dim3 g = dim3(unsigned(groups * nBlks));
+    hipLaunchKernelGGL(k_spod_fft<NF>, g, dim3(256), lds, st, X, ldx, M, remove_mean ? mean : nullptr, win, twc,
+                       tws, int(nDFT), logN, int(step), int(nBlks), int(nf), fsc, (float2*)Q);
+  };
+  if (pow2 && nDFT <= 512) fft(std::integral_constant<int, 4>{});
+  else if (pow2 && nDFT == 1024) fft(std::integral_constant<int, 2>{});
+  else if (pow2 && nDFT == 2048) fft(std::integral_constant<int, 1>{});
+  else {
+    const size_t lds = (size_t(SPR) * TC + 2 * size_t(nDFT)) * 4;
+    dim3 g = dim3(unsigned(ceil_div(M, SPR)), unsigned(nBlks));
+    hipLaunchKernelGGL(k_spod_blocks, g, dim3(256), lds, st, X, ldx, M, remove_mean ? mean : nullptr, win, twc, tws,
+                       int(nDFT), int(step), int(nBlks), int(nf), fsc, (float2*)Q);
+  }
   PC_HIP(ctx, hipGetLastError());
   // the host fscale buffer must outlive the copy
   PC_HIP(ctx, hipStreamSynchronize(st));
