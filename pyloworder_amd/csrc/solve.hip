@@ -45,17 +45,26 @@ constexpr double kJacTolMx = 1e-10;  // Jacobi orthogonality tolerance, mixed pa
 // One-sided block Jacobi on H (rows x cols, column-major) with J accumulated:
 // one launch per round-robin step of a sweep, a check launch per sweep.  After
 // convergence every remaining launch exits immediately.
+// Block layout of the one-sided Jacobi of a rows x cols problem: real FP64
+// with rows <= 256 takes Gram-driven MFMA steps on 16-column blocks, other
+// problems column-pair rotations staged in LDS.  b = block width, nb2 = even
+// number of blocks (>= 2); nb2 == 2 means one block pair holds every column.
+template <class T>
+void jacobi_blocks(int rows, int cols, bool& gram_steps, int& b, int& nb2) {
+  gram_steps = std::is_same<T, double>::value && rows <= GROWS;
+  b = gram_steps ? GB : bjac_width(std::max(rows, cols), int(sizeof(T)));
+  nb2 = int(ceil_div(cols, b));
+  nb2 += nb2 & 1;
+  if (nb2 < 2) nb2 = 2;
+}
+
 template <class T>
 void launch_block_jacobi(pc_ctx* ctx, T* H, int ldh, int rows, int cols, T* J, int ldj, double* sig, int* state,
                          double* dstate, int* status, double min_tol) {
   hipStream_t st = ctx->stream;
-  // real FP64 with rows <= 256: Gram-driven MFMA steps on 16-column blocks;
-  // otherwise column-pair rotations staged in LDS
-  const bool gram_steps = std::is_same<T, double>::value && rows <= GROWS;
-  int b = gram_steps ? GB : bjac_width(std::max(rows, cols), int(sizeof(T)));
-  int nb2 = int(ceil_div(cols, b));
-  nb2 += nb2 & 1;
-  if (nb2 < 2) nb2 = 2;
+  bool gram_steps;
+  int b, nb2;
+  jacobi_blocks<T>(rows, cols, gram_steps, b, nb2);
   hipLaunchKernelGGL((k_bjac_init<T>), dim3(1), dim3(1024), 0, st, H, ldh, rows, cols, J, ldj, state, dstate,
                      kBjacMaxSweeps + 1);
   // one block pair (all columns in one workgroup): each launch iterates to
@@ -659,7 +668,20 @@ static int core_t(pc_ctx* ctx, const Plan& p, const void* A, int64_t lda, int ce
   hipLaunchKernelGGL((k_pad_identity<T>), dim3(1), dim3(256), 0, st, b.R2, n, n, b.rank);
   run_trinv<T>(st, b.R2, n, b.R2i);
   int nc = n;
-  if (mode == CORE_SVD) {
+  bool gram_steps_;
+  int jb_, jnb2_;
+  jacobi_blocks<T>(n, n, gram_steps_, jb_, jnb2_);
+  if (mode == CORE_SVD && jnb2_ == 2) {
+    // every column fits one block pair: the Jacobi launch iterates to
+    // convergence on its own, so M (Hc, column-major) is rotated directly and
+    // V = J; no preconditioner is needed
+    launch_block_jacobi<T>(ctx, b.Hc, n, n, n, b.J, n, b.sig, b.jstate, b.jdstate, b.status,
+                           p.mixed ? kJacTolMx : 0.0);
+    hipLaunchKernelGGL(k_sort_desc, dim3(1), dim3(1024), 0, st, b.sig, n, b.order);
+    hipLaunchKernelGGL((k_svd_finish<T, ST, typename VtType<T, ST>::type>), dim3(nblk(nn)), dim3(256), 0, st, b.Hc,
+                       n, n, n, b.sig, b.order, b.J, n, p.mixed ? kNullMx : kNullF64, (const int*)b.rank,
+                       (ST*)out.S, (typename VtType<T, ST>::type*)out.VT, out.ldvt, b.UM, n, b.nnull);
+  } else if (mode == CORE_SVD) {
     // Right preconditioner V0 = P3 Q4 (Drmac-Veselic): P3^T (M^H M) P3 = R3^H R3
     // (pivoted Cholesky), R3^H = Q4 R4 (Householder).  One-sided Jacobi on
     // M V0 takes the same rotations as on the lower-triangular R4^H and
