@@ -132,6 +132,10 @@ int pc_gemm(pc_ctx* ctx, pc_dtype dtype, int transA, int transB, int64_t m, int6
  * C[i,:] = v[i] * A[i,:] (v real for real dtypes, same type otherwise).    */
 int pc_vecmat(pc_ctx* ctx, pc_dtype dtype, const void* v, const void* A, int64_t lda, int64_t m,
               int64_t n, void* C, int64_t ldc);
+/* Y += alpha X (m x n blocks).  No reference function of its own: it is the
+ * sketch accumulation Yo += Yn of update_qr_streaming, pyLOM/vmmath/svd.py:217. */
+int pc_axpy(pc_ctx* ctx, pc_dtype dtype, double alpha, const void* X, int64_t ldx, void* Y, int64_t ldy,
+            int64_t m, int64_t n);
 /* transpose  pyLOM/vmmath/maths.py:17-29; src/vector_matrix.c:30-52.       */
 int pc_transpose(pc_ctx* ctx, pc_dtype dtype, const void* A, int64_t lda, int64_t m, int64_t n,
                  void* B, int64_t ldb);

@@ -19,7 +19,7 @@ __all__ = [
     "temporal_mean", "subtract_mean", "temporal_variance", "norm_variance",
     "transpose", "matmul", "matmulp", "vecmat", "compute_truncation_residual",
     "energy", "RMSE", "pod_run", "pod_truncate", "pod_reconstruct",
-    "pod_project", "hammwin", "spod_blocks", "spod_run",
+    "pod_project", "hammwin", "spod_blocks", "spod_run", "init_qr_streaming", "update_qr_streaming",
 ]
 
 
@@ -184,6 +184,57 @@ def randomized_qr(blocks, r: int, q: int, seed: int = -1, omega=None):
     Qi, _ = tsqr(Y)
     B = matmulp([qq.T for qq in Qi], blocks)
     return Qi, B
+
+
+def _qr_signs(blocks, positive):
+    """tsqr of the blocks; with ``positive`` the column signs are chosen so
+    that diag(R) >= 0 (the convention of the MI355X tsqr).  The reference
+    keeps LAPACK's signs (pyLOM/vmmath/svd.py:49-118 via np.linalg.qr)."""
+    Qi, R = tsqr(blocks)
+    if positive:
+        d = np.where(np.diag(R) < 0, -1.0, 1.0)
+        Qi = [qq * d for qq in Qi]
+    return Qi
+
+
+def init_qr_streaming(blocks, r: int, q: int, seed=None, omega=None, positive=False):
+    """pyLOM/vmmath/svd.py:175-197: randomized_qr that also returns the last
+    sketch Y (per rank).  Returns (Qi, B, Yi)."""
+    n = blocks[0].shape[1]
+    dt = blocks[0].dtype
+    if omega is None:
+        omega = reference_omega(n, r, int(time.time()) if seed is None else seed, dt)
+    Y = [b @ omega for b in blocks]
+    for _ in range(q):
+        Qi = _qr_signs(Y, positive)
+        Q2 = matmulp([b.T for b in blocks], Qi)
+        Y = [b @ Q2 for b in blocks]
+    Qi = _qr_signs(Y, positive)
+    B = matmulp([qq.T for qq in Qi], blocks)
+    return Qi, B, Y
+
+
+def update_qr_streaming(blocks, Q1, B1, Yo, r: int, q: int, omega=None, positive=False):
+    """pyLOM/vmmath/svd.py:199-225: fold a new batch of snapshots into the
+    sketch.  Yn = (A A^T)^q A Omega with Omega from numpy's global stream
+    (unseeded, svd.py:212); Yo += Yn (in place, as the reference); Q2 = tsqr(Yo);
+    B2 = [Q2^T Q1 B1, Q2^T A].  Returns (Q2, B2, Yo)."""
+    n = blocks[0].shape[1]
+    dt = blocks[0].dtype
+    if omega is None:
+        omega = np.random.rand(n, r).astype(dt)
+    Yn = [b @ omega for b in blocks]
+    for _ in range(q):
+        Qp = _qr_signs(Yn, positive)
+        O2 = matmulp([b.T for b in blocks], Qp)
+        Yn = [b @ O2 for b in blocks]
+    for yo, yn in zip(Yo, Yn):
+        yo += yn
+    Q2 = _qr_signs(Yo, positive)
+    Q2Q1 = matmulp([qq.T for qq in Q2], Q1)
+    B2o = Q2Q1 @ B1
+    B2n = matmulp([qq.T for qq in Q2], blocks)
+    return Q2, np.hstack((B2o, B2n)), Yo
 
 
 def randomized_svd(blocks, r: int, q: int, seed: int = -1, omega=None):

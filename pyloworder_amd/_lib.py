@@ -45,6 +45,7 @@ SIGNATURES = {
     "pc_randomized_svd": (ip, [vp, ip, vp, i64, i64, i64, i64, ip, C.c_uint64, vp, vp, i64, vp, vp, i64, ip]),
     "pc_gemm": (ip, [vp, ip, ip, ip, i64, i64, i64, C.c_double, vp, i64, vp, i64, C.c_double, vp, i64, ip]),
     "pc_vecmat": (ip, [vp, ip, vp, vp, i64, i64, i64, vp, i64]),
+    "pc_axpy": (ip, [vp, ip, C.c_double, vp, i64, vp, i64, i64, i64]),
     "pc_transpose": (ip, [vp, ip, vp, i64, i64, i64, vp, i64]),
     "pc_reconstruct": (ip, [vp, ip, vp, i64, vp, vp, i64, i64, i64, i64, vp, i64, ip]),
     "pc_project": (ip, [vp, ip, vp, i64, vp, i64, i64, i64, i64, vp, i64, ip]),

@@ -208,6 +208,37 @@ extern "C" int pc_vecmat(pc_ctx* ctx, pc_dtype dt, const void* v, const void* A,
   return PC_OK;
 }
 
+// Y[i][j] += alpha X[i][j] over an m x n block (complex data as interleaved reals)
+template <class T>
+__global__ __launch_bounds__(256) void k_axpy(T alpha, const T* __restrict__ X, int64_t ldx, T* __restrict__ Y,
+                                              int64_t ldy, int64_t m, int64_t n) {
+  const int64_t tot = m * n;
+  for (int64_t idx = int64_t(blockIdx.x) * 256 + threadIdx.x; idx < tot; idx += int64_t(gridDim.x) * 256) {
+    const int64_t i = idx / n, j = idx - i * n;
+    Y[i * ldy + j] += alpha * X[i * ldx + j];
+  }
+}
+
+extern "C" int pc_axpy(pc_ctx* ctx, pc_dtype dt, double alpha, const void* X, int64_t ldx, void* Y, int64_t ldy,
+                       int64_t m, int64_t n) {
+  if (!ctx) return pc_fail(nullptr, PC_ERR_INVALID, "pc_axpy: ctx is NULL");
+  PC_CHECK(ctx, check_mat(ctx, X, ldx, m, n, "pc_axpy X"));
+  PC_CHECK(ctx, check_mat(ctx, Y, ldy, m, n, "pc_axpy Y"));
+  if (m == 0 || n == 0) return PC_OK;
+  const int w = dtype_complex(dt) ? 2 : 1;  // complex rows as 2n reals
+  int64_t blocks = ceil_div(m * n * w, 256);
+  if (blocks > 65536) blocks = 65536;
+  dim3 g = dim3(unsigned(blocks));
+  if (dt == PC_F64 || dt == PC_C128)
+    hipLaunchKernelGGL(k_axpy<double>, g, dim3(256), 0, ctx->stream, alpha, (const double*)X, w * ldx, (double*)Y,
+                       w * ldy, m, w * n);
+  else
+    hipLaunchKernelGGL(k_axpy<float>, g, dim3(256), 0, ctx->stream, float(alpha), (const float*)X, w * ldx,
+                       (float*)Y, w * ldy, m, w * n);
+  PC_HIP(ctx, hipGetLastError());
+  return PC_OK;
+}
+
 extern "C" int pc_transpose(pc_ctx* ctx, pc_dtype dt, const void* A, int64_t lda, int64_t m, int64_t n,
                             void* B, int64_t ldb) {
   if (!ctx) return pc_fail(nullptr, PC_ERR_INVALID, "pc_transpose: ctx is NULL");

@@ -189,6 +189,25 @@ __global__ __launch_bounds__(256) void k_store_r(const T* Rm, int ldi, int n, TO
 }
 
 
+// QR sign convention: R := D^H R and Q := Q D with D = diag(R_ii / |R_ii|)
+// (1 where R_ii = 0), so that diag(R) is real and non-negative; Q R is unchanged
+template <class T>
+__global__ __launch_bounds__(256) void k_qr_phase(const T* Rm, int ldr, int n, T* ph) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const T d = Rm[int64_t(i) * ldr + i];
+  const double a = sqrt(abs2_(d));
+  ph[i] = a > 0.0 ? phase_(d, a) : T(1.0);
+}
+template <class T>
+__global__ __launch_bounds__(256) void k_qr_apply_phase(T* Rm, int ldr, T* Qm, int ldq, int n, const T* ph) {
+  const int64_t idx = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (idx >= int64_t(n) * n) return;
+  const int i = int(idx / n), j = int(idx % n);
+  Rm[int64_t(i) * ldr + j] = conj_(ph[i]) * Rm[int64_t(i) * ldr + j];
+  Qm[int64_t(i) * ldq + j] = Qm[int64_t(i) * ldq + j] * ph[j];
+}
+
 template <class T> struct SmallBufs {
   T *G1, *R1, *W1, *R1h, *G2, *Awk, *R2, *Hc, *J, *UM, *R2i, *tmp, *Wf;
   double* sig;
@@ -713,6 +732,10 @@ static int core_t(pc_ctx* ctx, const Plan& p, const void* A, int64_t lda, int ce
     // QR of M (n x n, rows >= rank zero): M = Q_M R_M;  R output, UM := Q_M
     hipLaunchKernelGGL((k_copy2d<T>), dim3(nblk(nn)), dim3(256), 0, st, b.Hc, n, n, n, b.tmp, n, 1);
     run_house_qr_sq<T>(st, b.tmp, n, b.UM, b.J, nwork);
+    // diag(R) real and non-negative (the phases go to Q_M's columns); tmp is free here
+    hipLaunchKernelGGL((k_qr_phase<T>), dim3(nblk(n)), dim3(256), 0, st, b.J, n, n, b.tmp);
+    hipLaunchKernelGGL((k_qr_apply_phase<T>), dim3(nblk(nn)), dim3(256), 0, st, b.J, n, b.UM, n, n,
+                       (const T*)b.tmp);
     hipLaunchKernelGGL((k_store_r<T, typename VtType<T, ST>::type>), dim3(nblk(nn)), dim3(256), 0, st, b.J, n, n,
                        (typename VtType<T, ST>::type*)out.VT, out.ldvt, b.rank, b.nnull);
   }

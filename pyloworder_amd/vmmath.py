@@ -24,7 +24,7 @@ from ._tensor import Arg, empty, pc_dtype, real_dtype
 from .cr import cr
 
 __all__ = [
-    "qr", "svd", "tsqr", "tsqr_svd", "tsqr_svd_grouped", "GroupedGraph", "randomized_qr", "randomized_svd", "matmul", "matmulp",
+    "qr", "svd", "tsqr", "tsqr_svd", "tsqr_svd_grouped", "GroupedGraph", "randomized_qr", "randomized_svd", "init_qr_streaming", "update_qr_streaming", "matmul", "matmulp",
     "vecmat", "transpose", "temporal_mean", "subtract_mean", "temporal_variance", "norm_variance",
     "compute_truncation_residual", "RMSE", "energy", "reference_omega",
 ]
@@ -247,6 +247,101 @@ def randomized_svd(Ai, r: int, q: int, seed: int = -1, omega=None, precision: st
                                         om.data_ptr(), U.data_ptr(), U.stride(0), S.data_ptr(), V.data_ptr(),
                                         V.stride(0), _prec(precision)))
     return a.give(U), a.give(S), a.give(V)
+
+
+def _dev(x, dtype=None):
+    """(device tensor, was_numpy): numpy inputs are copied to the GPU."""
+    if isinstance(x, np.ndarray):
+        t = torch.from_numpy(np.ascontiguousarray(x)).cuda()
+        return (t.to(dtype) if dtype is not None else t), True
+    t = x if x.is_cuda else x.cuda()
+    return (t.to(dtype) if dtype is not None and t.dtype != dtype else t), False
+
+
+def _gemm_into(opa, A, opb, B, C, beta, precision):
+    """C = op(A) op(B) + beta C on the device (pc_gemm); row-major operands,
+    op 1 = transpose.  Shapes are taken from C and op(A)."""
+    M, N = C.shape
+    K = A.shape[0] if opa else A.shape[1]
+    ctx = _ctx()
+    ctx.check(ctx.lib.pc_gemm(ctx.h, pc_dtype(C), opa, opb, M, N, K, 1.0, A.data_ptr(), A.stride(0), B.data_ptr(),
+                              B.stride(0), float(beta), C.data_ptr(), C.stride(0), _prec(precision)))
+    return C
+
+
+def _tsqr_q(Y, precision):
+    m, r = Y.shape
+    Q = empty((m, r), Y.dtype, Y)
+    R = empty((r, r), Y.dtype, Y)
+    ctx = _ctx()
+    ctx.check(ctx.lib.pc_tsqr(ctx.h, pc_dtype(Y), Y.data_ptr(), Y.stride(0), m, r, Q.data_ptr(), Q.stride(0),
+                              R.data_ptr(), R.stride(0), _prec(precision)))
+    return Q
+
+
+def _power_sketch(A, omega, q, precision):
+    """Y = (A A^T)^q A Omega (pyLOM/vmmath/svd.py:186-191, 212-216)."""
+    m, n = A.shape
+    r = omega.shape[1]
+    Y = _gemm_into(0, A, 0, omega, empty((m, r), A.dtype, A), 0.0, precision)
+    for _ in range(q):
+        Qp = _tsqr_q(Y, precision)
+        O2 = _gemm_into(1, A, 0, Qp, empty((n, r), A.dtype, A), 0.0, precision)
+        Y = _gemm_into(0, A, 0, O2, empty((m, r), A.dtype, A), 0.0, precision)
+    return Y
+
+
+@cr("math.init_qr_streaming")
+def init_qr_streaming(Ai, r: int, q: int, seed=None, omega=None, precision: str = "fp64"):
+    """Qi (m, r), B (r, n), Yi (m, r) — pyLOM/vmmath/svd.py:175-197 (the
+    randomized QR of a first batch of snapshots, keeping the sketch Y).
+    ``seed`` None draws a time-based seed as the reference does; ``omega``
+    injects the sketch matrix (n, r)."""
+    A, was_np = _dev(Ai)
+    if A.dtype not in (torch.float32, torch.float64):
+        raise TypeError("init_qr_streaming: real float32 / float64 data")
+    m, n = A.shape
+    if omega is None:
+        omega = reference_omega(n, r, int(time.time()) if seed is None else int(seed), np.float64)
+    om, _ = _dev(omega, A.dtype)
+    Y = _power_sketch(A, om, q, precision)
+    Q = _tsqr_q(Y, precision)
+    B = _gemm_into(1, Q, 0, A, empty((r, n), A.dtype, A), 0.0, precision)
+    if was_np:
+        return Q.cpu().numpy(), B.cpu().numpy(), Y.cpu().numpy()
+    return Q, B, Y
+
+
+@cr("math.update_qr_streaming")
+def update_qr_streaming(Ai, Q1, B1, Yo, r: int, q: int, omega=None, precision: str = "fp64"):
+    """Q2 (m, r), B2 (r, n_old + n), Yo (m, r) — pyLOM/vmmath/svd.py:199-225:
+    fold a new batch Ai (m, n) into the sketch.  Omega is drawn from numpy's
+    global stream (unseeded, as svd.py:212) unless ``omega`` is given.  Yo is
+    updated in place when it is a device tensor (the reference's Yo += Yn)."""
+    A, was_np = _dev(Ai)
+    m, n = A.shape
+    dt = A.dtype
+    Q1d, _ = _dev(Q1, dt)
+    B1d, _ = _dev(B1, dt)
+    Yd, y_np = _dev(Yo, dt)
+    if omega is None:
+        omega = np.random.rand(n, r)
+    om, _ = _dev(omega, dt)
+    Yn = _power_sketch(A, om, q, precision)
+    ctx = _ctx()
+    ctx.check(ctx.lib.pc_axpy(ctx.h, pc_dtype(Yd), 1.0, Yn.data_ptr(), Yn.stride(0), Yd.data_ptr(), Yd.stride(0), m, r))
+    Q2 = _tsqr_q(Yd, precision)
+    Q2Q1 = _gemm_into(1, Q2, 0, Q1d, empty((r, Q1d.shape[1]), dt, A), 0.0, precision)
+    nold = B1d.shape[1]
+    B2 = empty((r, nold + n), dt, A)
+    # B2 = [Q2^T Q1 B1, Q2^T A], both written in place into the column blocks
+    _gemm_into(0, Q2Q1, 0, B1d, B2[:, :nold], 0.0, precision)
+    _gemm_into(1, Q2, 0, A, B2[:, nold:], 0.0, precision)
+    if y_np and isinstance(Yo, np.ndarray):
+        Yo[...] = Yd.cpu().numpy()
+    if was_np:
+        return Q2.cpu().numpy(), B2.cpu().numpy(), (Yo if y_np else Yd)
+    return Q2, B2, Yd
 
 
 # ---------------------------------------------------------------------------

@@ -128,10 +128,21 @@ def test_tsqr_qr(rng):
     Q, R = pl.math.tsqr(A)
     Q, R = _np(Q), _np(R)
     assert np.abs(np.tril(R, -1)).max() == 0.0
+    assert np.all(np.diag(R) >= 0.0)  # the QR sign convention: diag(R) >= 0
     assert np.abs(Q @ R - A).max() <= 1e-12 * np.abs(A).max()
     assert metrics.orthogonality(Q) <= 1e-10
     Qo, Ro = oracle.tsqr(oracle.split_rows(A, 3))
     assert np.allclose(np.abs(R), np.abs(Ro), rtol=1e-9, atol=1e-12 * np.abs(Ro).max())
+
+
+def test_tsqr_complex_real_nonnegative_diagonal(rng):
+    A = (rng.standard_normal((3000, 12)) + 1j * rng.standard_normal((3000, 12))) @ np.diag(np.logspace(0, -3, 12))
+    Q, R = pl.math.tsqr(A)
+    Q, R = _np(Q), _np(R)
+    d = np.diag(R)
+    assert np.abs(d.imag).max() <= 1e-12 * np.abs(d).max() and np.all(d.real >= 0.0)
+    assert np.abs(Q @ R - A).max() <= 1e-12 * np.abs(A).max()
+    assert metrics.orthogonality(Q) <= 1e-10
 
 
 def test_dense_svd_square_and_wide(rng):
@@ -234,3 +245,44 @@ def test_centered_full_width_rank_deficient(precision, dt):
     assert metrics.orthogonality(U, k) <= (1e-10 if precision == "fp64" else 1e-4)
     assert metrics.orthogonality(U) <= (1e-10 if precision == "fp64" else 5e-2)
     assert metrics.recon_rmse(Y, U, S, V) <= (1e-12 if precision == "fp64" else 1e-5)
+
+
+@pytest.mark.parametrize("q", [0, 2])
+def test_streaming_qr_matches_oracle(q, rng):
+    """init_qr_streaming / update_qr_streaming (pyLOM/vmmath/svd.py:175-225)
+    against the oracle with the same Omega and the QR sign convention of this
+    build (diag(R) >= 0), on full-rank data with a decaying spectrum (every
+    sketch has full rank r): sketches and Q2 B2 to 1e-8 relative."""
+    m, r = 6000, 12
+    A1 = rng.standard_normal((m, 40)) @ np.diag(np.logspace(0, -2, 40))
+    A2 = rng.standard_normal((m, 30)) @ np.diag(np.logspace(0, -2, 30))
+    om1 = rng.random((40, r)); om2 = rng.random((30, r))
+    Q, B, Y = pl.math.init_qr_streaming(A1, r, q, omega=om1)
+    Qo, Bo, Yo_ = oracle.init_qr_streaming([A1], r, q, omega=om1, positive=True)
+    assert np.abs(Y - Yo_[0]).max() <= 1e-8 * np.abs(Yo_[0]).max()
+    assert np.abs(Q @ B - Qo[0] @ Bo).max() <= 1e-8 * np.abs(A1).max()
+    Yd = torch.from_numpy(Y.copy()).cuda()
+    Q2, B2, Y2 = pl.math.update_qr_streaming(A2, torch.from_numpy(Q).cuda(), torch.from_numpy(B).cuda(), Yd, r, q,
+                                              omega=om2)
+    assert Y2 is Yd  # updated in place on the device
+    Q2o, B2o, Y2o = oracle.update_qr_streaming([A2], Qo, Bo, [Yo_[0].copy()], r, q, omega=om2, positive=True)
+    Q2, B2, Y2 = (_np(t) for t in (Q2, B2, Y2))
+    assert B2.shape == (r, 70)
+    assert np.abs(Y2 - Y2o[0]).max() <= 1e-8 * np.abs(Y2o[0]).max()
+    assert np.abs(Q2 @ B2 - Q2o[0] @ B2o).max() <= 1e-8 * np.abs(A1).max()
+    assert np.abs(Q2.T @ Q2 - np.eye(r)).max() <= 1e-10
+
+
+def test_streaming_qr_recovers_low_rank_history(rng):
+    """rank(A1) + rank(A2) <= r and no power steps (q = 0): every sketch
+    column carries signal, Yo = A1 O1 + A2 O2 has rank 11, and Q2 B2
+    reproduces every snapshot seen.  (With power steps the sketch columns
+    beyond the rank are rounding-level; a rank-revealing QR treats them as
+    null, so exact recovery is not a property of the method then.)"""
+    m, r = 5000, 14
+    A1 = rng.standard_normal((m, 6)) @ rng.standard_normal((6, 40))
+    A2 = rng.standard_normal((m, 5)) @ rng.standard_normal((5, 30))
+    Q, B, Y = pl.math.init_qr_streaming(A1, r, 0, seed=7)
+    Q2, B2, _ = pl.math.update_qr_streaming(A2, Q, B, Y, r, 0)
+    X = np.hstack((A1, A2))
+    assert np.abs(Q2 @ B2 - X).max() <= 1e-9 * np.abs(X).max()

@@ -230,3 +230,50 @@ def test_metrics_detect_planted_errors():
     assert metrics.max_principal_angle(-U, U, 10) < 1e-12   # signs do not matter
     assert metrics.recon_rmse(X, U, S, V) < 1e-14
     assert metrics.recon_rmse(X, U, S, V.T) > 1e-2          # transposed operand is caught
+
+
+# ---------------------------------------------------------------------------
+# streaming randomized QR (pyLOM/vmmath/svd.py:175-225)
+# ---------------------------------------------------------------------------
+def test_init_qr_streaming_is_randomized_qr_plus_sketch():
+    rng = np.random.default_rng(21)
+    A = rng.standard_normal((300, 20)) @ np.diag(np.logspace(0, -3, 20))
+    Q, B, Y = oracle.init_qr_streaming([A], 6, 2, seed=4)
+    Q0, B0 = oracle.randomized_qr([A], 6, 2, seed=4)
+    assert np.array_equal(Q[0], Q0[0]) and np.array_equal(B, B0)
+    # the returned sketch is the last power iterate, and Q spans it
+    assert np.allclose(Q[0] @ (Q[0].T @ Y[0]), Y[0], atol=1e-10 * np.abs(Y[0]).max())
+
+
+def test_update_qr_streaming_recovers_low_rank_history():
+    """rank(A1) + rank(A2) <= r, no power steps: the updated basis reproduces
+    every snapshot seen so far, Q2 B2 = [A1, A2], whatever the sign
+    convention of the QR."""
+    rng = np.random.default_rng(22)
+    A1 = rng.standard_normal((400, 5)) @ rng.standard_normal((5, 30))
+    A2 = rng.standard_normal((400, 4)) @ rng.standard_normal((4, 25))
+    for positive in (False, True):
+        Q, B, Y = oracle.init_qr_streaming([A1], 12, 0, seed=3, positive=positive)
+        Yo = [Y[0].copy()]
+        Q2, B2, Yo2 = oracle.update_qr_streaming([A2], Q, B, Yo, 12, 0, omega=np.random.default_rng(5).random((25, 12)),
+                                                 positive=positive)
+        X = np.hstack((A1, A2))
+        assert np.abs(Q2[0] @ B2 - X).max() <= 1e-12 * np.abs(X).max()
+        assert Yo2 is Yo and B2.shape == (12, 55)  # Yo is updated in place
+        assert np.allclose(Q2[0].T @ Q2[0], np.eye(12), atol=1e-12)
+
+
+def test_update_qr_streaming_sign_convention_changes_the_sketch():
+    """Yo accumulates (A A^T)^q A Omega whose columns carry the QR's signs: the
+    two conventions give different sketches (the reason this path is compared
+    to the oracle under one stated convention)."""
+    rng = np.random.default_rng(23)
+    A1 = rng.standard_normal((200, 10))
+    A2 = rng.standard_normal((200, 8))
+    om1 = rng.random((10, 4)); om2 = rng.random((8, 4))
+    outs = []
+    for positive in (False, True):
+        Q, B, Y = oracle.init_qr_streaming([A1], 4, 1, omega=om1, positive=positive)
+        _, _, Yo = oracle.update_qr_streaming([A2], Q, B, [Y[0].copy()], 4, 1, omega=om2, positive=positive)
+        outs.append(Yo[0])
+    assert not np.allclose(outs[0], outs[1])
