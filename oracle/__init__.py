@@ -24,6 +24,11 @@ Pinning (how each function is checked, see tests/test_oracle.py):
     matmul / matmulp / vecmat / reconstruct / RMSE / energy: closed forms
     checked on hand-computed integer cases.
   * compute_truncation_residual: hand-computed cases of the three r regimes.
+  * init_qr_streaming / update_qr_streaming: the first equals randomized_qr
+    plus its last sketch; the update reproduces a low-rank history exactly
+    (Q2 B2 = [A1, A2] when rank(A1) + rank(A2) <= r, no power steps); a test
+    shows the sketch depends on the QR sign convention (``positive`` selects
+    diag(R) >= 0, the MI355X convention).
   * spod_run: a single-frequency travelling wave whose spectrum peaks at the
     known bin, Parseval-type energy identities, and a hand-built 2-block case.
 """
