@@ -40,7 +40,7 @@ constexpr int kBjacMaxSweeps = 48;
 __global__ void k_status_copy(const int* src, int* dst) {
   if (threadIdx.x < 4) dst[threadIdx.x] = src ? src[threadIdx.x] : 0;
 }
-constexpr double kJacTolMx = 1e-10;  // Jacobi orthogonality tolerance, mixed path
+constexpr double kJacTolMx = 1e-8;  // Jacobi orthogonality tolerance, mixed path
 
 // One-sided block Jacobi on H (rows x cols, column-major) with J accumulated:
 // one launch per round-robin step of a sweep, a check launch per sweep.  After
@@ -718,7 +718,7 @@ static int core_t(pc_ctx* ctx, const Plan& p, const void* A, int64_t lda, int ce
     hipLaunchKernelGGL((k_gemm_small<T>), gg, dim3(256), 0, st, 0, 0, n, n, n, T(1.0), b.tmp, n, (const int*)nullptr,
                        b.Awk, n, (const int*)nullptr, T(0.0), b.Hc, n, 1);
     // mixed precision: the factor carries ~1e-7 relative data error, so the
-    // columns need only be orthogonal to 1e-10 (sigma errs by O(tol^2))
+    // columns need only be orthogonal to 1e-8 (sigma errs by O(tol^2))
     launch_block_jacobi<T>(ctx, b.Hc, n, n, n, b.J, n, b.sig, b.jstate, b.jdstate, b.status,
                            p.mixed ? kJacTolMx : 0.0);
     // V = V0 J (column-major like J)
