@@ -108,75 +108,111 @@ __global__ __launch_bounds__(256) void k_spod_blocks(const float* __restrict__ X
   }
 }
 
-// Power-of-two nDFT: radix-2 Stockham FFT in LDS.  A workgroup takes 2 NF
-// rows of one block; rows 2p and 2p+1 form one complex sequence
-// z = y_a + i y_b, and after the FFT Z the two real spectra are
+// Power-of-two nDFT: radix-2 Stockham FFT in LDS.  A workgroup owns one row
+// pair and runs through the blocks in chunks of BC; rows a, b form one
+// complex sequence z = y_a + i y_b, and after the FFT Z the two real spectra
+// are
 //   Y_a[f] = (Z[f] + conj Z[N-f]) / 2,   Y_b[f] = (Z[f] - conj Z[N-f]) / 2i.
 // Stage with half-size m (m = 1, 2, ..., N/2), butterfly j < N/2, k = j mod m:
 //   a = A[j], b = A[j + N/2] w^(k N / 2m);  B[2j - k] = a + b, B[2j - k + m] = a - b.
-// Twiddles w^q = exp(-2 pi i q / N) come from the FP64-accurate tables.
-// Workgroup x runs on XCD x mod 8 (round-robin dispatch); the blocks of one
-// row group are given to consecutive workgroups of the same XCD, so that
-// their writes Q[f][row][0..nBlks) (nBlks consecutive complex values) merge
-// in that XCD's L2 and the overlapping input samples are read once from HBM.
-template <int NF>
+// Twiddles w^q = exp(-2 pi i q / N) come from the FP64-accurate tables.  The
+// spectra of a chunk are staged in LDS as [f][row][block], so that each
+// frequency's two rows leave as one contiguous run Q[f][ra][b0..b1),
+// Q[f][ra+1][b0..b1) (consecutive rows are adjacent in Q).
 __global__ __launch_bounds__(256) void k_spod_fft(const float* __restrict__ X, int64_t ldx, int64_t M,
                                                   const double* __restrict__ mean, const float* __restrict__ win,
                                                   const float* __restrict__ twc, const float* __restrict__ tws,
-                                                  int nDFT, int logN, int step, int nBlks, int nf,
+                                                  int nDFT, int logN, int step, int nBlks, int nf, int BC, int P,
                                                   const float* __restrict__ fscale, float2* __restrict__ Q) {
   extern __shared__ float2 fsm[];
-  float2* A = fsm;                 // [NF][N]
-  float2* B = A + NF * nDFT;       // [NF][N]
-  float2* tw = B + NF * nDFT;      // [N / 2]
   const int N = nDFT, H = nDFT / 2;
-  const int64_t kx = int64_t(blockIdx.x) >> 3;
-  const int64_t rg = (kx / nBlks) * 8 + (blockIdx.x & 7);
-  const int b = int(kx % nBlks);
-  const int64_t row0 = rg * (2 * NF);
-  if (row0 >= M) return;  // padding of the grid to whole XCD rounds (the whole workgroup leaves)
-  const int64_t t0 = int64_t(b) * step;
+  float2* A = fsm;              // [P][N]: P blocks transformed side by side
+  float2* B = A + P * N;        // [P][N]
+  float2* tw = B + P * N;       // [N / 2]
+  float2* stg = tw + H;         // [nf][2][BC]
+  // Row pairs are split into 8 contiguous ranges, one per XCD (workgroup x
+  // runs on XCD x mod 8), so neighbouring pairs, whose output runs share
+  // cache lines, are written through the same L2.
+  const int64_t nWG = gridDim.x, per = nWG >> 3, rem = nWG & 7;
+  const int64_t xcd = blockIdx.x & 7, xi = blockIdx.x >> 3;
+  const int64_t pair = nWG < 8 ? int64_t(blockIdx.x) : xcd * per + (xcd < rem ? xcd : rem) + xi;
+  const int64_t ra = pair * 2, rb = ra + 1;
+  const bool va = ra < M, vb = rb < M;
+  const double mua = (mean && va) ? mean[ra] : 0.0, mub = (mean && vb) ? mean[rb] : 0.0;
   for (int q = threadIdx.x; q < H; q += 256) tw[q] = make_float2(twc[q], -tws[q]);
-  for (int idx = threadIdx.x; idx < NF * N; idx += 256) {
-    const int p = idx / N, t = idx - p * N;
-    const int64_t ra = row0 + 2 * p, rb = ra + 1;
-    float ya = 0.0f, yb = 0.0f;
-    if (ra < M) {
-      double x = double(X[ra * ldx + t0 + t]);
-      if (mean) x -= mean[ra];
-      ya = float(x) * win[t];
+  // Samples of the next pass are fetched into registers while the current
+  // pass is transformed (at most P N / 256 <= 8 per thread and row).
+  constexpr int EMAX = 8;
+  float xa[EMAX], xb[EMAX];
+  auto fetch = [&](int gb, int np) {
+#pragma unroll
+    for (int u = 0; u < EMAX; u++) {
+      const int e = threadIdx.x + u * 256;
+      if (e < np * N) {
+        const int pi = e / N, t = e - pi * N;
+        const int64_t t0 = int64_t(gb + pi) * step + t;
+        xa[u] = va ? X[ra * ldx + t0] : 0.0f;
+        xb[u] = vb ? X[rb * ldx + t0] : 0.0f;
+      }
     }
-    if (rb < M) {
-      double x = double(X[rb * ldx + t0 + t]);
-      if (mean) x -= mean[rb];
-      yb = float(x) * win[t];
+  };
+  fetch(0, nBlks < P ? nBlks : P);
+  for (int b0 = 0; b0 < nBlks; b0 += BC) {
+    const int nb = nBlks - b0 < BC ? nBlks - b0 : BC;
+    for (int bp = 0; bp < nb; bp += P) {
+      const int np = nb - bp < P ? nb - bp : P;  // blocks in this pass
+#pragma unroll
+      for (int u = 0; u < EMAX; u++) {
+        const int e = threadIdx.x + u * 256;
+        if (e < np * N) {
+          const int t = e & (N - 1);
+          A[e] = make_float2(float(double(xa[u]) - mua) * win[t], float(double(xb[u]) - mub) * win[t]);
+        }
+      }
+      __syncthreads();
+      {  // next pass: the rest of this chunk, else the start of the next one
+        int gb = b0 + bp + P, cnt = nb - bp - P;
+        if (cnt <= 0) {
+          gb = b0 + BC;
+          cnt = nBlks - gb < BC ? nBlks - gb : BC;
+        }
+        if (cnt > 0) fetch(gb, cnt < P ? cnt : P);
+      }
+      float2* src = A;
+      float2* dst = B;
+      for (int s = 0, m = 1; s < logN; s++, m <<= 1) {
+        const int tstride = N >> (s + 1);  // N / 2m
+        for (int jj = threadIdx.x; jj < np * H; jj += 256) {
+          const int pi = jj / H, j = jj - pi * H, k = j & (m - 1);
+          const float2* sp = src + pi * N;
+          float2* dp = dst + pi * N;
+          const float2 a = sp[j], bv = sp[j + H], w = tw[k * tstride];
+          const float2 wb = make_float2(w.x * bv.x - w.y * bv.y, w.x * bv.y + w.y * bv.x);
+          dp[2 * j - k] = make_float2(a.x + wb.x, a.y + wb.y);
+          dp[2 * j - k + m] = make_float2(a.x - wb.x, a.y - wb.y);
+        }
+        __syncthreads();
+        float2* t = src; src = dst; dst = t;
+      }
+      for (int e = threadIdx.x; e < np * nf; e += 256) {
+        const int pi = e / nf, f = e - pi * nf, bi = bp + pi;
+        const float2 z = src[pi * N + f], zr = src[pi * N + ((N - f) & (N - 1))];
+        const float sc = fscale[f];
+        // Y_a = (z + conj zr) / 2,  Y_b = (z - conj zr) / 2i
+        stg[(f * 2 + 0) * BC + bi] = make_float2(0.5f * (z.x + zr.x) * sc, 0.5f * (z.y - zr.y) * sc);
+        stg[(f * 2 + 1) * BC + bi] = make_float2(0.5f * (z.y + zr.y) * sc, -0.5f * (z.x - zr.x) * sc);
+      }
+      __syncthreads();  // A is reloaded by the next pass
     }
-    A[idx] = make_float2(ya, yb);
-  }
-  __syncthreads();
-  for (int s = 0, m = 1; s < logN; s++, m <<= 1) {
-    const int tstride = N >> (s + 1);  // N / 2m
-    for (int idx = threadIdx.x; idx < NF * H; idx += 256) {
-      const int p = idx / H, j = idx - p * H;
-      const int k = j & (m - 1);
-      const float2 a = A[p * N + j], bv = A[p * N + j + H], w = tw[k * tstride];
-      const float2 wb = make_float2(w.x * bv.x - w.y * bv.y, w.x * bv.y + w.y * bv.x);
-      B[p * N + 2 * j - k] = make_float2(a.x + wb.x, a.y + wb.y);
-      B[p * N + 2 * j - k + m] = make_float2(a.x - wb.x, a.y - wb.y);
+    // frequency f: rows ra, ra + 1 x blocks b0 .. b0 + nb, one contiguous run
+    // when both rows exist (Q[f][row][b] with row stride nBlks)
+    const int run = 2 * nb;
+    for (int idx = threadIdx.x; idx < nf * run; idx += 256) {
+      const int f = idx / run, e = idx - f * run, r = e / nb, bi = e - r * nb;
+      const int64_t row = ra + r;
+      if (row < M) Q[(int64_t(f) * M + row) * nBlks + b0 + bi] = stg[(f * 2 + r) * BC + bi];
     }
-    __syncthreads();
-    float2* t = A; A = B; B = t;
-  }
-  for (int idx = threadIdx.x; idx < NF * nf; idx += 256) {
-    const int p = idx / nf, f = idx - p * nf;
-    const float2 z = A[p * N + f], zr = A[p * N + ((N - f) & (N - 1))];
-    const float sc = fscale[f];
-    // Y_a = (z + conj zr) / 2,  Y_b = (z - conj zr) / 2i
-    const float2 ya = make_float2(0.5f * (z.x + zr.x), 0.5f * (z.y - zr.y));
-    const float2 yb = make_float2(0.5f * (z.y + zr.y), -0.5f * (z.x - zr.x));
-    const int64_t ra = row0 + 2 * p, rb = ra + 1;
-    if (ra < M) Q[(int64_t(f) * M + ra) * nBlks + b] = make_float2(ya.x * sc, ya.y * sc);
-    if (rb < M) Q[(int64_t(f) * M + rb) * nBlks + b] = make_float2(yb.x * sc, yb.y * sc);
+    __syncthreads();  // the staging area is refilled by the next chunk
   }
 }
 
@@ -218,18 +254,20 @@ extern "C" int pc_spod_blocks(pc_ctx* ctx, const float* X, int64_t ldx, int64_t 
   int logN = 0;
   while ((int64_t(1) << logN) < nDFT) logN++;
   const bool pow2 = (int64_t(1) << logN) == nDFT;
-  // LDS of the FFT: 2 NF N + N/2 complex, kept within 64 KB
-  auto fft = [&](auto nfc) {
-    constexpr int NF = decltype(nfc)::value;
-    const size_t lds = (size_t(2 * NF) * size_t(nDFT) + size_t(nDFT / 2)) * sizeof(float2);
-    const int64_t groups = round_up(ceil_div(M, 2 * NF), 8);
-    dim3 g = dim3(unsigned(groups * nBlks));
-    hipLaunchKernelGGL(k_spod_fft<NF>, g, dim3(256), lds, st, X, ldx, M, remove_mean ? mean : nullptr, win, twc,
-                       tws, int(nDFT), logN, int(step), int(nBlks), int(nf), fsc, (float2*)Q);
-  };
-  if (pow2 && nDFT <= 512) fft(std::integral_constant<int, 4>{});
-  else if (pow2 && nDFT == 1024) fft(std::integral_constant<int, 2>{});
-  else if (pow2 && nDFT == 2048) fft(std::integral_constant<int, 1>{});
+  // LDS of the FFT: A, B (P blocks of N complex each, P = 512 / N so that the
+  // 256 threads all have a butterfly), twiddles (N/2) and the staging of BC
+  // blocks' spectra (nf x 2 x BC complex), kept within 56 KB
+  if (pow2 && nDFT <= 2048) {
+    int64_t P = nDFT >= 512 ? 1 : 512 / nDFT;
+    P = P > nBlks ? nBlks : P;
+    const int64_t fixed = (2 * P * nDFT + nDFT / 2) * int64_t(sizeof(float2));
+    int64_t bc = (57344 - fixed) / (nf * 2 * int64_t(sizeof(float2)));
+    bc = bc < 1 ? 1 : (bc > nBlks ? nBlks : bc);
+    const size_t lds = size_t(fixed + nf * 2 * bc * int64_t(sizeof(float2)));
+    hipLaunchKernelGGL(k_spod_fft, dim3(unsigned(ceil_div(M, 2))), dim3(256), lds, st, X, ldx, M,
+                       remove_mean ? mean : nullptr, win, twc, tws, int(nDFT), logN, int(step), int(nBlks), int(nf),
+                       int(bc), int(P), fsc, (float2*)Q);
+  }
   else {
     const size_t lds = (size_t(SPR) * TC + 2 * size_t(nDFT)) * 4;
     dim3 g = dim3(unsigned(ceil_div(M, SPR)), unsigned(nBlks));
