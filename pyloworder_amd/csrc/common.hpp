@@ -68,6 +68,13 @@ template <class T> PC_HD T mk_(double re, double im);
 template <> PC_HD double mk_<double>(double re, double) { return re; }
 template <> PC_HD cdouble mk_<cdouble>(double re, double im) { return cdouble(re, im); }
 
+// Batched launches of the small-stage kernels: blockIdx.z selects the problem
+// and each pointer moves by a fixed byte stride per problem (zb = 0 and
+// gridDim.z = 1 for a single problem).  Null pointers stay null.
+template <class P> PC_DEV P* zoff(P* p, int64_t zb) {
+  return p ? (P*)((const char*)p + int64_t(blockIdx.z) * zb) : p;
+}
+
 // ---------------------------------------------------------------------------
 // context
 // ---------------------------------------------------------------------------

@@ -25,7 +25,12 @@ PC_DEV int scale_exp(float mx) {
 // per 32-column block.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_pack_w(const double* __restrict__ W, int ldw, int NB, int NCB, h8* Wh, h8* Wl,
-                                                float* wscale, double* Wr, int ldwr) {
+                                                float* wscale, double* Wr, int ldwr, int64_t zb = 0) {
+  W = zoff(W, zb);
+  Wh = zoff(Wh, zb);
+  Wl = zoff(Wl, zb);
+  wscale = zoff(wscale, zb);
+  Wr = zoff(Wr, zb);
   int J = blockIdx.x;
   int ncb = NCB / 32;
   __shared__ float fsc[32];
@@ -65,8 +70,9 @@ __global__ __launch_bounds__(256) void k_pack_w(const double* __restrict__ W, in
 }  // namespace
 
 int launch_pack_w_mx(hipStream_t st, const double* W, int ldw, int n, int nc, int NB, int NCB, _Float16* Wh,
-                     _Float16* Wl, float* wscale, double* Wr, int ldwr) {
+                     _Float16* Wl, float* wscale, double* Wr, int ldwr, int nz, int64_t zb) {
   (void)n; (void)nc;
-  hipLaunchKernelGGL(k_pack_w, dim3(NCB / 32), dim3(256), 0, st, W, ldw, NB, NCB, (h8*)Wh, (h8*)Wl, wscale, Wr, ldwr);
+  hipLaunchKernelGGL(k_pack_w, dim3(NCB / 32, 1, nz), dim3(256), 0, st, W, ldw, NB, NCB, (h8*)Wh, (h8*)Wl, wscale, Wr,
+                     ldwr, zb);
   return 0;
 }

@@ -69,7 +69,8 @@ PC_DEV double block_max(double v, double* sh) {
 // accumulators combined in a fixed order, then lane 0 adds the 16 lane sums in
 // order: the summation order depends only on (chunks, NB), so results are
 // deterministic.  Entry e is read and written only by its own workgroup.
-__global__ __launch_bounds__(1024) void k_reduce_slab(double* P, int chunks, int64_t pstride, int NB, int nr) {
+__global__ __launch_bounds__(1024) void k_reduce_slab(double* P, int chunks, int64_t pstride, int NB, int nr, int64_t zb = 0) {
+  P = zoff(P, zb);
   __shared__ double red[16][65];
   const int le = threadIdx.x & 63, k = threadIdx.x >> 6;
   const int e = blockIdx.x * 64 + le;
@@ -100,7 +101,9 @@ __global__ __launch_bounds__(1024) void k_reduce_slab(double* P, int chunks, int
 }
 
 __global__ __launch_bounds__(256) void k_reduce_gram_real(const double* __restrict__ P, int chunks, int64_t pstride,
-                                                         int NB, int n, double* G, int ldg) {
+                                                         int NB, int n, double* G, int ldg, int64_t zb = 0) {
+  P = zoff(P, zb);
+  G = zoff(G, zb);
   int64_t idx = int64_t(blockIdx.x) * 256 + threadIdx.x;
   int64_t tot = int64_t(n) * n;
   if (idx >= tot) return;
@@ -114,7 +117,9 @@ __global__ __launch_bounds__(256) void k_reduce_gram_real(const double* __restri
 }
 
 __global__ __launch_bounds__(256) void k_reduce_gram_cplx(const double* __restrict__ P, int chunks, int64_t pstride,
-                                                         int NB, int n, cdouble* G, int ldg) {
+                                                         int NB, int n, cdouble* G, int ldg, int64_t zb = 0) {
+  P = zoff(P, zb);
+  G = zoff(G, zb);
   int64_t idx = int64_t(blockIdx.x) * 256 + threadIdx.x;
   int64_t tot = int64_t(n) * n;
   if (idx >= tot) return;
@@ -139,7 +144,10 @@ __global__ __launch_bounds__(256) void k_reduce_gram_cplx(const double* __restri
 // ---------------------------------------------------------------------------
 template <class T>
 __global__ __launch_bounds__(NT) void k_chol_shift(const T* G, int ldg, int n, double shift_rel, T* R, int ldr,
-                                                   int* status) {
+                                                   int* status, int64_t zb = 0) {
+  G = zoff(G, zb);
+  R = zoff(R, zb);
+  status = zoff(status, zb);
   __shared__ double sh[32];
   __shared__ double piv;
   __shared__ int restart;
@@ -201,7 +209,9 @@ __global__ __launch_bounds__(NT) void k_chol_shift(const T* G, int ldg, int n, d
 // waves in turn (back substitution, dot products over the solved part).
 // ---------------------------------------------------------------------------
 template <class T>
-__global__ __launch_bounds__(NT) void k_trinv_upper(const T* R, int ldr, int n, T* W, int ldw) {
+__global__ __launch_bounds__(NT) void k_trinv_upper(const T* R, int ldr, int n, T* W, int ldw, int64_t zb = 0) {
+  R = zoff(R, zb);
+  W = zoff(W, zb);
   for (int64_t idx = threadIdx.x; idx < int64_t(n) * n; idx += NT) {
     int i = int(idx / n), j = int(idx % n);
     if (i > j) W[int64_t(i) * ldw + j] = T(0.0);
@@ -234,7 +244,13 @@ __global__ __launch_bounds__(NT) void k_trinv_upper(const T* R, int ldr, int n, 
 // ---------------------------------------------------------------------------
 template <class T>
 __global__ __launch_bounds__(NT) void k_pchol(const T* G, int ldg, int n, double tol, T* A, T* R, int ldr, int* perm,
-                                              int* rank_out, int* status) {
+                                              int* rank_out, int* status, int64_t zb = 0) {
+  G = zoff(G, zb);
+  A = zoff(A, zb);
+  R = zoff(R, zb);
+  perm = zoff(perm, zb);
+  rank_out = zoff(rank_out, zb);
+  status = zoff(status, zb);
   __shared__ double sh[32];
   __shared__ int sidx[32];
   __shared__ int piv_s;
@@ -324,7 +340,12 @@ __global__ __launch_bounds__(NT) void k_pchol(const T* G, int ldg, int n, double
 template <class T>
 __global__ __launch_bounds__(256) void k_gemm_small(int opA, int opB, int M, int N, int K, T alpha, const T* A,
                                                     int lda, const int* acol, const T* B, int ldb, const int* brow,
-                                                    T beta, T* C, int ldc, int ctrans) {
+                                                    T beta, T* C, int ldc, int ctrans, int64_t zb = 0) {
+  A = zoff(A, zb);
+  acol = zoff(acol, zb);
+  B = zoff(B, zb);
+  brow = zoff(brow, zb);
+  C = zoff(C, zb);
   __shared__ T As[32][33];
   __shared__ T Bs[32][33];
   int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // ty 0..7
@@ -383,7 +404,11 @@ constexpr int BJ_LDS = 131072;
 
 template <class T>
 __global__ __launch_bounds__(NT) void k_bjac_init(const T* H, int ldh, int rows, int cols, T* J, int ldj, int* state,
-                                                  double* dstate, int nflags) {
+                                                  double* dstate, int nflags, int64_t zb = 0) {
+  H = zoff(H, zb);
+  J = zoff(J, zb);
+  state = zoff(state, zb);
+  dstate = zoff(dstate, zb);
   __shared__ double sh[32];
   for (int i = threadIdx.x; i < nflags; i += NT) state[i] = 0;
   for (int64_t idx = threadIdx.x; idx < int64_t(cols) * cols; idx += NT) {
@@ -500,7 +525,8 @@ __global__ __launch_bounds__(NT) void k_bjac_step(T* H, int ldh, int rows, int c
 }
 
 // close sweep `sweep`: no workgroup reported a rotation -> done
-__global__ void k_bjac_check(int* state, int sweep) {
+__global__ void k_bjac_check(int* state, int sweep, int64_t zb = 0) {
+  state = zoff(state, zb);
   if (threadIdx.x == 0 && !state[0] && state[1 + sweep] == 0) state[0] = 1;
 }
 
@@ -677,7 +703,11 @@ __global__ __launch_bounds__(NT) void k_bjac_gstep(double* H, int ldh, int rows,
 // the last inner sweep still rotated, as k_bjac_step does for one pair.
 template <class T>
 __global__ __launch_bounds__(64) void k_bjac_small(T* H, int ldh, int rows, int cols, T* J, int ldj, int b, int sweep,
-                                                   int* state, const double* dstate, double tol, int max_inner) {
+                                                   int* state, const double* dstate, double tol, int max_inner, int64_t zb = 0) {
+  H = zoff(H, zb);
+  J = zoff(J, zb);
+  state = zoff(state, zb);
+  dstate = zoff(dstate, zb);
   __shared__ T Hs[32 * 64];
   __shared__ T Js[32 * 32];
   if (state[0]) return;
@@ -779,7 +809,11 @@ __global__ __launch_bounds__(256) void k_permute_rows(const T* Qm, int ldq, int 
 // column norms, non-finite and non-convergence flags
 template <class T>
 __global__ __launch_bounds__(NT) void k_bjac_norms(const T* H, int ldh, int rows, int cols, double* sig,
-                                                   const int* state, int* status) {
+                                                   const int* state, int* status, int64_t zb = 0) {
+  H = zoff(H, zb);
+  sig = zoff(sig, zb);
+  state = zoff(state, zb);
+  status = zoff(status, zb);
   int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   for (int k = wv; k < cols; k += NT / 64) {
     double s = 0;
@@ -805,7 +839,9 @@ inline int bjac_width(int n, int esz) {
 // ---------------------------------------------------------------------------
 // order[k] = index of the k-th largest sig (stable; ties by index)
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(NT) void k_sort_desc(const double* sig, int n, int* order) {
+__global__ __launch_bounds__(NT) void k_sort_desc(const double* sig, int n, int* order, int64_t zb = 0) {
+  sig = zoff(sig, zb);
+  order = zoff(order, zb);
   // NaN keys rank as -inf, ties by index: the ranks always form a permutation,
   // so every order[] entry is written with a valid column index.
   for (int i = threadIdx.x; i < n; i += NT) {
@@ -960,7 +996,16 @@ template <class T, class ST, class VT_>
 __global__ __launch_bounds__(256) void k_svd_finish(const T* Hc, int ldh, int rows, int n, const double* sig,
                                                     const int* order, const T* J, int ldj, double null_tol,
                                                     const int* rank_in, ST* S, VT_* VT, int64_t ldvt, T* UM, int ldum,
-                                                    int* nnull) {
+                                                    int* nnull, int64_t zb = 0, int64_t zs = 0, int64_t zv = 0) {
+  Hc = zoff(Hc, zb);
+  sig = zoff(sig, zb);
+  order = zoff(order, zb);
+  J = zoff(J, zb);
+  rank_in = zoff(rank_in, zb);
+  UM = zoff(UM, zb);
+  nnull = zoff(nnull, zb);
+  S = zoff(S, zs);
+  VT = zoff(VT, zv);
   int64_t idx = int64_t(blockIdx.x) * 256 + threadIdx.x;
   double smax = sig[order[0]];
   int rk = rank_in ? rank_in[0] : n;

@@ -33,4 +33,4 @@ void launch_row_stats_f32(hipStream_t st, const float* X, int64_t ldx, int64_t m
 // per-column power-of-two scales; also writes the FP64 value of hi*2^-s (Wr)
 // (k_mfma_mx.hip)
 int launch_pack_w_mx(hipStream_t st, const double* W, int ldw, int n, int nc, int NB, int NCB, _Float16* Wh,
-                     _Float16* Wl, float* wscale, double* Wr, int ldwr);
+                     _Float16* Wl, float* wscale, double* Wr, int ldwr, int nz = 1, int64_t zb = 0);

@@ -37,7 +37,9 @@ inline unsigned nblk(int64_t count, int per = 256) { return unsigned(ceil_div(co
 constexpr int kBjacMaxSweeps = 48;
 
 // dst[0..3] = src ? src[0..3] : 0 (the four status words of a solve)
-__global__ void k_status_copy(const int* src, int* dst) {
+__global__ void k_status_copy(const int* src, int* dst, int64_t zs = 0, int64_t zd = 0) {
+  src = zoff(src, zs);
+  dst = zoff(dst, zd);
   if (threadIdx.x < 4) dst[threadIdx.x] = src ? src[threadIdx.x] : 0;
 }
 constexpr double kJacTolMx = 1e-8;  // Jacobi orthogonality tolerance, mixed path
@@ -60,13 +62,13 @@ void jacobi_blocks(int rows, int cols, bool& gram_steps, int& b, int& nb2) {
 
 template <class T>
 void launch_block_jacobi(pc_ctx* ctx, T* H, int ldh, int rows, int cols, T* J, int ldj, double* sig, int* state,
-                         double* dstate, int* status, double min_tol) {
+                         double* dstate, int* status, double min_tol, int nz = 1, int64_t zb = 0) {
   hipStream_t st = ctx->stream;
   bool gram_steps;
   int b, nb2;
   jacobi_blocks<T>(rows, cols, gram_steps, b, nb2);
-  hipLaunchKernelGGL((k_bjac_init<T>), dim3(1), dim3(1024), 0, st, H, ldh, rows, cols, J, ldj, state, dstate,
-                     kBjacMaxSweeps + 1);
+  hipLaunchKernelGGL((k_bjac_init<T>), dim3(1, 1, nz), dim3(1024), 0, st, H, ldh, rows, cols, J, ldj, state, dstate,
+                     kBjacMaxSweeps + 1, zb);
   // one block pair (all columns in one workgroup): each launch iterates to
   // convergence on its own, so a short fixed run of sweeps is queued and the
   // host never waits; several pairs: sweeps in host-checked batches
@@ -101,47 +103,48 @@ void launch_block_jacobi(pc_ctx* ctx, T* H, int ldh, int rows, int cols, T* J, i
         }
       }
       if (one_pair && rows <= 64 && 2 * b <= 32) {
-        hipLaunchKernelGGL((k_bjac_small<T>), dim3(1), dim3(64), 0, st, H, ldh, rows, cols, J, ldj, b, s, state, dstate,
-                           tol, max_inner);
+        hipLaunchKernelGGL((k_bjac_small<T>), dim3(1, 1, nz), dim3(64), 0, st, H, ldh, rows, cols, J, ldj, b, s, state,
+                           dstate, tol, max_inner, zb);
         continue;
       }
       hipLaunchKernelGGL((k_bjac_step<T>), dim3(nb2 / 2), dim3(1024), 0, st, H, ldh, rows, cols, J, ldj, b, nb2, step,
                          s, state, dstate, tol, max_inner);
     }
-    hipLaunchKernelGGL(k_bjac_check, dim3(1), dim3(64), 0, st, state, s);
+    hipLaunchKernelGGL(k_bjac_check, dim3(1, 1, nz), dim3(64), 0, st, state, s, zb);
   }
-  hipLaunchKernelGGL((k_bjac_norms<T>), dim3(1), dim3(1024), 0, st, H, ldh, rows, cols, sig, state, status);
+  hipLaunchKernelGGL((k_bjac_norms<T>), dim3(1, 1, nz), dim3(1024), 0, st, H, ldh, rows, cols, sig, state, status, zb);
 }
 
 
 // Small dense factorisations: blocked MFMA kernels for real n <= 256, the
 // generic single-workgroup kernels otherwise (complex, larger n).
 template <class T>
-void run_chol_shift(hipStream_t st, const T* G, int n, double shift, T* R, int* status) {
+void run_chol_shift(hipStream_t st, const T* G, int n, double shift, T* R, int* status, int nz = 1, int64_t zb = 0) {
   if constexpr (std::is_same<T, double>::value)
-    if (n <= dense::NMAX) {
+    if (n <= dense::NMAX && nz == 1) {
       hipLaunchKernelGGL(k_chol_fast, dim3(1), dim3(NT), 0, st, G, n, n, shift, R, n, status);
       return;
     }
-  hipLaunchKernelGGL((k_chol_shift<T>), dim3(1), dim3(NT), 0, st, G, n, n, shift, R, n, status);
+  hipLaunchKernelGGL((k_chol_shift<T>), dim3(1, 1, nz), dim3(NT), 0, st, G, n, n, shift, R, n, status, zb);
 }
 template <class T>
-void run_trinv(hipStream_t st, const T* R, int n, T* W) {
+void run_trinv(hipStream_t st, const T* R, int n, T* W, int nz = 1, int64_t zb = 0) {
   if constexpr (std::is_same<T, double>::value)
-    if (n <= dense::NMAX) {
+    if (n <= dense::NMAX && nz == 1) {
       hipLaunchKernelGGL(k_trinv_cols, dim3(n), dim3(64), 0, st, R, n, n, W, n);
       return;
     }
-  hipLaunchKernelGGL((k_trinv_upper<T>), dim3(1), dim3(NT), 0, st, R, n, n, W, n);
+  hipLaunchKernelGGL((k_trinv_upper<T>), dim3(1, 1, nz), dim3(NT), 0, st, R, n, n, W, n, zb);
 }
 template <class T>
-void run_pchol(hipStream_t st, const T* G, int n, double tol, T* A, T* R, int* perm, int* rank, int* status) {
+void run_pchol(hipStream_t st, const T* G, int n, double tol, T* A, T* R, int* perm, int* rank, int* status,
+               int nz = 1, int64_t zb = 0) {
   if constexpr (std::is_same<T, double>::value)
-    if (n <= dense::NMAX) {
+    if (n <= dense::NMAX && nz == 1) {
       hipLaunchKernelGGL(k_pchol_fast, dim3(1), dim3(NT), 0, st, G, n, n, tol, A, R, n, perm, rank, status);
       return;
     }
-  hipLaunchKernelGGL((k_pchol<T>), dim3(1), dim3(NT), 0, st, G, n, n, tol, A, R, n, perm, rank, status);
+  hipLaunchKernelGGL((k_pchol<T>), dim3(1, 1, nz), dim3(NT), 0, st, G, n, n, tol, A, R, n, perm, rank, status, zb);
 }
 // square Householder QR (n x n), Q and R explicit
 template <class T>
@@ -311,7 +314,14 @@ int64_t carve(WsCursor& c, const Plan& p, SmallBufs<T>& b, double** mean, double
 template <class T, class TS>
 __global__ __launch_bounds__(256) void k_null_lev(const TS* X, int64_t ldx, int64_t m, int n, int center,
                                                   const double* mean, const double* var, const T* W, int ldw,
-                                                  const int* nnull, int ncand, double* lev, T* urows) {
+                                                  const int* nnull, int ncand, double* lev, T* urows, int64_t zb = 0, int64_t zx = 0) {
+  X = zoff(X, zx);
+  mean = zoff(mean, zb);
+  var = zoff(var, zb);
+  W = zoff(W, zb);
+  nnull = zoff(nnull, zb);
+  lev = zoff(lev, zb);
+  urows = zoff(urows, zb);
   int d = nnull[0];
   if (d == 0) return;
   int c = blockIdx.x;
@@ -350,26 +360,28 @@ __global__ __launch_bounds__(256) void k_null_lev(const TS* X, int64_t ldx, int6
 
 template <class T>
 void launch_null_lev(hipStream_t st, const Plan& p, const void* A, int64_t lda, int center, const double* mean,
-                     const double* var, SmallBufs<T>& b, int ncand);
+                     const double* var, SmallBufs<T>& b, int ncand, int nz = 1, int64_t zb = 0, int64_t zx = 0);
 template <>
 void launch_null_lev<double>(hipStream_t st, const Plan& p, const void* A, int64_t lda, int center, const double* mean,
-                             const double* var, SmallBufs<double>& b, int ncand) {
+                             const double* var, SmallBufs<double>& b, int ncand, int nz,
+                             int64_t zb, int64_t zx) {
   if (p.dt == PC_F64)
-    hipLaunchKernelGGL((k_null_lev<double, double>), dim3(ncand), dim3(256), 0, st, (const double*)A, lda, p.m, p.n,
-                       center, mean, var, b.Wf, p.n, b.nnull, ncand, b.lev, b.urows);
+    hipLaunchKernelGGL((k_null_lev<double, double>), dim3(ncand, 1, nz), dim3(256), 0, st, (const double*)A, lda, p.m, p.n,
+                       center, mean, var, b.Wf, p.n, b.nnull, ncand, b.lev, b.urows, zb, zx);
   else
-    hipLaunchKernelGGL((k_null_lev<double, float>), dim3(ncand), dim3(256), 0, st, (const float*)A, lda, p.m, p.n,
-                       center, mean, var, b.Wf, p.n, b.nnull, ncand, b.lev, b.urows);
+    hipLaunchKernelGGL((k_null_lev<double, float>), dim3(ncand, 1, nz), dim3(256), 0, st, (const float*)A, lda, p.m, p.n,
+                       center, mean, var, b.Wf, p.n, b.nnull, ncand, b.lev, b.urows, zb, zx);
 }
 template <>
 void launch_null_lev<cdouble>(hipStream_t st, const Plan& p, const void* A, int64_t lda, int center, const double* mean,
-                              const double* var, SmallBufs<cdouble>& b, int ncand) {
+                              const double* var, SmallBufs<cdouble>& b, int ncand, int nz,
+                              int64_t zb, int64_t zx) {
   if (p.dt == PC_C128)
-    hipLaunchKernelGGL((k_null_lev<cdouble, cdouble>), dim3(ncand), dim3(256), 0, st, (const cdouble*)A, lda, p.m,
-                       p.n, center, mean, var, b.Wf, p.n, b.nnull, ncand, b.lev, b.urows);
+    hipLaunchKernelGGL((k_null_lev<cdouble, cdouble>), dim3(ncand, 1, nz), dim3(256), 0, st, (const cdouble*)A, lda, p.m,
+                       p.n, center, mean, var, b.Wf, p.n, b.nnull, ncand, b.lev, b.urows, zb, zx);
   else
-    hipLaunchKernelGGL((k_null_lev<cdouble, cfloat>), dim3(ncand), dim3(256), 0, st, (const cfloat*)A, lda, p.m, p.n,
-                       center, mean, var, b.Wf, p.n, b.nnull, ncand, b.lev, b.urows);
+    hipLaunchKernelGGL((k_null_lev<cdouble, cfloat>), dim3(ncand, 1, nz), dim3(256), 0, st, (const cfloat*)A, lda, p.m, p.n,
+                       center, mean, var, b.Wf, p.n, b.nnull, ncand, b.lev, b.urows, zb, zx);
 }
 
 // select the d lowest-leverage candidates, build the completion:
@@ -378,7 +390,15 @@ void launch_null_lev<cdouble>(hipStream_t st, const Plan& p, const void* A, int6
 template <class T>
 __global__ __launch_bounds__(1024) void k_null_build(int64_t m, int n, const int* nnull, int ncand, const double* lev,
                                                      const T* urows, T* W, int ldw, int* jsel, T* cnull, T* work,
-                                                     int* status) {
+                                                     int* status, int64_t zb = 0) {
+  nnull = zoff(nnull, zb);
+  lev = zoff(lev, zb);
+  urows = zoff(urows, zb);
+  W = zoff(W, zb);
+  jsel = zoff(jsel, zb);
+  cnull = zoff(cnull, zb);
+  work = zoff(work, zb);
+  status = zoff(status, zb);
   int d = nnull[0];
   if (d == 0) return;
   int g = n - d;
@@ -481,7 +501,9 @@ __global__ __launch_bounds__(256) void k_null_fix(const int* nnull, int64_t m, i
 
 // set unit diagonal on the rows >= rank (keeps R2 invertible)
 template <class T>
-__global__ void k_pad_identity(T* R, int ldr, int n, const int* rank) {
+__global__ void k_pad_identity(T* R, int ldr, int n, const int* rank, int64_t zb = 0) {
+  R = zoff(R, zb);
+  rank = zoff(rank, zb);
   int r = rank[0];
   for (int i = r + threadIdx.x; i < n; i += blockDim.x) R[int64_t(i) * ldr + i] = T(1.0);
 }
@@ -494,7 +516,9 @@ __global__ __launch_bounds__(256) void k_pad_real(const double* W, int ldw, int 
   P[idx] = (i < n && j < nc) ? W[int64_t(i) * ldw + j] : 0.0;
 }
 
-__global__ __launch_bounds__(256) void k_embed_pad(const cdouble* W, int ldw, int n, int nc, double* P, int NB, int NCB) {
+__global__ __launch_bounds__(256) void k_embed_pad(const cdouble* W, int ldw, int n, int nc, double* P, int NB, int NCB, int64_t zb = 0) {
+  W = zoff(W, zb);
+  P = zoff(P, zb);
   int64_t idx = int64_t(blockIdx.x) * 256 + threadIdx.x;
   if (idx >= int64_t(NB) * NCB) return;
   int i = int(idx / NCB), j = int(idx % NCB);
@@ -509,7 +533,8 @@ __global__ __launch_bounds__(256) void k_embed_pad(const cdouble* W, int ldw, in
 
 // M <- M C, C = I - 1 1^T / n  (M n x n, column-major: M[i][j] = Hc[j * ld + i])
 template <class T>
-__global__ __launch_bounds__(256) void k_center_right(T* Hc, int ld, int n) {
+__global__ __launch_bounds__(256) void k_center_right(T* Hc, int ld, int n, int64_t zb = 0) {
+  Hc = zoff(Hc, zb);
   int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
   T s = T(0.0);
@@ -525,7 +550,9 @@ __global__ __launch_bounds__(256) void k_unpad_real(const double* P, int NB, int
   int i = int(idx / n), j = int(idx % n);
   W[idx] = P[int64_t(i) * NB + j];
 }
-__global__ __launch_bounds__(256) void k_unembed(const double* P, int NB, int n, cdouble* W) {
+__global__ __launch_bounds__(256) void k_unembed(const double* P, int NB, int n, cdouble* W, int64_t zb = 0) {
+  P = zoff(P, zb);
+  W = zoff(W, zb);
   int64_t idx = int64_t(blockIdx.x) * 256 + threadIdx.x;
   if (idx >= int64_t(n) * n) return;
   int a = int(idx / n), b = int(idx % n);
@@ -586,33 +613,39 @@ static int launch_streams(pc_ctx* ctx, const Plan& p, const void* A, int64_t lda
 }
 
 template <class T>
-static void reduce_gram(hipStream_t st, const Plan& p, const double* part, int nparts, T* G);
+static void reduce_gram(hipStream_t st, const Plan& p, const double* part, int nparts, T* G, int nz = 1,
+                        int64_t zb = 0);
 // the chunk partials are summed in place into chunk 0 (k_reduce_slab), then
 // chunk 0 is unpacked into the symmetric / Hermitian n x n Gram
-static void reduce_slab(hipStream_t st, const Plan& p, const double* part, int nparts) {
+static void reduce_slab(hipStream_t st, const Plan& p, const double* part, int nparts, int nz, int64_t zb) {
   const int64_t e = int64_t(p.NB) * p.NB;
-  hipLaunchKernelGGL(k_reduce_slab, dim3(unsigned(ceil_div(e, 64))), dim3(1024), 0, st, const_cast<double*>(part),
-                     nparts, e, p.NB, p.nr);
+  hipLaunchKernelGGL(k_reduce_slab, dim3(unsigned(ceil_div(e, 64)), 1, nz), dim3(1024), 0, st,
+                     const_cast<double*>(part), nparts, e, p.NB, p.nr, zb);
 }
 template <>
-void reduce_gram<double>(hipStream_t st, const Plan& p, const double* part, int nparts, double* G) {
-  reduce_slab(st, p, part, nparts);
-  hipLaunchKernelGGL(k_reduce_gram_real, dim3(nblk(int64_t(p.n) * p.n)), dim3(256), 0, st, part, 1,
-                     int64_t(p.NB) * p.NB, p.NB, p.n, G, p.n);
+void reduce_gram<double>(hipStream_t st, const Plan& p, const double* part, int nparts, double* G, int nz,
+                         int64_t zb) {
+  reduce_slab(st, p, part, nparts, nz, zb);
+  hipLaunchKernelGGL(k_reduce_gram_real, dim3(nblk(int64_t(p.n) * p.n), 1, nz), dim3(256), 0, st, part, 1,
+                     int64_t(p.NB) * p.NB, p.NB, p.n, G, p.n, zb);
 }
 template <>
-void reduce_gram<cdouble>(hipStream_t st, const Plan& p, const double* part, int nparts, cdouble* G) {
-  reduce_slab(st, p, part, nparts);
-  hipLaunchKernelGGL(k_reduce_gram_cplx, dim3(nblk(int64_t(p.n) * p.n)), dim3(256), 0, st, part, 1,
-                     int64_t(p.NB) * p.NB, p.NB, p.n, G, p.n);
+void reduce_gram<cdouble>(hipStream_t st, const Plan& p, const double* part, int nparts, cdouble* G, int nz,
+                          int64_t zb) {
+  reduce_slab(st, p, part, nparts, nz, zb);
+  hipLaunchKernelGGL(k_reduce_gram_cplx, dim3(nblk(int64_t(p.n) * p.n), 1, nz), dim3(256), 0, st, part, 1,
+                     int64_t(p.NB) * p.NB, p.NB, p.n, G, p.n, zb);
 }
 
 template <class T>
-static void to_operand(hipStream_t st, const Plan& p, const T* W, int nc, SmallBufs<T>& b, bool split);
+static void to_operand(hipStream_t st, const Plan& p, const T* W, int nc, SmallBufs<T>& b, bool split, int nz = 1,
+                       int64_t zb = 0);
 // FP64 path: padded / embedded FP64 operand; mixed path: fp16 hi (and lo) + scales,
 // with Wr the FP64 value of the rounded hi part (used as the exact preconditioner)
 template <>
-void to_operand<double>(hipStream_t st, const Plan& p, const double* W, int nc, SmallBufs<double>& b, bool split) {
+void to_operand<double>(hipStream_t st, const Plan& p, const double* W, int nc, SmallBufs<double>& b, bool split,
+                        int nz, int64_t zb) {
+  if (nz != 1) return;  // batched solves are complex only
   if (!p.mixed) {
     hipLaunchKernelGGL(k_pad_real, dim3(nblk(int64_t(p.NB) * p.NB)), dim3(256), 0, st, W, p.n, p.n, nc, b.Wpad, p.NB,
                        p.NB);
@@ -623,19 +656,49 @@ void to_operand<double>(hipStream_t st, const Plan& p, const double* W, int nc, 
   }
 }
 template <>
-void to_operand<cdouble>(hipStream_t st, const Plan& p, const cdouble* W, int nc, SmallBufs<cdouble>& b, bool split) {
-  hipLaunchKernelGGL(k_embed_pad, dim3(nblk(int64_t(p.NB) * p.NB)), dim3(256), 0, st, W, p.n, p.n, nc, b.Wpad, p.NB,
-                     p.NB);
+void to_operand<cdouble>(hipStream_t st, const Plan& p, const cdouble* W, int nc, SmallBufs<cdouble>& b, bool split,
+                         int nz, int64_t zb) {
+  hipLaunchKernelGGL(k_embed_pad, dim3(nblk(int64_t(p.NB) * p.NB), 1, nz), dim3(256), 0, st, W, p.n, p.n, nc, b.Wpad,
+                     p.NB, p.NB, zb);
   if (p.mixed)
-    launch_pack_w_mx(st, b.Wpad, p.NB, p.NB, p.NB, p.NB, p.NB, b.Wh, split ? b.Wl : nullptr, b.wscale, b.Wr, p.NB);
+    launch_pack_w_mx(st, b.Wpad, p.NB, p.NB, p.NB, p.NB, p.NB, b.Wh, split ? b.Wl : nullptr, b.wscale, b.Wr, p.NB, nz,
+                     zb);
 }
 
-template <class T> static void rounded_back(hipStream_t st, const Plan& p, SmallBufs<T>& b, T* out);
-template <> void rounded_back<double>(hipStream_t st, const Plan& p, SmallBufs<double>& b, double* out) {
+template <class T>
+static void rounded_back(hipStream_t st, const Plan& p, SmallBufs<T>& b, T* out, int nz = 1, int64_t zb = 0);
+template <> void rounded_back<double>(hipStream_t st, const Plan& p, SmallBufs<double>& b, double* out, int nz,
+                                      int64_t zb) {
+  if (nz != 1) return;  // batched solves are complex only
   hipLaunchKernelGGL(k_unpad_real, dim3(nblk(int64_t(p.n) * p.n)), dim3(256), 0, st, b.Wr, p.NB, p.n, out);
 }
-template <> void rounded_back<cdouble>(hipStream_t st, const Plan& p, SmallBufs<cdouble>& b, cdouble* out) {
-  hipLaunchKernelGGL(k_unembed, dim3(nblk(int64_t(p.n) * p.n)), dim3(256), 0, st, b.Wr, p.NB, p.n, out);
+template <> void rounded_back<cdouble>(hipStream_t st, const Plan& p, SmallBufs<cdouble>& b, cdouble* out, int nz,
+                                       int64_t zb) {
+  hipLaunchKernelGGL(k_unembed, dim3(nblk(int64_t(p.n) * p.n), 1, nz), dim3(256), 0, st, b.Wr, p.NB, p.n, out, zb);
+}
+
+// P3: U = Y W (the operand prepared by to_operand), then the null-space fix of U
+template <class T, class ST>
+static int launch_p3(pc_ctx* ctx, const Plan& p, const void* A, int64_t lda, int center, double* mean, double* var,
+                     SmallBufs<T>& b, CoreOut out) {
+  hipStream_t st = ctx->stream;
+  int rc;
+  hipEvent_t ev3;
+  pc_prof_begin(ctx, &ev3);
+  pc_dtype sdt = p.cplx ? (p.dt == PC_C64 ? PC_F32 : PC_F64) : p.dt;
+  int64_t ldr = p.cplx ? 2 * lda : lda;
+  int64_t ldu = p.cplx ? 2 * out.ldu : out.ldu;
+  if (p.mixed)
+    rc = launch_gemm_mx2(st, (const float*)A, ldr, p.m, p.nr, center, mean, var, b.Wh, b.Wl, b.wscale, p.nr,
+                         p.chunks3, p.tpc3, (float*)out.U, ldu);
+  else
+    rc = launch_gemm_f64(st, sdt, A, ldr, p.m, p.nr, center, mean, var, b.Wpad, p.nr, p.chunks3, p.tpc3, out.U, ldu);
+  if (rc != 0) return pc_fail(ctx, PC_ERR_SHAPE, "no gemm variant for n = %d", p.n);
+  pc_prof_end(ctx, PCP_GEMM, ev3);
+  hipLaunchKernelGGL((k_null_fix<T, typename VtType<T, ST>::type>), dim3(nblk(int64_t(p.n) * p.n)), dim3(256), 0, st,
+                     b.nnull, p.m, p.n, b.jsel, b.cnull, (typename VtType<T, ST>::type*)out.U, out.ldu);
+  PC_HIP(ctx, hipGetLastError());
+  return PC_OK;
 }
 
 template <class T, class ST>
@@ -751,24 +814,7 @@ static int core_t(pc_ctx* ctx, const Plan& p, const void* A, int64_t lda, int ce
                      b.jsel, b.cnull, nwork, b.status);
   // ---- P3
   to_operand<T>(st, p, b.Wf, nc, b, true);
-  {
-    int rc;
-    hipEvent_t ev3;
-    pc_prof_begin(ctx, &ev3);
-    pc_dtype sdt = p.cplx ? (p.dt == PC_C64 ? PC_F32 : PC_F64) : p.dt;
-    int64_t ldr = p.cplx ? 2 * lda : lda;
-    int64_t ldu = p.cplx ? 2 * out.ldu : out.ldu;
-    if (p.mixed)
-      rc = launch_gemm_mx2(st, (const float*)A, ldr, p.m, p.nr, center, mean, var, b.Wh, b.Wl, b.wscale, p.nr,
-                           p.chunks3, p.tpc3, (float*)out.U, ldu);
-    else
-      rc = launch_gemm_f64(st, sdt, A, ldr, p.m, p.nr, center, mean, var, b.Wpad, p.nr, p.chunks3, p.tpc3, out.U, ldu);
-    if (rc != 0) return pc_fail(ctx, PC_ERR_SHAPE, "no gemm variant for n = %d", n);
-    pc_prof_end(ctx, PCP_GEMM, ev3);
-  }
-  hipLaunchKernelGGL((k_null_fix<T, typename VtType<T, ST>::type>), dim3(nblk(nn)), dim3(256), 0, st, b.nnull, p.m, n,
-                     b.jsel, b.cnull, (typename VtType<T, ST>::type*)out.U, out.ldu);
-  PC_HIP(ctx, hipGetLastError());
+  PC_CHECK(ctx, (launch_p3<T, ST>(ctx, p, A, lda, center, mean, var, b, out)));
   if (ctx->gstat) {  // grouped solve: statuses are read once for the whole group
     hipLaunchKernelGGL(k_status_copy, dim3(1), dim3(64), 0, st, b.status, ctx->gstat);
     PC_HIP(ctx, hipGetLastError());
@@ -778,6 +824,164 @@ static int core_t(pc_ctx* ctx, const Plan& p, const void* A, int64_t lda, int ce
   PC_HIP(ctx, hipMemcpyAsync(ctx->hflag, b.status, 16, hipMemcpyDeviceToHost, st));
   PC_HIP(ctx, hipStreamSynchronize(st));
   return status_error(ctx, ctx->hflag[0]);
+}
+
+// ---------------------------------------------------------------------------
+// Batched solve of a uniform group: every problem has the same shape, leading
+// dimensions and centring, and A, S and VT sit at constant strides (the
+// per-frequency blocks of a spectral POD).  The streaming passes run per
+// problem on the group streams; each small stage between them is one launch
+// over all problems (blockIdx.z = problem, workspace slices zb bytes apart).
+// Kernels and arguments are those of core_t's one-block-pair path, so each
+// result is bit-identical to a lone solve of that problem.
+// ---------------------------------------------------------------------------
+struct BatchDesc {
+  int64_t zA, zS, zV;  // byte strides of A, S and VT between consecutive problems
+};
+
+inline int64_t byte_diff(const void* a, const void* b) {
+  return int64_t(reinterpret_cast<const char*>(a) - reinterpret_cast<const char*>(b));
+}
+
+// complex SVD problems whose Jacobi is one block pair on one wavefront
+// (k_bjac_small), uniform as described above
+static bool batch_eligible(pc_dtype dt, const pc_problem* pr, int64_t count, BatchDesc& d) {
+  if (count < 2 || count > 65535 || !dtype_complex(dt)) return false;
+  const pc_problem& q = pr[0];
+  bool gs;
+  int jb, jnb2;
+  jacobi_blocks<cdouble>(int(q.n), int(q.n), gs, jb, jnb2);
+  if (!(jnb2 == 2 && q.n <= 64 && 2 * jb <= 32)) return false;
+  d.zA = byte_diff(pr[1].A, q.A);
+  d.zS = byte_diff(pr[1].S, q.S);
+  d.zV = byte_diff(pr[1].VT, q.VT);
+  if (d.zS == 0 || d.zV == 0) return false;  // outputs must not alias
+  for (int64_t i = 1; i < count; i++) {
+    const pc_problem& r = pr[i];
+    if (r.m != q.m || r.n != q.n || r.lda != q.lda || r.ldu != q.ldu || r.ldvt != q.ldvt || r.center != q.center)
+      return false;
+    if (byte_diff(r.A, q.A) != i * d.zA || byte_diff(r.S, q.S) != i * d.zS || byte_diff(r.VT, q.VT) != i * d.zV)
+      return false;
+  }
+  return true;
+}
+
+// the group streams of a grouped solve: st[s] is driven through sub-context
+// sub[s]; ev[0..ns] are the fork / join events
+struct GroupLanes {
+  int ns;
+  hipStream_t* st;
+  pc_ctx** sub;
+  hipEvent_t* ev;
+};
+
+// Runs the batch on ctx->stream (forking to and joining the group streams);
+// problem i's workspace slice starts at ctx->ws + base + i zb, its four status
+// words are copied to gstat + 4 i.
+template <class T, class ST>
+static int core_batch_t(pc_ctx* ctx, const Plan& p, const pc_problem* pr, int64_t count, const GroupLanes& gl,
+                        const BatchDesc& d, int64_t base, int64_t zb, int* gstat) {
+  const int ns = gl.ns;
+  hipStream_t st = ctx->stream;
+  const int n = p.n;
+  const int64_t nn = int64_t(n) * n;
+  const int nz = int(count);
+  const int center = pr[0].center;
+  const int64_t lda = pr[0].lda;
+  const int64_t slabs = std::max(p.nparts, p.mixed ? p.chunks3 : p.chunks);
+  auto carve_at = [&](int64_t i, SmallBufs<T>& bi, double** mi, double** vi, double** pi) {
+    WsCursor c{ctx->ws + base + i * zb, 0, zb};
+    carve<T>(c, p, bi, mi, vi, pi, slabs);
+    return c.take<T>(dense_work(n));
+  };
+  SmallBufs<T> b;
+  double *mean, *var, *part;
+  T* nwork = carve_at(0, b, &mean, &var, &part);
+  auto fork = [&]() -> int {
+    PC_HIP(ctx, hipEventRecord(gl.ev[ns], st));
+    for (int s = 0; s < ns; s++) PC_HIP(ctx, hipStreamWaitEvent(gl.st[s], gl.ev[ns], 0));
+    return PC_OK;
+  };
+  auto join = [&]() -> int {
+    for (int s = 0; s < ns; s++) {
+      PC_HIP(ctx, hipEventRecord(gl.ev[s], gl.st[s]));
+      PC_HIP(ctx, hipStreamWaitEvent(st, gl.ev[s], 0));
+    }
+    return PC_OK;
+  };
+  // one streaming pass per problem, round-robin over the group streams
+  auto per_problem = [&](int pass) -> int {
+    PC_CHECK(ctx, fork());
+    for (int64_t i = 0; i < count; i++) {
+      pc_ctx* sc = gl.sub[i % ns];
+      SmallBufs<T> bi;
+      double *mi, *vi, *pi;
+      carve_at(i, bi, &mi, &vi, &pi);
+      int r = PC_OK;
+      if (pass < 3) {
+        r = launch_streams<T>(sc, p, pr[i].A, lda, center, mi, vi, pi, bi, pass);
+      } else {
+        CoreOut oi{pr[i].U, pr[i].ldu, pr[i].S, pr[i].VT, pr[i].ldvt};
+        r = launch_p3<T, ST>(sc, p, pr[i].A, lda, center, mi, vi, bi, oi);
+      }
+      if (r != PC_OK) {
+        std::string e = sc->err;
+        (void)join();
+        return pc_fail(ctx, r, "problem %lld: %s", (long long)i, e.c_str());
+      }
+    }
+    return join();
+  };
+  const unsigned z = unsigned(nz);
+  hipLaunchKernelGGL(k_status_copy, dim3(1, 1, z), dim3(64), 0, st, nullptr, b.status, int64_t(0), zb);
+  // ---- P1
+  PC_CHECK(ctx, per_problem(1));
+  reduce_gram<T>(st, p, part, p.nparts, b.G1, nz, zb);
+  run_chol_shift<T>(st, b.G1, n, p.mixed ? kShiftMx : kShiftF64, b.R1, b.status, nz, zb);
+  run_trinv<T>(st, b.R1, n, b.W1, nz, zb);
+  to_operand<T>(st, p, b.W1, n, b, false, nz, zb);
+  if (p.mixed) rounded_back<T>(st, p, b, b.W1, nz, zb);
+  run_trinv<T>(st, b.W1, n, b.R1h, nz, zb);
+  PC_HIP(ctx, hipGetLastError());
+  // ---- P2
+  PC_CHECK(ctx, per_problem(2));
+  reduce_gram<T>(st, p, part, p.mixed ? p.chunks3 : p.chunks, b.G2, nz, zb);
+  run_pchol<T>(st, b.G2, n, p.mixed ? kTolMx : kTolF64, b.Awk, b.R2, b.perm, b.rank, b.status, nz, zb);
+  dim3 gg(unsigned(ceil_div(n, 32)), unsigned(ceil_div(n, 32)), z);
+  hipLaunchKernelGGL((k_gemm_small<T>), gg, dim3(256), 0, st, 0, 0, n, n, n, T(1.0), b.R2, n, (const int*)nullptr,
+                     b.R1h, n, (const int*)b.perm, T(0.0), b.Hc, n, 1, zb);
+  if (p.mixed && center)
+    hipLaunchKernelGGL((k_center_right<T>), dim3(nblk(n), 1, z), dim3(256), 0, st, b.Hc, n, n, zb);
+  hipLaunchKernelGGL((k_pad_identity<T>), dim3(1, 1, z), dim3(256), 0, st, b.R2, n, n, b.rank, zb);
+  run_trinv<T>(st, b.R2, n, b.R2i, nz, zb);
+  launch_block_jacobi<T>(ctx, b.Hc, n, n, n, b.J, n, b.sig, b.jstate, b.jdstate, b.status,
+                         p.mixed ? kJacTolMx : 0.0, nz, zb);
+  hipLaunchKernelGGL(k_sort_desc, dim3(1, 1, z), dim3(1024), 0, st, b.sig, n, b.order, zb);
+  hipLaunchKernelGGL((k_svd_finish<T, ST, typename VtType<T, ST>::type>), dim3(nblk(nn), 1, z), dim3(256), 0, st,
+                     b.Hc, n, n, n, b.sig, b.order, b.J, n, p.mixed ? kNullMx : kNullF64, (const int*)b.rank,
+                     (ST*)pr[0].S, (typename VtType<T, ST>::type*)pr[0].VT, pr[0].ldvt, b.UM, n, b.nnull, zb, d.zS,
+                     d.zV);
+  hipLaunchKernelGGL((k_gemm_small<T>), gg, dim3(256), 0, st, 0, 0, n, n, n, T(1.0), b.R2i, n, (const int*)nullptr,
+                     b.UM, n, (const int*)nullptr, T(0.0), b.tmp, n, 0, zb);
+  hipLaunchKernelGGL((k_gemm_small<T>), gg, dim3(256), 0, st, 0, 0, n, n, n, T(1.0), b.W1, n, (const int*)b.perm,
+                     b.tmp, n, (const int*)nullptr, T(0.0), b.Wf, n, 0, zb);
+  const int ncand = int(std::min<int64_t>(p.m, 4 * int64_t(n) + 16));
+  launch_null_lev<T>(st, p, pr[0].A, lda, (p.mixed && center == 1) ? 3 : center, mean, var, b, ncand, nz, zb, d.zA);
+  hipLaunchKernelGGL((k_null_build<T>), dim3(1, 1, z), dim3(1024), 0, st, p.m, n, b.nnull, ncand, b.lev, b.urows,
+                     b.Wf, n, b.jsel, b.cnull, nwork, b.status, zb);
+  to_operand<T>(st, p, b.Wf, n, b, true, nz, zb);
+  PC_HIP(ctx, hipGetLastError());
+  // ---- P3 (and the null-space fix of each U)
+  PC_CHECK(ctx, per_problem(3));
+  hipLaunchKernelGGL(k_status_copy, dim3(1, 1, z), dim3(64), 0, st, b.status, gstat, zb, int64_t(16));
+  PC_HIP(ctx, hipGetLastError());
+  return PC_OK;
+}
+
+static int core_batch(pc_ctx* ctx, pc_dtype dt, const Plan& p, const pc_problem* pr, int64_t count,
+                      const GroupLanes& gl, const BatchDesc& d, int64_t base, int64_t zb, int* gstat) {
+  if (dt == PC_C64) return core_batch_t<cdouble, float>(ctx, p, pr, count, gl, d, base, zb, gstat);
+  return core_batch_t<cdouble, double>(ctx, p, pr, count, gl, d, base, zb, gstat);
 }
 
 static int core_dispatch(pc_ctx* ctx, pc_dtype dt, const void* A, int64_t lda, int64_t m, int64_t n, int center,

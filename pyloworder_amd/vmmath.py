@@ -67,6 +67,25 @@ def tsqr_svd(Ai, precision: str = "fp64", center: int = 0):
 
 
 @cr("math.tsqr_svd_grouped")
+def _group_outputs(args, dt):
+    """Output tensors (U, S, V) per problem.  Problems of one shape get views of
+    three stacked tensors, so S and V sit at constant strides: the library then
+    runs the small stages of all problems as single launches."""
+    shapes = {tuple(a.t.shape) for a in args}
+    if len(shapes) == 1:
+        m, n = args[0].t.shape
+        k = len(args)
+        Ub = empty((k, m, n), dt, args[0].t)
+        Sb = empty((k, n), real_dtype(dt), args[0].t)
+        Vb = empty((k, n, n), dt, args[0].t)
+        return [(Ub[i], Sb[i], Vb[i]) for i in range(k)]
+    out = []
+    for a in args:
+        m, n = a.t.shape
+        out.append((empty((m, n), dt, a.t), empty((n,), real_dtype(dt), a.t), empty((n, n), dt, a.t)))
+    return out
+
+
 def tsqr_svd_grouped(As, precision: str = "fp64", center: int = 0):
     """tsqr_svd of every matrix of ``As`` in one grouped call (pc_grouped_tsqr_svd).
 
@@ -88,11 +107,10 @@ def tsqr_svd_grouped(As, precision: str = "fp64", center: int = 0):
             raise TypeError("tsqr_svd_grouped: all matrices must share one dtype")
     Us, Ss, Vs = [], [], []
     probs = (_lib.PCProblem * len(args))()
+    outs = _group_outputs(args, dt)
     for i, a in enumerate(args):
         m, n = a.t.shape
-        U = empty((m, n), dt, a.t)
-        S = empty((n,), real_dtype(dt), a.t)
-        V = empty((n, n), dt, a.t)
+        U, S, V = outs[i]
         Us.append(U)
         Ss.append(S)
         Vs.append(V)
@@ -125,11 +143,13 @@ class GroupedGraph:
         dt = self.args[0].t.dtype
         self.Us, self.Ss, self.Vs = [], [], []
         self._probs = (_lib.PCProblem * len(self.args))()
-        for i, a in enumerate(self.args):
+        for a in self.args:
             if a.t.dtype != dt:
                 raise TypeError("GroupedGraph: all matrices must share one dtype")
+        outs = _group_outputs(self.args, dt)
+        for i, a in enumerate(self.args):
             m, n = a.t.shape
-            U, S, V = empty((m, n), dt, a.t), empty((n,), real_dtype(dt), a.t), empty((n, n), dt, a.t)
+            U, S, V = outs[i]
             self.Us.append(U)
             self.Ss.append(S)
             self.Vs.append(V)

@@ -62,6 +62,37 @@ def test_grouped_complex64_uneven(precision, rng):
         assert metrics.max_principal_angle(U.astype(np.complex128), Uo[0], e["n_gated"]) <= 1e-4
 
 
+@pytest.mark.parametrize("dtype,precision", [(np.complex128, "fp64"), (np.complex64, "fp64"), (np.complex64, "mixed")])
+def test_uniform_complex_group_bitwise_equals_single(dtype, precision, rng):
+    """Equal shapes in one stacked tensor: the small stages of all members run
+    as single launches, and every member still equals its lone solve bit for
+    bit.  Member 3 has rank 4 (null-space completion of U)."""
+    k, m, n = 7, 3000, 15
+    A = (rng.standard_normal((k, m, n)) + 1j * rng.standard_normal((k, m, n))) @ np.diag(np.logspace(0, -3, n))
+    A[3] = A[3][:, :4] @ (rng.standard_normal((4, n)) + 1j * rng.standard_normal((4, n)))
+    At = torch.from_numpy(A.astype(dtype)).cuda()
+    Ug, Sg, Vg = pl.math.tsqr_svd_grouped(At, precision=precision)
+    for i in range(k):
+        U1, S1, V1 = pl.math.tsqr_svd(At[i], precision=precision)
+        assert torch.equal(Sg[i], S1) and torch.equal(Ug[i], U1) and torch.equal(Vg[i], V1), i
+    U3 = Ug[3].to(torch.complex128)
+    assert torch.allclose(U3.conj().T @ U3, torch.eye(n, dtype=torch.complex128, device="cuda"), atol=1e-5)
+
+
+def test_grouped_rejects_complex_centring(rng):
+    A = torch.zeros((3, 100, 4), dtype=torch.complex64, device="cuda")
+    with pytest.raises(pl.PodCoreError, match="problem 0: .*real data"):
+        pl.math.tsqr_svd_grouped(A, center=1)
+
+
+def test_uniform_group_error_names_the_problem(rng):
+    A = torch.from_numpy((rng.standard_normal((5, 800, 9)) + 1j * rng.standard_normal((5, 800, 9)))
+                         .astype(np.complex64)).cuda()
+    A[2, 17, 3] = float("nan")
+    with pytest.raises(pl.PodCoreError, match="problem 2"):
+        pl.math.tsqr_svd_grouped(A, precision="mixed")
+
+
 def test_grouped_errors_name_the_problem(rng):
     A = rng.standard_normal((100, 4))
     with pytest.raises(pl.PodCoreError, match="problem 1"):
