@@ -26,6 +26,11 @@ import time
 
 import numpy as np
 
+# Grouped and serving workloads spread independent solves over one stream per
+# hardware queue (the library reads this variable); HIP opens 4 queues per
+# process unless told otherwise.  Set before anything initialises HIP.
+os.environ["GPU_MAX_HW_QUEUES"] = "16"
+
 M_DEFAULT, N_DEFAULT = 8_000_000, 256
 HBM_PEAK_GBS = 8000.0
 
@@ -257,7 +262,7 @@ def run_grouped(a):
         t_spod = (time.perf_counter() - t0) / a.steps
         # the SVD stage alone: one grouped call over the 129 blocks
         Q, _, _ = pl.SPOD.blocks(X, t, nDFT, nolap)
-        for _ in range(a.warmup):
+        for _ in range(max(2, a.warmup)):  # the first calls size the workspaces
             pl.math.tsqr_svd_grouped(Q, precision=prec)
         sync()
         t0 = time.perf_counter()

@@ -91,12 +91,12 @@ struct pc_ctx {
   int* gstat = nullptr;   // grouped solve: where the current problem leaves its status words
   // grouped solve: internal streams (default priority) the problems are spread
   // over, and the fork / join events; created on first use
-  static constexpr int kGroupStreams = 4;
-  hipStream_t gs[kGroupStreams] = {nullptr, nullptr, nullptr, nullptr};
-  hipEvent_t gev[kGroupStreams + 1] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  static constexpr int kGroupStreams = 16;  // upper bound; see group_streams()
+  hipStream_t gs[kGroupStreams] = {};
+  hipEvent_t gev[kGroupStreams + 1] = {};
   // one sub-context per internal stream (own workspace and status words),
   // driven by its own host thread during a grouped solve
-  pc_ctx* sub[kGroupStreams] = {nullptr, nullptr, nullptr, nullptr};
+  pc_ctx* sub[kGroupStreams] = {};
   std::string err;
   // optional per-stage timing (pc_profile): events around each streaming pass
   int prof = 0;
