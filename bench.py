@@ -33,6 +33,7 @@ os.environ["GPU_MAX_HW_QUEUES"] = "16"
 
 M_DEFAULT, N_DEFAULT = 8_000_000, 256
 HBM_PEAK_GBS = 8000.0
+MFMA_F16_PEAK_TFS = 2500.0  # dense fp16 MFMA, MI355X
 
 
 def parse():
@@ -164,16 +165,37 @@ def measured_traffic(kernel, m, n):
     return None
 
 
+def pass_flops(name, m, n):
+    """fp16 MFMA flops of a mixed pass (n = 32 NBLK padded columns):
+    P1 G = Ys^T Ys, three split products on the NU = NBLK (NBLK + 1) / 2 upper
+    32 x 32 blocks: 3 NU 2048 m; P2 Z = Ys W1 (hi and lo of Ys against fp16
+    W1: 4 m n^2) plus the same three-product Gram of Z; P3 C = Ys W, three
+    products: 6 m n^2."""
+    nblk = -(-n // 32)
+    gram = 3 * (nblk * (nblk + 1) // 2) * 32 * 32 * 2 * m
+    return {"gram": gram, "zgram": 4 * m * (32 * nblk) ** 2 + gram, "gemm": 6 * m * (32 * nblk) ** 2}[name]
+
+
 def roofline_for(stage_ms, m, n, b):
-    """Dominant streaming kernel: algorithmic bytes per launch / its mean duration."""
+    """Dominant streaming kernel against the roofline that binds it: the larger
+    of its HBM time (algorithmic bytes / 8 TB/s) and its MFMA time (split
+    products / the fp16 dense peak).  Both fractions are reported."""
     algo = {"gram": m * n * b, "zgram": m * n * b, "gemm": 2 * m * n * b}
     kern = {"gram": "k_gram_mx2", "zgram": "k_zgram_mx2", "gemm": "k_gemm_mx2"}
     name = max(stage_ms, key=lambda k: stage_ms[k][0])
     t = stage_ms[name][0] * 1e-3
-    ach = algo[name] / t / 1e9
-    return name, {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                  "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": measured_traffic(kern[name], m, n),
-                  "kernel": kern[name], "algorithmic_bytes": algo[name], "avg_ms": round(t * 1e3, 3)}
+    gbs = algo[name] / t / 1e9
+    flops = pass_flops(name, m, n)
+    tfs = flops / t / 1e12
+    hbm_frac, mfma_frac = gbs / HBM_PEAK_GBS, tfs / MFMA_F16_PEAK_TFS
+    common = {"traffic": measured_traffic(kern[name], m, n), "kernel": kern[name], "algorithmic_bytes": algo[name],
+              "algorithmic_flops": flops, "avg_ms": round(t * 1e3, 3), "hbm_frac": round(hbm_frac, 4),
+              "mfma_frac": round(mfma_frac, 4)}
+    if flops / (MFMA_F16_PEAK_TFS * 1e12) > algo[name] / (HBM_PEAK_GBS * 1e9):
+        return name, {"bound": "mfma", "achieved": round(tfs, 1), "peak": MFMA_F16_PEAK_TFS, "unit": "TFLOP/s",
+                      "frac": round(mfma_frac, 4), **common}
+    return name, {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                  "frac": round(hbm_frac, 4), **common}
 
 
 def cpu_baseline(m_sample, n, seed=7):
