@@ -160,7 +160,14 @@ template <class TS, int NB, bool ZMODE>
 __global__ __launch_bounds__(NTH) void k_gram_f64(const TS* __restrict__ X, int64_t ldx, int64_t m, int n, int center,
                                                   int stats_in, const double* mean_in, const double* var_in,
                                                   double* mean_out, double* var_out, const double* __restrict__ W,
-                                                  int64_t tiles_per_chunk, double* __restrict__ part) {
+                                                  int64_t tiles_per_chunk, double* __restrict__ part, int64_t zx = 0, int64_t zb = 0) {
+  X = zoff(X, zx);
+  mean_in = zoff(mean_in, zb);
+  var_in = zoff(var_in, zb);
+  mean_out = zoff(mean_out, zb);
+  var_out = zoff(var_out, zb);
+  W = zoff(W, zb);
+  part = zoff(part, zb);
   constexpr int nb = NB / 16;
   constexpr int NBLK = nb * (nb + 1) / 2;
   constexpr int SL = (NBLK + 7) / 8;
@@ -211,7 +218,12 @@ template <class TS, class TO, int NB, int NCB>
 __global__ __launch_bounds__(NTH) void k_gemm_f64(const TS* __restrict__ X, int64_t ldx, int64_t m, int n, int center,
                                                   const double* mean_in, const double* var_in,
                                                   const double* __restrict__ W, int nc, int64_t tiles_per_chunk,
-                                                  TO* __restrict__ C, int64_t ldc) {
+                                                  TO* __restrict__ C, int64_t ldc, int64_t zx = 0, int64_t zb = 0, int64_t zc = 0) {
+  X = zoff(X, zx);
+  mean_in = zoff(mean_in, zb);
+  var_in = zoff(var_in, zb);
+  W = zoff(W, zb);
+  C = zoff(C, zc);
   constexpr int ncb = NCB / 16;
   constexpr int NBLK = 2 * ncb;
   constexpr int SLZ = (NBLK + 7) / 8;
@@ -332,40 +344,41 @@ int f64_tile_rows() { return TR; }
 template <class TS>
 static int launch_gram_t(hipStream_t st, const TS* X, int64_t ldx, int64_t m, int n, int center, int stats_in,
                          const double* mean_in, const double* var_in, double* mean_out, double* var_out,
-                         const double* W, int chunks, int64_t tpc, double* part) {
+                         const double* W, int chunks, int64_t tpc, double* part, const PassBatch& bz) {
   return dispatch_nb(n, [&](auto nbc) {
     constexpr int NB = decltype(nbc)::value;
+    const dim3 g(unsigned(chunks), 1, unsigned(bz.nz));
     if (W)
-      hipLaunchKernelGGL((k_gram_f64<TS, NB, true>), dim3(chunks), dim3(NTH), 0, st, X, ldx, m, n, center, stats_in,
-                         mean_in, var_in, mean_out, var_out, W, tpc, part);
+      hipLaunchKernelGGL((k_gram_f64<TS, NB, true>), g, dim3(NTH), 0, st, X, ldx, m, n, center, stats_in, mean_in,
+                         var_in, mean_out, var_out, W, tpc, part, bz.zx, bz.zb);
     else
-      hipLaunchKernelGGL((k_gram_f64<TS, NB, false>), dim3(chunks), dim3(NTH), 0, st, X, ldx, m, n, center, stats_in,
-                         mean_in, var_in, mean_out, var_out, W, tpc, part);
+      hipLaunchKernelGGL((k_gram_f64<TS, NB, false>), g, dim3(NTH), 0, st, X, ldx, m, n, center, stats_in, mean_in,
+                         var_in, mean_out, var_out, W, tpc, part, bz.zx, bz.zb);
     return 0;
   });
 }
 
 int launch_gram_f64(hipStream_t st, pc_dtype dt, const void* X, int64_t ldx, int64_t m, int n, int center,
                     int stats_in, const double* mean_in, const double* var_in, double* mean_out, double* var_out,
-                    const double* W, int chunks, int64_t tpc, double* part) {
+                    const double* W, int chunks, int64_t tpc, double* part, const PassBatch& bz) {
   if (dt == PC_F64)
     return launch_gram_t<double>(st, (const double*)X, ldx, m, n, center, stats_in, mean_in, var_in, mean_out, var_out,
-                                 W, chunks, tpc, part);
+                                 W, chunks, tpc, part, bz);
   return launch_gram_t<float>(st, (const float*)X, ldx, m, n, center, stats_in, mean_in, var_in, mean_out, var_out, W,
-                              chunks, tpc, part);
+                              chunks, tpc, part, bz);
 }
 
 template <class TS, class TO>
 static int launch_gemm_t(hipStream_t st, const TS* X, int64_t ldx, int64_t m, int n, int center,
                          const double* mean_in, const double* var_in, const double* W, int nc, int chunks,
-                         int64_t tpc, TO* C, int64_t ldc) {
+                         int64_t tpc, TO* C, int64_t ldc, const PassBatch& bz) {
   int ncb = f64_nb(nc);
   return dispatch_nb(n, [&](auto nbc) {
     constexpr int NB = decltype(nbc)::value;
     return dispatch_nb(ncb, [&](auto ncc) {
       constexpr int NCB = decltype(ncc)::value;
-      hipLaunchKernelGGL((k_gemm_f64<TS, TO, NB, NCB>), dim3(chunks), dim3(NTH), 0, st, X, ldx, m, n, center, mean_in,
-                         var_in, W, nc, tpc, C, ldc);
+      hipLaunchKernelGGL((k_gemm_f64<TS, TO, NB, NCB>), dim3(unsigned(chunks), 1, unsigned(bz.nz)), dim3(NTH), 0, st,
+                         X, ldx, m, n, center, mean_in, var_in, W, nc, tpc, C, ldc, bz.zx, bz.zb, bz.zc);
       return 0;
     });
   });
@@ -373,12 +386,12 @@ static int launch_gemm_t(hipStream_t st, const TS* X, int64_t ldx, int64_t m, in
 
 int launch_gemm_f64(hipStream_t st, pc_dtype dt, const void* X, int64_t ldx, int64_t m, int n, int center,
                     const double* mean_in, const double* var_in, const double* W, int nc, int chunks, int64_t tpc,
-                    void* C, int64_t ldc) {
+                    void* C, int64_t ldc, const PassBatch& bz) {
   if (dt == PC_F64)
     return launch_gemm_t<double, double>(st, (const double*)X, ldx, m, n, center, mean_in, var_in, W, nc, chunks, tpc,
-                                         (double*)C, ldc);
+                                         (double*)C, ldc, bz);
   return launch_gemm_t<float, float>(st, (const float*)X, ldx, m, n, center, mean_in, var_in, W, nc, chunks, tpc,
-                                     (float*)C, ldc);
+                                     (float*)C, ldc, bz);
 }
 
 template <class T>

@@ -168,7 +168,11 @@ template <int NB, bool VEC, int CENTER>
 __global__ __launch_bounds__(NT2, 1) void k_gram_mx2(const float* __restrict__ X, int64_t ldx, int64_t m, int n,
                                                      double* __restrict__ mean_io,
                                                      const double* __restrict__ var, int64_t tpc,
-                                                     double* __restrict__ part) {
+                                                     double* __restrict__ part, int64_t zx = 0, int64_t zb = 0) {
+  X = zoff(X, zx);
+  mean_io = zoff(mean_io, zb);
+  var = zoff(var, zb);
+  part = zoff(part, zb);
   constexpr int V = NB >= 128 ? 4 : NB / 32;
   constexpr int NG = NB / (32 * V);  // column groups of 32 V columns
   constexpr int NBLK = NB / 32;
@@ -370,7 +374,14 @@ __global__ __launch_bounds__(NT2, 1) void k_gemm_mx2(const float* __restrict__ X
                                                      const double* __restrict__ mean,
                                                      const double* __restrict__ var, const h8* __restrict__ Wh,
                                                      const h8* __restrict__ Wl, const float* __restrict__ wscale,
-                                                     int nc, int64_t tpc, float* __restrict__ C, int64_t ldc) {
+                                                     int nc, int64_t tpc, float* __restrict__ C, int64_t ldc, int64_t zx = 0, int64_t zb = 0, int64_t zc = 0) {
+  X = zoff(X, zx);
+  mean = zoff(mean, zb);
+  var = zoff(var, zb);
+  Wh = zoff(Wh, zb);
+  Wl = zoff(Wl, zb);
+  wscale = zoff(wscale, zb);
+  C = zoff(C, zc);
   using RT = RowTile<NB>;
   constexpr int KSW = RT::KSW, NCBLK = NB / 32;  // output columns padded like the input
   __shared__ __align__(16) char Fs[RT::FBYTES];
@@ -455,7 +466,13 @@ __global__ __launch_bounds__(NT2, 1) void k_zgram_mx2(const float* __restrict__ 
                                                       const double* __restrict__ mean,
                                                       const double* __restrict__ var, const h8* __restrict__ Wf,
                                                       const float* __restrict__ wscale, int64_t tpc,
-                                                      double* __restrict__ part) {
+                                                      double* __restrict__ part, int64_t zx = 0, int64_t zb = 0) {
+  X = zoff(X, zx);
+  mean = zoff(mean, zb);
+  var = zoff(var, zb);
+  Wf = zoff(Wf, zb);
+  wscale = zoff(wscale, zb);
+  part = zoff(part, zb);
   using RT = RowTile<NB>;
   constexpr int KSW = RT::KSW, NBLK = NB / 32;
   constexpr int NU = NBLK * (NBLK + 1) / 2;
@@ -596,12 +613,13 @@ int mx2_tile_rows(int n) {
 }
 
 int launch_gram_mx2(hipStream_t st, const float* X, int64_t ldx, int64_t m, int n, int center, double* mean_io,
-                    const double* var, int chunks, int64_t tpc, double* part) {
+                    const double* var, int chunks, int64_t tpc, double* part, const PassBatch& bz) {
   return dispatch_nb(n, [&](auto nbc) {
     constexpr int NB = decltype(nbc)::value;
-    return dispatch_vc(vec_ok(X, ldx, n), center, [&](auto vv, auto cc) {
-      hipLaunchKernelGGL((k_gram_mx2<NB, decltype(vv)::value, decltype(cc)::value>), dim3(chunks), dim3(NT2), 0, st,
-                         X, ldx, m, n, mean_io, var, tpc, part);
+    return dispatch_vc(vec_ok(X, ldx, n) && bz.zx % 16 == 0, center, [&](auto vv, auto cc) {
+      hipLaunchKernelGGL((k_gram_mx2<NB, decltype(vv)::value, decltype(cc)::value>),
+                         dim3(unsigned(chunks), 1, unsigned(bz.nz)), dim3(NT2), 0, st, X, ldx, m, n, mean_io, var, tpc,
+                         part, bz.zx, bz.zb);
       return 0;
     });
   });
@@ -609,12 +627,13 @@ int launch_gram_mx2(hipStream_t st, const float* X, int64_t ldx, int64_t m, int 
 
 int launch_zgram_mx2(hipStream_t st, const float* X, int64_t ldx, int64_t m, int n, int center, const double* mean,
                      const double* var, const _Float16* Wf, const float* wscale, int chunks, int64_t tpc,
-                     double* part) {
+                     double* part, const PassBatch& bz) {
   return dispatch_nb(n, [&](auto nbc) {
     constexpr int NB = decltype(nbc)::value;
-    return dispatch_vc(vec_ok(X, ldx, n), center, [&](auto vv, auto cc) {
-      hipLaunchKernelGGL((k_zgram_mx2<NB, decltype(vv)::value, decltype(cc)::value>), dim3(chunks), dim3(NT2), 0, st,
-                         X, ldx, m, n, mean, var, (const h8*)Wf, wscale, tpc, part);
+    return dispatch_vc(vec_ok(X, ldx, n) && bz.zx % 16 == 0, center, [&](auto vv, auto cc) {
+      hipLaunchKernelGGL((k_zgram_mx2<NB, decltype(vv)::value, decltype(cc)::value>),
+                         dim3(unsigned(chunks), 1, unsigned(bz.nz)), dim3(NT2), 0, st, X, ldx, m, n, mean, var,
+                         (const h8*)Wf, wscale, tpc, part, bz.zx, bz.zb);
       return 0;
     });
   });
@@ -623,13 +642,14 @@ int launch_zgram_mx2(hipStream_t st, const float* X, int64_t ldx, int64_t m, int
 // the W operand is packed NB x NB (nc <= n output columns, padded like the input)
 int launch_gemm_mx2(hipStream_t st, const float* X, int64_t ldx, int64_t m, int n, int center, const double* mean,
                     const double* var, const _Float16* Wh, const _Float16* Wl, const float* wscale, int nc,
-                    int chunks, int64_t tpc, float* C, int64_t ldc) {
+                    int chunks, int64_t tpc, float* C, int64_t ldc, const PassBatch& bz) {
   if (mx2_nb(nc) < 0 || mx2_nb(nc) > mx2_nb(n)) return -1;
   return dispatch_nb(n, [&](auto nbc) {
     constexpr int NB = decltype(nbc)::value;
-    return dispatch_vc(vec_ok(X, ldx, n), center, [&](auto vv, auto cc) {
-      hipLaunchKernelGGL((k_gemm_mx2<NB, decltype(vv)::value, decltype(cc)::value>), dim3(chunks), dim3(NT2), 0, st,
-                         X, ldx, m, n, mean, var, (const h8*)Wh, (const h8*)Wl, wscale, nc, tpc, C, ldc);
+    return dispatch_vc(vec_ok(X, ldx, n) && bz.zx % 16 == 0, center, [&](auto vv, auto cc) {
+      hipLaunchKernelGGL((k_gemm_mx2<NB, decltype(vv)::value, decltype(cc)::value>),
+                         dim3(unsigned(chunks), 1, unsigned(bz.nz)), dim3(NT2), 0, st, X, ldx, m, n, mean, var,
+                         (const h8*)Wh, (const h8*)Wl, wscale, nc, tpc, C, ldc, bz.zx, bz.zb, bz.zc);
       return 0;
     });
   });

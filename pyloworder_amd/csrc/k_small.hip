@@ -568,7 +568,10 @@ __global__ __launch_bounds__(NT) void k_bjac_gstep(double* H, int ldh, int rows,
   if (state[0]) return;
   const int RP = (rows + 15) & ~15;
   const int ld = RP + 1;
-  const int p = blockIdx.x;
+  // the launch holds 8 workgroups per pair and only every eighth works:
+  // workgroup x runs on XCD x mod 8, so the working ones share one L2
+  if (blockIdx.x & 7) return;
+  const int p = blockIdx.x >> 3;
   const int bi = rr_player(p, step, nb2), bj = rr_player(nb2 - 1 - p, step, nb2);
   const double negl = dstate[0];
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63, lr = l & 15, lk = l >> 4;

@@ -5,12 +5,18 @@
 // FP64 MFMA streaming kernels (k_mfma_f64.hip); return 0, or -1 when n is too large
 int f64_nb(int n);
 int f64_tile_rows();
+// Batched streaming passes: nz problems in one launch (blockIdx.z), the input
+// moving by zx bytes, the workspace operands by zb and the output by zc.
+struct PassBatch {
+  int nz = 1;
+  int64_t zx = 0, zb = 0, zc = 0;
+};
 int launch_gram_f64(hipStream_t st, pc_dtype dt, const void* X, int64_t ldx, int64_t m, int n, int center,
                     int stats_in, const double* mean_in, const double* var_in, double* mean_out, double* var_out,
-                    const double* W, int chunks, int64_t tpc, double* part);
+                    const double* W, int chunks, int64_t tpc, double* part, const PassBatch& bz = PassBatch());
 int launch_gemm_f64(hipStream_t st, pc_dtype dt, const void* X, int64_t ldx, int64_t m, int n, int center,
                     const double* mean_in, const double* var_in, const double* W, int nc, int chunks, int64_t tpc,
-                    void* C, int64_t ldc);
+                    void* C, int64_t ldc, const PassBatch& bz = PassBatch());
 int launch_xgram_f64(hipStream_t st, pc_dtype dt, const void* A, int64_t lda, int p, const void* B, int64_t ldb, int q,
                      int64_t m, int chunks, int64_t tpc, double* part);
 
@@ -19,13 +25,13 @@ int mx2_nb(int n);
 int mx2_gram_rows(int n);
 int mx2_tile_rows(int n);
 int launch_gram_mx2(hipStream_t st, const float* X, int64_t ldx, int64_t m, int n, int center, double* mean_io,
-                    const double* var, int chunks, int64_t tpc, double* part);
+                    const double* var, int chunks, int64_t tpc, double* part, const PassBatch& bz = PassBatch());
 int launch_zgram_mx2(hipStream_t st, const float* X, int64_t ldx, int64_t m, int n, int center, const double* mean,
                      const double* var, const _Float16* Wf, const float* wscale, int chunks, int64_t tpc,
-                     double* part);
+                     double* part, const PassBatch& bz = PassBatch());
 int launch_gemm_mx2(hipStream_t st, const float* X, int64_t ldx, int64_t m, int n, int center, const double* mean,
                     const double* var, const _Float16* Wh, const _Float16* Wl, const float* wscale, int nc,
-                    int chunks, int64_t tpc, float* C, int64_t ldc);
+                    int chunks, int64_t tpc, float* C, int64_t ldc, const PassBatch& bz = PassBatch());
 // FP64 row mean / variance of float32 rows (k_stream.hip)
 void launch_row_stats_f32(hipStream_t st, const float* X, int64_t ldx, int64_t m, int64_t n, double* mean,
                           double* var);

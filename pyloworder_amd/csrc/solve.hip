@@ -97,7 +97,7 @@ void launch_block_jacobi(pc_ctx* ctx, T* H, int ldh, int rows, int cols, T* J, i
     for (int step = 0; step < nb2 - 1; step++) {
       if constexpr (std::is_same<T, double>::value) {
         if (gram_steps) {
-          hipLaunchKernelGGL(k_bjac_gstep, dim3(nb2 / 2), dim3(1024), 0, st, H, ldh, rows, cols, J, ldj, nb2, step, s,
+          hipLaunchKernelGGL(k_bjac_gstep, dim3(nb2 / 2 * 8), dim3(1024), 0, st, H, ldh, rows, cols, J, ldj, nb2, step, s,
                              state, dstate, tol, max_inner);
           continue;
         }
@@ -487,7 +487,11 @@ __global__ __launch_bounds__(1024) void k_null_build(int64_t m, int n, const int
 
 template <class T, class TO>
 __global__ __launch_bounds__(256) void k_null_fix(const int* nnull, int64_t m, int n, const int* jsel, const T* cnull, TO* U,
-                                                  int64_t ldu) {
+                                                  int64_t ldu, int64_t zb = 0, int64_t zc = 0) {
+  nnull = zoff(nnull, zb);
+  jsel = zoff(jsel, zb);
+  cnull = zoff(cnull, zb);
+  U = zoff(U, zc);
   int d = nnull[0];
   int idx = blockIdx.x * 256 + threadIdx.x;
   if (idx >= d * d) return;
@@ -584,7 +588,7 @@ struct CoreOut {
 
 template <class T>
 static int launch_streams(pc_ctx* ctx, const Plan& p, const void* A, int64_t lda, int center, double* mean,
-                          double* var, double* part, SmallBufs<T>& b, int pass) {
+                          double* var, double* part, SmallBufs<T>& b, int pass, const PassBatch& bz = PassBatch()) {
   hipStream_t st = ctx->stream;
   int rc = 0;
   pc_dtype sdt = p.cplx ? (p.dt == PC_C64 ? PC_F32 : PC_F64) : p.dt;
@@ -594,17 +598,22 @@ static int launch_streams(pc_ctx* ctx, const Plan& p, const void* A, int64_t lda
   if (p.mixed) {
     // center 1: the kernels shift each row by its first element and P1 records
     // that shift in `mean`; center 2: exact FP64 statistics first
-    if (pass == 1 && center == 2) launch_row_stats_f32(st, (const float*)A, ldr, p.m, p.nr, mean, var);
-    if (pass == 1) rc = launch_gram_mx2(st, (const float*)A, ldr, p.m, p.nr, center, mean, var, p.chunks, p.tpc, part);
-    else rc = launch_zgram_mx2(st, (const float*)A, ldr, p.m, p.nr, center, mean, var, b.Wh, b.wscale, p.chunks3,
-                               p.tpc3, part);
+    if (pass == 1 && center == 2) {
+      if (bz.nz != 1) return pc_fail(ctx, PC_ERR_INVALID, "batched passes take no variance scaling");
+      launch_row_stats_f32(st, (const float*)A, ldr, p.m, p.nr, mean, var);
+    }
+    if (pass == 1)
+      rc = launch_gram_mx2(st, (const float*)A, ldr, p.m, p.nr, center, mean, var, p.chunks, p.tpc, part, bz);
+    else
+      rc = launch_zgram_mx2(st, (const float*)A, ldr, p.m, p.nr, center, mean, var, b.Wh, b.wscale, p.chunks3,
+                            p.tpc3, part, bz);
   } else {
     if (pass == 1)
       rc = launch_gram_f64(st, sdt, A, ldr, p.m, p.nr, center, 0, nullptr, nullptr, center ? mean : nullptr,
-                           center == 2 ? var : nullptr, nullptr, p.chunks, p.tpc, part);
+                           center == 2 ? var : nullptr, nullptr, p.chunks, p.tpc, part, bz);
     else
       rc = launch_gram_f64(st, sdt, A, ldr, p.m, p.nr, center, 1, mean, var, nullptr, nullptr, b.Wpad, p.chunks,
-                           p.tpc, part);
+                           p.tpc, part, bz);
   }
   if (rc != 0) return pc_fail(ctx, PC_ERR_SHAPE, "streaming kernel has no variant for n = %d", p.n);
   PC_HIP(ctx, hipGetLastError());
@@ -680,7 +689,7 @@ template <> void rounded_back<cdouble>(hipStream_t st, const Plan& p, SmallBufs<
 // P3: U = Y W (the operand prepared by to_operand), then the null-space fix of U
 template <class T, class ST>
 static int launch_p3(pc_ctx* ctx, const Plan& p, const void* A, int64_t lda, int center, double* mean, double* var,
-                     SmallBufs<T>& b, CoreOut out) {
+                     SmallBufs<T>& b, CoreOut out, const PassBatch& bz = PassBatch()) {
   hipStream_t st = ctx->stream;
   int rc;
   hipEvent_t ev3;
@@ -690,13 +699,15 @@ static int launch_p3(pc_ctx* ctx, const Plan& p, const void* A, int64_t lda, int
   int64_t ldu = p.cplx ? 2 * out.ldu : out.ldu;
   if (p.mixed)
     rc = launch_gemm_mx2(st, (const float*)A, ldr, p.m, p.nr, center, mean, var, b.Wh, b.Wl, b.wscale, p.nr,
-                         p.chunks3, p.tpc3, (float*)out.U, ldu);
+                         p.chunks3, p.tpc3, (float*)out.U, ldu, bz);
   else
-    rc = launch_gemm_f64(st, sdt, A, ldr, p.m, p.nr, center, mean, var, b.Wpad, p.nr, p.chunks3, p.tpc3, out.U, ldu);
+    rc = launch_gemm_f64(st, sdt, A, ldr, p.m, p.nr, center, mean, var, b.Wpad, p.nr, p.chunks3, p.tpc3, out.U, ldu,
+                         bz);
   if (rc != 0) return pc_fail(ctx, PC_ERR_SHAPE, "no gemm variant for n = %d", p.n);
   pc_prof_end(ctx, PCP_GEMM, ev3);
-  hipLaunchKernelGGL((k_null_fix<T, typename VtType<T, ST>::type>), dim3(nblk(int64_t(p.n) * p.n)), dim3(256), 0, st,
-                     b.nnull, p.m, p.n, b.jsel, b.cnull, (typename VtType<T, ST>::type*)out.U, out.ldu);
+  hipLaunchKernelGGL((k_null_fix<T, typename VtType<T, ST>::type>), dim3(nblk(int64_t(p.n) * p.n), 1, bz.nz),
+                     dim3(256), 0, st, b.nnull, p.m, p.n, b.jsel, b.cnull, (typename VtType<T, ST>::type*)out.U,
+                     out.ldu, bz.zb, bz.zc);
   PC_HIP(ctx, hipGetLastError());
   return PC_OK;
 }
@@ -828,15 +839,15 @@ static int core_t(pc_ctx* ctx, const Plan& p, const void* A, int64_t lda, int ce
 
 // ---------------------------------------------------------------------------
 // Batched solve of a uniform group: every problem has the same shape, leading
-// dimensions and centring, and A, S and VT sit at constant strides (the
-// per-frequency blocks of a spectral POD).  The streaming passes run per
-// problem on the group streams; each small stage between them is one launch
-// over all problems (blockIdx.z = problem, workspace slices zb bytes apart).
-// Kernels and arguments are those of core_t's one-block-pair path, so each
-// result is bit-identical to a lone solve of that problem.
+// dimensions and centring, and A, U, S and VT sit at constant strides (the
+// per-frequency blocks of a spectral POD).  Every kernel of the solve, the
+// streaming passes included, is one launch over all problems (blockIdx.z =
+// problem, workspace slices zb bytes apart).  Kernels and arguments are those
+// of core_t's one-block-pair path, so each result is bit-identical to a lone
+// solve of that problem.
 // ---------------------------------------------------------------------------
 struct BatchDesc {
-  int64_t zA, zS, zV;  // byte strides of A, S and VT between consecutive problems
+  int64_t zA, zU, zS, zV;  // byte strides of A, U, S and VT between consecutive problems
 };
 
 inline int64_t byte_diff(const void* a, const void* b) {
@@ -853,35 +864,26 @@ static bool batch_eligible(pc_dtype dt, const pc_problem* pr, int64_t count, Bat
   jacobi_blocks<cdouble>(int(q.n), int(q.n), gs, jb, jnb2);
   if (!(jnb2 == 2 && q.n <= 64 && 2 * jb <= 32)) return false;
   d.zA = byte_diff(pr[1].A, q.A);
+  d.zU = byte_diff(pr[1].U, q.U);
   d.zS = byte_diff(pr[1].S, q.S);
   d.zV = byte_diff(pr[1].VT, q.VT);
-  if (d.zS == 0 || d.zV == 0) return false;  // outputs must not alias
+  if (d.zU == 0 || d.zS == 0 || d.zV == 0) return false;  // outputs must not alias
   for (int64_t i = 1; i < count; i++) {
     const pc_problem& r = pr[i];
     if (r.m != q.m || r.n != q.n || r.lda != q.lda || r.ldu != q.ldu || r.ldvt != q.ldvt || r.center != q.center)
       return false;
-    if (byte_diff(r.A, q.A) != i * d.zA || byte_diff(r.S, q.S) != i * d.zS || byte_diff(r.VT, q.VT) != i * d.zV)
+    if (byte_diff(r.A, q.A) != i * d.zA || byte_diff(r.U, q.U) != i * d.zU || byte_diff(r.S, q.S) != i * d.zS ||
+        byte_diff(r.VT, q.VT) != i * d.zV)
       return false;
   }
   return true;
 }
 
-// the group streams of a grouped solve: st[s] is driven through sub-context
-// sub[s]; ev[0..ns] are the fork / join events
-struct GroupLanes {
-  int ns;
-  hipStream_t* st;
-  pc_ctx** sub;
-  hipEvent_t* ev;
-};
-
-// Runs the batch on ctx->stream (forking to and joining the group streams);
-// problem i's workspace slice starts at ctx->ws + base + i zb, its four status
-// words are copied to gstat + 4 i.
+// Runs the batch on ctx->stream: problem i's workspace slice starts at
+// ctx->ws + base + i zb, its four status words are copied to gstat + 4 i.
 template <class T, class ST>
-static int core_batch_t(pc_ctx* ctx, const Plan& p, const pc_problem* pr, int64_t count, const GroupLanes& gl,
-                        const BatchDesc& d, int64_t base, int64_t zb, int* gstat) {
-  const int ns = gl.ns;
+static int core_batch_t(pc_ctx* ctx, const Plan& p, const pc_problem* pr, int64_t count, const BatchDesc& d,
+                        int64_t base, int64_t zb, int* gstat) {
   hipStream_t st = ctx->stream;
   const int n = p.n;
   const int64_t nn = int64_t(n) * n;
@@ -889,62 +891,24 @@ static int core_batch_t(pc_ctx* ctx, const Plan& p, const pc_problem* pr, int64_
   const int center = pr[0].center;
   const int64_t lda = pr[0].lda;
   const int64_t slabs = std::max(p.nparts, p.mixed ? p.chunks3 : p.chunks);
-  auto carve_at = [&](int64_t i, SmallBufs<T>& bi, double** mi, double** vi, double** pi) {
-    WsCursor c{ctx->ws + base + i * zb, 0, zb};
-    carve<T>(c, p, bi, mi, vi, pi, slabs);
-    return c.take<T>(dense_work(n));
-  };
   SmallBufs<T> b;
   double *mean, *var, *part;
-  T* nwork = carve_at(0, b, &mean, &var, &part);
-  auto fork = [&]() -> int {
-    PC_HIP(ctx, hipEventRecord(gl.ev[ns], st));
-    for (int s = 0; s < ns; s++) PC_HIP(ctx, hipStreamWaitEvent(gl.st[s], gl.ev[ns], 0));
-    return PC_OK;
-  };
-  auto join = [&]() -> int {
-    for (int s = 0; s < ns; s++) {
-      PC_HIP(ctx, hipEventRecord(gl.ev[s], gl.st[s]));
-      PC_HIP(ctx, hipStreamWaitEvent(st, gl.ev[s], 0));
-    }
-    return PC_OK;
-  };
-  // one streaming pass per problem, round-robin over the group streams
-  auto per_problem = [&](int pass) -> int {
-    PC_CHECK(ctx, fork());
-    for (int64_t i = 0; i < count; i++) {
-      pc_ctx* sc = gl.sub[i % ns];
-      SmallBufs<T> bi;
-      double *mi, *vi, *pi;
-      carve_at(i, bi, &mi, &vi, &pi);
-      int r = PC_OK;
-      if (pass < 3) {
-        r = launch_streams<T>(sc, p, pr[i].A, lda, center, mi, vi, pi, bi, pass);
-      } else {
-        CoreOut oi{pr[i].U, pr[i].ldu, pr[i].S, pr[i].VT, pr[i].ldvt};
-        r = launch_p3<T, ST>(sc, p, pr[i].A, lda, center, mi, vi, bi, oi);
-      }
-      if (r != PC_OK) {
-        std::string e = sc->err;
-        (void)join();
-        return pc_fail(ctx, r, "problem %lld: %s", (long long)i, e.c_str());
-      }
-    }
-    return join();
-  };
+  WsCursor cur{ctx->ws + base, 0, zb};
+  carve<T>(cur, p, b, &mean, &var, &part, slabs);
+  T* nwork = cur.take<T>(dense_work(n));
+  const PassBatch pb{nz, d.zA, zb, d.zU};
   const unsigned z = unsigned(nz);
   hipLaunchKernelGGL(k_status_copy, dim3(1, 1, z), dim3(64), 0, st, nullptr, b.status, int64_t(0), zb);
   // ---- P1
-  PC_CHECK(ctx, per_problem(1));
+  PC_CHECK(ctx, launch_streams<T>(ctx, p, pr[0].A, lda, center, mean, var, part, b, 1, pb));
   reduce_gram<T>(st, p, part, p.nparts, b.G1, nz, zb);
   run_chol_shift<T>(st, b.G1, n, p.mixed ? kShiftMx : kShiftF64, b.R1, b.status, nz, zb);
   run_trinv<T>(st, b.R1, n, b.W1, nz, zb);
   to_operand<T>(st, p, b.W1, n, b, false, nz, zb);
   if (p.mixed) rounded_back<T>(st, p, b, b.W1, nz, zb);
   run_trinv<T>(st, b.W1, n, b.R1h, nz, zb);
-  PC_HIP(ctx, hipGetLastError());
   // ---- P2
-  PC_CHECK(ctx, per_problem(2));
+  PC_CHECK(ctx, launch_streams<T>(ctx, p, pr[0].A, lda, center, mean, var, part, b, 2, pb));
   reduce_gram<T>(st, p, part, p.mixed ? p.chunks3 : p.chunks, b.G2, nz, zb);
   run_pchol<T>(st, b.G2, n, p.mixed ? kTolMx : kTolF64, b.Awk, b.R2, b.perm, b.rank, b.status, nz, zb);
   dim3 gg(unsigned(ceil_div(n, 32)), unsigned(ceil_div(n, 32)), z);
@@ -970,18 +934,18 @@ static int core_batch_t(pc_ctx* ctx, const Plan& p, const pc_problem* pr, int64_
   hipLaunchKernelGGL((k_null_build<T>), dim3(1, 1, z), dim3(1024), 0, st, p.m, n, b.nnull, ncand, b.lev, b.urows,
                      b.Wf, n, b.jsel, b.cnull, nwork, b.status, zb);
   to_operand<T>(st, p, b.Wf, n, b, true, nz, zb);
-  PC_HIP(ctx, hipGetLastError());
   // ---- P3 (and the null-space fix of each U)
-  PC_CHECK(ctx, per_problem(3));
+  CoreOut o0{pr[0].U, pr[0].ldu, pr[0].S, pr[0].VT, pr[0].ldvt};
+  PC_CHECK(ctx, (launch_p3<T, ST>(ctx, p, pr[0].A, lda, center, mean, var, b, o0, pb)));
   hipLaunchKernelGGL(k_status_copy, dim3(1, 1, z), dim3(64), 0, st, b.status, gstat, zb, int64_t(16));
   PC_HIP(ctx, hipGetLastError());
   return PC_OK;
 }
 
 static int core_batch(pc_ctx* ctx, pc_dtype dt, const Plan& p, const pc_problem* pr, int64_t count,
-                      const GroupLanes& gl, const BatchDesc& d, int64_t base, int64_t zb, int* gstat) {
-  if (dt == PC_C64) return core_batch_t<cdouble, float>(ctx, p, pr, count, gl, d, base, zb, gstat);
-  return core_batch_t<cdouble, double>(ctx, p, pr, count, gl, d, base, zb, gstat);
+                      const BatchDesc& d, int64_t base, int64_t zb, int* gstat) {
+  if (dt == PC_C64) return core_batch_t<cdouble, float>(ctx, p, pr, count, d, base, zb, gstat);
+  return core_batch_t<cdouble, double>(ctx, p, pr, count, d, base, zb, gstat);
 }
 
 static int core_dispatch(pc_ctx* ctx, pc_dtype dt, const void* A, int64_t lda, int64_t m, int64_t n, int center,
