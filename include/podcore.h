@@ -202,6 +202,16 @@ int pc_graph_destroy(pc_graph* graph);
 int pc_spod_blocks(pc_ctx* ctx, const float* X, int64_t ldx, int64_t M, int64_t N, int64_t nDFT,
                    int64_t nolap, int remove_mean, void* Q);
 
+/* SPOD modes (pyLOM/SPOD/wrapper.py:84 and the sort of wrapper.py:90-94):
+ * P[(j*M + i)*nf + k] = Re U[order[k]][i][j], i.e. column k of the
+ * (nBlks*M) x nf matrix P is the real part of frequency order[k]'s M x nBlks
+ * left singular vectors flattened column-major.  U: nf complex64 M x nBlks
+ * matrices, contiguous (as pc_spod_blocks lays them out); order: nf int64
+ * indices on the device; P float32 (out_f64 = 0) or float64 (out_f64 = 1).
+ * Queued on the context stream; nBlks <= 12288.                               */
+int pc_spod_modes(pc_ctx* ctx, const void* U, int64_t M, int64_t nBlks, int64_t nf, const int64_t* order, void* P,
+                  int out_f64);
+
 #ifdef __cplusplus
 }
 #endif

@@ -3,7 +3,8 @@
 Signature and (L, P, f) outputs follow pyLOM/SPOD/wrapper.py:24-95.  The Welch
 blocks (mean removal, Hamming window, DFT, scaling) are one kernel
 (pc_spod_blocks); the nf per-frequency SVDs of the M x nBlks complex64 block
-matrices are one grouped call (pc_grouped_tsqr_svd) over a single workspace.
+matrices are one grouped call (pc_grouped_tsqr_svd) over a single workspace;
+the sorted real parts of the modes are one more kernel (pc_spod_modes).
 """
 from __future__ import annotations
 
@@ -116,8 +117,10 @@ def run(X, t, nDFT: int = 0, nolap: int = 0, remove_mean: bool = True, precision
     L = (S.double() * S.double()).to(real)
     order = np.argsort(L[:, 0].cpu().numpy())[::-1].copy()
     oi = torch.from_numpy(order).to(dev)
-    # P[j*M + i, f] = Re U_f[i, j]  (column-major flattening, wrapper.py:84)
-    P = U.real.permute(2, 1, 0).index_select(2, oi).reshape(nBlks * M, nf).to(real)
+    # P[j*M + i, k] = Re U_order[k][i, j]  (column-major flattening, wrapper.py:84)
+    P = torch.empty((nBlks * M, nf), dtype=real, device=dev)
+    ctx.check(ctx.lib.pc_spod_modes(ctx.h, U.data_ptr(), M, nBlks, nf, oi.data_ptr(), P.data_ptr(),
+                                    1 if real == torch.float64 else 0))
     L = L.index_select(0, oi)
     f = f[order]
     cr_stop("SPOD.sort", 0)

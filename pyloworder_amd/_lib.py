@@ -53,6 +53,7 @@ SIGNATURES = {
     "pc_truncation_residual": (ip, [C.POINTER(C.c_double), i64, C.c_double, C.POINTER(i64)]),
     "pc_grouped_tsqr_svd": (ip, [vp, ip, vp, i64, ip]),
     "pc_spod_blocks": (ip, [vp, vp, i64, i64, i64, i64, i64, ip, vp]),
+    "pc_spod_modes": (ip, [vp, vp, i64, i64, i64, vp, vp, ip]),
     "pc_grouped_graph_create": (ip, [vp, ip, vp, i64, ip, C.POINTER(vp)]),
     "pc_grouped_graph_launch": (ip, [vp, vp]),
     "pc_graph_destroy": (ip, [vp]),

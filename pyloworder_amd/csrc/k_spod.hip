@@ -8,6 +8,7 @@
 // (wrapper.py:73-74 and the 1/sqrt(nBlks) of wrapper.py:82).  Q[f] is the
 // M x nBlks complex64 matrix of frequency f with contiguous rows, i.e. exactly
 // the operand the per-frequency tall-skinny SVD reads.
+#include <algorithm>
 #include <cmath>
 #include <type_traits>
 #include <vector>
@@ -216,7 +217,61 @@ __global__ __launch_bounds__(256) void k_spod_fft(const float* __restrict__ X, i
   }
 }
 
+// SPOD modes: P[(j M + i) nf + k] = Re U[order[k]][i][j], the real parts of
+// the left singular vectors in pyLOM's column-major flattening with the
+// frequencies in sorted order (pyLOM/SPOD/wrapper.py:84, 90-94).  U holds nf
+// complex64 M x nB matrices.  One workgroup per tile of I points x KC
+// frequencies, staged through LDS: each U row segment is read contiguously
+// (I nB values per frequency) and each P row segment written contiguously.
+template <class TO>
+__global__ __launch_bounds__(256) void k_spod_modes(const float2* __restrict__ U, int64_t M, int nB, int nf,
+                                                    const int64_t* __restrict__ order, int I, int KC,
+                                                    TO* __restrict__ P) {
+  extern __shared__ float sre[];  // [KC][I nB]
+  const int64_t i0 = int64_t(blockIdx.x) * I;
+  const int k0 = int(blockIdx.y) * KC;
+  const int ni = int(M - i0 < I ? M - i0 : I);
+  const int nk = nf - k0 < KC ? nf - k0 : KC;
+  const int run = ni * nB, ld = I * nB;
+  for (int idx = threadIdx.x; idx < nk * run; idx += 256) {
+    const int kl = idx / run, e = idx - kl * run;
+    int64_t ok = order[k0 + kl];
+    ok = ok < 0 ? 0 : (ok >= nf ? nf - 1 : ok);  // order is a permutation; the clamp only guards
+    sre[kl * ld + e] = U[(ok * M + i0) * nB + e].x;
+  }
+  __syncthreads();
+  for (int idx = threadIdx.x; idx < run * nk; idx += 256) {
+    const int r = idx / nk, kl = idx - r * nk;
+    const int j = r / ni, il = r - j * ni;
+    P[(int64_t(j) * M + i0 + il) * nf + k0 + kl] = TO(sre[kl * ld + il * nB + j]);
+  }
+}
+
 }  // namespace
+
+extern "C" int pc_spod_modes(pc_ctx* ctx, const void* U, int64_t M, int64_t nB, int64_t nf, const int64_t* order,
+                             void* P, int out_f64) {
+  if (!ctx) return pc_fail(nullptr, PC_ERR_INVALID, "ctx is NULL");
+  if (!U || !order || !P) return pc_fail(ctx, PC_ERR_INVALID, "pc_spod_modes: NULL pointer");
+  if (M <= 0 || nB <= 0 || nf <= 0) return pc_fail(ctx, PC_ERR_SHAPE, "pc_spod_modes: empty shape");
+  constexpr int64_t kLds = 12288;  // staged floats per workgroup (48 KB)
+  if (nB > kLds || nf > (int64_t(1) << 30))
+    return pc_fail(ctx, PC_ERR_SHAPE, "pc_spod_modes: nBlks = %lld is larger than supported", (long long)nB);
+  int64_t KC = std::min<int64_t>(nf, 64);
+  if (KC * nB > kLds) KC = std::max<int64_t>(1, kLds / nB);
+  const int64_t I = std::max<int64_t>(1, kLds / (KC * nB));
+  const dim3 g(unsigned(ceil_div(M, I)), unsigned(ceil_div(nf, KC)));
+  const size_t lds = size_t(KC * I * nB) * sizeof(float);
+  hipStream_t st = ctx->stream;
+  if (out_f64)
+    hipLaunchKernelGGL((k_spod_modes<double>), g, dim3(256), lds, st, (const float2*)U, M, int(nB), int(nf), order,
+                       int(I), int(KC), (double*)P);
+  else
+    hipLaunchKernelGGL((k_spod_modes<float>), g, dim3(256), lds, st, (const float2*)U, M, int(nB), int(nf), order,
+                       int(I), int(KC), (float*)P);
+  PC_HIP(ctx, hipGetLastError());
+  return PC_OK;
+}
 
 extern "C" int pc_spod_blocks(pc_ctx* ctx, const float* X, int64_t ldx, int64_t M, int64_t N, int64_t nDFT,
                               int64_t nolap, int remove_mean, void* Q) {

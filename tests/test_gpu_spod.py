@@ -79,6 +79,23 @@ def test_uniform_complex_group_bitwise_equals_single(dtype, precision, rng):
     assert torch.allclose(U3.conj().T @ U3, torch.eye(n, dtype=torch.complex128, device="cuda"), atol=1e-5)
 
 
+@pytest.mark.parametrize("M,nB,nf", [(1000, 7, 130), (37, 15, 129), (50, 300, 9), (5, 1, 3)])
+@pytest.mark.parametrize("out", [torch.float32, torch.float64])
+def test_spod_modes_layout_bitwise(M, nB, nf, out, rng):
+    """pc_spod_modes equals the plain gather P[j M + i, k] = Re U[order[k]][i, j]
+    (pyLOM/SPOD/wrapper.py:84, 90-94), bit for bit."""
+    U = torch.from_numpy((rng.standard_normal((nf, M, nB)) + 1j * rng.standard_normal((nf, M, nB)))
+                         .astype(np.complex64)).cuda()
+    order = torch.from_numpy(rng.permutation(nf).astype(np.int64)).cuda()
+    ref = U.real.permute(2, 1, 0).index_select(2, order).reshape(nB * M, nf).to(out)
+    P = torch.empty((nB * M, nf), dtype=out, device="cuda")
+    ctx = pl._lib.context()
+    ctx.check(ctx.lib.pc_spod_modes(ctx.h, U.data_ptr(), M, nB, nf, order.data_ptr(), P.data_ptr(),
+                                    1 if out == torch.float64 else 0))
+    torch.cuda.synchronize()
+    assert torch.equal(P, ref)
+
+
 def test_grouped_rejects_complex_centring(rng):
     A = torch.zeros((3, 100, 4), dtype=torch.complex64, device="cuda")
     with pytest.raises(pl.PodCoreError, match="problem 0: .*real data"):
