@@ -14,6 +14,8 @@
 //       W = W1 P R2^-1 U_M               (U_M = H sigma^-1)
 //   P3  U = Y W                          (MFMA streaming pass)
 // plus an orthonormal completion of U for numerically null singular values.
+#include <algorithm>
+#include <atomic>
 #include <thread>
 #include "common.hpp"
 #include "kernels.hpp"
