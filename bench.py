@@ -53,7 +53,7 @@ def parse():
     ap.add_argument("--rate", type=float, default=200.0, help="serve: Poisson arrival rate, requests/s")
     ap.add_argument("--duration", type=float, default=20.0, help="serve: arrival window, s")
     ap.add_argument("--slots", type=int, default=4, help="serve: requests in flight")
-    ap.add_argument("--workload", default="pod", choices=["pod", "grouped", "serve", "copy"],
+    ap.add_argument("--workload", default="pod", choices=["pod", "grouped", "spod", "serve", "copy"],
                     help="pod: tall-skinny SVD 8M x 256 (default); grouped: SPOD 200000 x 2048 "
                          "(129 complex64 200000 x 15 blocks) + 64 uneven real POD problems; serve: "
                          "Poisson stream of POD / project / reconstruct requests; copy: one plain device "
@@ -469,6 +469,30 @@ def run_serve(a):
     print(json.dumps(out), flush=True)
 
 
+def run_spod(a):
+    """SPOD.run alone (nDFT 256, 50% overlap) on 200000 x 2048 float32: the
+    Welch blocks, one batched grouped solve over the 129 frequencies and the
+    sorted modes.  For per-kernel traces of that path."""
+    import torch
+    import pyloworder_amd as pl
+    M, N, nDFT, nolap = 200_000, 2048, 256, 128
+    X, t = waves_device(M, N)
+    res = {}
+    for prec in ("mixed", "fp64"):
+        for _ in range(a.warmup):
+            pl.SPOD.run(X, t, nDFT, nolap, precision=prec)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            L, P, f = pl.SPOD.run(X, t, nDFT, nolap, precision=prec)
+            del L, P, f
+        torch.cuda.synchronize()
+        res[prec] = round((time.perf_counter() - t0) / a.steps * 1e3, 3)
+        log(f"[bench] spod {prec}: {res[prec]} ms per SPOD.run")
+    print(json.dumps({"workload": "spod", "ms_per_run": res, "steps": a.steps, "warmup": a.warmup,
+                      "config": {"M": M, "N": N, "nDFT": nDFT, "nolap": nolap}}), flush=True)
+
+
 def run_copy(a):
     """One plain copy of an m x n float32 matrix: known bytes (m n 4 read, m n 4
     written) to calibrate the FETCH_SIZE / WRITE_SIZE counters."""
@@ -495,6 +519,8 @@ def main():
         return run_grouped(a)
     if a.workload == "serve":
         return run_serve(a)
+    if a.workload == "spod":
+        return run_spod(a)
     if a.workload == "copy":
         return run_copy(a)
     import torch

@@ -222,7 +222,8 @@ __global__ __launch_bounds__(256) void k_spod_fft(const float* __restrict__ X, i
 // frequencies in sorted order (pyLOM/SPOD/wrapper.py:84, 90-94).  U holds nf
 // complex64 M x nB matrices.  One workgroup per tile of I points x KC
 // frequencies, staged through LDS: each U row segment is read contiguously
-// (I nB values per frequency) and each P row segment written contiguously.
+// (I nB values per frequency) and each P row segment written contiguously
+// (whole rows when all frequencies fit one tile).
 template <class TO>
 __global__ __launch_bounds__(256) void k_spod_modes(const float2* __restrict__ U, int64_t M, int nB, int nf,
                                                     const int64_t* __restrict__ order, int I, int KC,
@@ -233,17 +234,37 @@ __global__ __launch_bounds__(256) void k_spod_modes(const float2* __restrict__ U
   const int ni = int(M - i0 < I ? M - i0 : I);
   const int nk = nf - k0 < KC ? nf - k0 : KC;
   const int run = ni * nB, ld = I * nB;
-  for (int idx = threadIdx.x; idx < nk * run; idx += 256) {
-    const int kl = idx / run, e = idx - kl * run;
-    int64_t ok = order[k0 + kl];
-    ok = ok < 0 ? 0 : (ok >= nf ? nf - 1 : ok);  // order is a permutation; the clamp only guards
-    sre[kl * ld + e] = U[(ok * M + i0) * nB + e].x;
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  // loads: wave w takes frequencies w, w + 4, ..., eight at a time so that
+  // eight independent loads are in flight per lane; one contiguous segment
+  // per frequency
+  constexpr int UF = 8;
+  for (int kb = w; kb < nk; kb += 4 * UF) {
+    const float2* src[UF];
+#pragma unroll
+    for (int u = 0; u < UF; u++) {
+      const int kl = kb + 4 * u;
+      int64_t ok = kl < nk ? order[k0 + kl] : 0;
+      ok = ok < 0 ? 0 : (ok >= nf ? nf - 1 : ok);  // order is a permutation; the clamp only guards
+      src[u] = U + (ok * M + i0) * nB;
+    }
+    for (int e = l; e < run; e += 64) {
+      float v[UF];
+#pragma unroll
+      for (int u = 0; u < UF; u++) v[u] = kb + 4 * u < nk ? src[u][e].x : 0.0f;
+#pragma unroll
+      for (int u = 0; u < UF; u++)
+        if (kb + 4 * u < nk) sre[(kb + 4 * u) * ld + e] = v[u];
+    }
   }
   __syncthreads();
-  for (int idx = threadIdx.x; idx < run * nk; idx += 256) {
-    const int r = idx / nk, kl = idx - r * nk;
+  // stores: wave w takes output rows (j, il) = w, w + 4, ...; lanes cover the
+  // row's frequencies
+  for (int r = w; r < run; r += 4) {
     const int j = r / ni, il = r - j * ni;
-    P[(int64_t(j) * M + i0 + il) * nf + k0 + kl] = TO(sre[kl * ld + il * nB + j]);
+    TO* dst = P + (int64_t(j) * M + i0 + il) * nf + k0;
+    const float* col = sre + il * nB + j;
+    for (int kl = l; kl < nk; kl += 64) dst[kl] = TO(col[kl * ld]);
   }
 }
 
@@ -257,8 +278,14 @@ extern "C" int pc_spod_modes(pc_ctx* ctx, const void* U, int64_t M, int64_t nB, 
   constexpr int64_t kLds = 12288;  // staged floats per workgroup (48 KB)
   if (nB > kLds || nf > (int64_t(1) << 30))
     return pc_fail(ctx, PC_ERR_SHAPE, "pc_spod_modes: nBlks = %lld is larger than supported", (long long)nB);
-  int64_t KC = std::min<int64_t>(nf, 64);
-  if (KC * nB > kLds) KC = std::max<int64_t>(1, kLds / nB);
+  // all frequencies in one chunk when they fit (whole P rows written
+  // contiguously), else balanced chunks of at most 64
+  int64_t KC = nf;
+  if (KC * nB > kLds) {
+    const int64_t nch = ceil_div(nf, 64);
+    KC = ceil_div(nf, nch);
+    if (KC * nB > kLds) KC = std::max<int64_t>(1, kLds / nB);
+  }
   const int64_t I = std::max<int64_t>(1, kLds / (KC * nB));
   const dim3 g(unsigned(ceil_div(M, I)), unsigned(ceil_div(nf, KC)));
   const size_t lds = size_t(KC * I * nB) * sizeof(float);

@@ -79,7 +79,7 @@ def test_uniform_complex_group_bitwise_equals_single(dtype, precision, rng):
     assert torch.allclose(U3.conj().T @ U3, torch.eye(n, dtype=torch.complex128, device="cuda"), atol=1e-5)
 
 
-@pytest.mark.parametrize("M,nB,nf", [(1000, 7, 130), (37, 15, 129), (50, 300, 9), (5, 1, 3)])
+@pytest.mark.parametrize("M,nB,nf", [(1000, 7, 130), (37, 15, 129), (50, 300, 9), (5, 1, 3), (40, 100, 200)])
 @pytest.mark.parametrize("out", [torch.float32, torch.float64])
 def test_spod_modes_layout_bitwise(M, nB, nf, out, rng):
     """pc_spod_modes equals the plain gather P[j M + i, k] = Re U[order[k]][i, j]
