@@ -15,7 +15,8 @@
 //                    G2 = Z^T Z (three products).
 //   P3 k_gemm_mx2  : C = Ys W (hi.hi + hi.lo + lo.hi), per-row scales of Ys,
 //                    per-column scales of W.
-// In P2/P3 every load instruction reads whole rows (float4 per lane); the fp16
+// In P2/P3 every load instruction reads whole rows (four columns per lane: one
+// float4, two float2 or four floats, by the alignment of the rows); the fp16
 // pieces go to LDS in A-fragment order, and each wave keeps the B fragments of
 // its own 32 output columns (W) in registers for the whole kernel.
 //
@@ -164,7 +165,7 @@ PC_DEV void write_upper(const f16v* acc, const int* sI, const int* sJ, const boo
 // ---------------------------------------------------------------------------
 // P1
 // ---------------------------------------------------------------------------
-template <int NB, bool VEC, int CENTER>
+template <int NB, int VEC, int CENTER>
 __global__ __launch_bounds__(NT2, 1) void k_gram_mx2(const float* __restrict__ X, int64_t ldx, int64_t m, int n,
                                                      double* __restrict__ mean_io,
                                                      const double* __restrict__ var, int64_t tpc,
@@ -208,7 +209,7 @@ __global__ __launch_bounds__(NT2, 1) void k_gram_mx2(const float* __restrict__ X
 #pragma unroll
     for (int t = 0; t < 8; t++) {
       const int rt = 16 * wk + 8 * hl + t;
-      if constexpr (VEC && V == 4) {
+      if constexpr (VEC == 1 && V == 4) {
         const uint32_t off = uint32_t((rt * ldx + (col < n ? col : 0)) * 4);
         f4 q = __builtin_bit_cast(f4, __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0));
 #pragma unroll
@@ -279,8 +280,8 @@ __global__ __launch_bounds__(NT2, 1) void k_gram_mx2(const float* __restrict__ X
 }
 
 // ---------------------------------------------------------------------------
-// P2 / P3 row tiles: 32 rows (64 for n <= 32), each wave loads whole rows with
-// one float4 per lane
+// P2 / P3 row tiles: 32 rows (64 for n <= 32), each wave loads whole rows, four
+// columns per lane
 // ---------------------------------------------------------------------------
 template <int NB>
 struct RowTile {
@@ -297,7 +298,7 @@ struct RowTile {
 
 // rows of one tile through a per-tile buffer resource: rows >= m read 0; the
 // columns >= n are masked in rows_split
-template <int NB, bool VEC>
+template <int NB, int VEC>
 PC_DEV void rows_load(const float* __restrict__ X, int64_t ldx, int64_t m, int n, int64_t tile, f4* v) {
   using RT = RowTile<NB>;
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
@@ -309,10 +310,24 @@ PC_DEV void rows_load(const float* __restrict__ X, int64_t ldx, int64_t m, int n
   for (int i = 0; i < RT::NLD; i++) {
     const int rt = w * RT::RPW + i * RT::RPI + l / RT::LPR;
     const uint32_t off = uint32_t((rt * ldx + (col < n ? col : 0)) * 4);
-    if constexpr (VEC) {
+    if constexpr (VEC == 1) {
       typedef int i4 __attribute__((ext_vector_type(4)));
       i4 q = __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0);
       v[i] = __builtin_bit_cast(f4, q);
+    } else if constexpr (VEC == 2) {
+      // two 8-byte global loads (columns col, col + 1 and col + 2, col + 3),
+      // predicated on the row: n is even, so a pair is wholly inside or wholly
+      // outside the valid columns, and a pair outside reads column 0 (masked
+      // in rows_split); rows >= m read 0 as through the buffer resource
+      const float* rp = X + (row0 + rt) * ldx;
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        const int c0 = col + 2 * h;
+        float2 q = make_float2(0.0f, 0.0f);
+        if (rt < nrow) q = *reinterpret_cast<const float2*>(rp + (c0 < n ? c0 : 0));
+        v[i][2 * h] = q.x;
+        v[i][2 * h + 1] = q.y;
+      }
     } else {
 #pragma unroll
       for (int u = 0; u < 4; u++)
@@ -369,7 +384,7 @@ PC_DEV h8 frag(const char* Fs, int idx, int fragb, int l) { return *reinterpret_
 // ---------------------------------------------------------------------------
 // P3: C = Ys W
 // ---------------------------------------------------------------------------
-template <int NB, bool VEC, int CENTER>
+template <int NB, int VEC, int CENTER>
 __global__ __launch_bounds__(NT2, 1) void k_gemm_mx2(const float* __restrict__ X, int64_t ldx, int64_t m, int n,
                                                      const double* __restrict__ mean,
                                                      const double* __restrict__ var, const h8* __restrict__ Wh,
@@ -461,7 +476,7 @@ __global__ __launch_bounds__(NT2, 1) void k_gemm_mx2(const float* __restrict__ X
 // ---------------------------------------------------------------------------
 // P2: Z = Ys W1 (fp16 W1, per-column scale), G2 = Z^T Z
 // ---------------------------------------------------------------------------
-template <int NB, bool VEC, int CENTER>
+template <int NB, int VEC, int CENTER>
 __global__ __launch_bounds__(NT2, 1) void k_zgram_mx2(const float* __restrict__ X, int64_t ldx, int64_t m, int n,
                                                       const double* __restrict__ mean,
                                                       const double* __restrict__ var, const h8* __restrict__ Wf,
@@ -581,22 +596,30 @@ int dispatch_nb(int n, F&& f) {
   if (n <= 256) return f(std::integral_constant<int, 256>{});
   return -1;
 }
-// (VEC, CENTER) as compile-time constants
+// (load mode, CENTER) as compile-time constants; mode 0 scalar, 1 float4,
+// 2 float2 loads
 template <class F>
-int dispatch_vc(bool vec, int center, F&& f) {
+int dispatch_vc(int vec, int center, F&& f) {
   if (center < 0 || center > 2) return -1;
   auto c = [&](auto vv) {
     if (center == 0) return f(vv, std::integral_constant<int, 0>{});
     if (center == 1) return f(vv, std::integral_constant<int, 1>{});
     return f(vv, std::integral_constant<int, 2>{});
   };
-  return vec ? c(std::true_type{}) : c(std::false_type{});
+  if (vec == 1) return c(std::integral_constant<int, 1>{});
+  if (vec == 2) return c(std::integral_constant<int, 2>{});
+  return c(std::integral_constant<int, 0>{});
 }
 
 // float4 path: 16-byte aligned rows and n a multiple of 4 (a row's float4 is
-// then either wholly inside or wholly outside the n valid columns)
-bool vec_ok(const float* X, int64_t ldx, int n) {
-  return (reinterpret_cast<uintptr_t>(X) & 15) == 0 && (ldx & 3) == 0 && (n & 3) == 0;
+// then either wholly inside or wholly outside the n valid columns); float2
+// path: 8-byte aligned rows and n even.  zx is the byte stride between the
+// problems of a batched launch.
+int load_mode(const float* X, int64_t ldx, int n, int64_t zx) {
+  const uintptr_t a = reinterpret_cast<uintptr_t>(X);
+  if ((a & 15) == 0 && (ldx & 3) == 0 && (n & 3) == 0 && zx % 16 == 0) return 1;
+  if ((a & 7) == 0 && (ldx & 1) == 0 && (n & 1) == 0 && zx % 8 == 0) return 2;
+  return 0;
 }
 
 }  // namespace
@@ -616,7 +639,8 @@ int launch_gram_mx2(hipStream_t st, const float* X, int64_t ldx, int64_t m, int 
                     const double* var, int chunks, int64_t tpc, double* part, const PassBatch& bz) {
   return dispatch_nb(n, [&](auto nbc) {
     constexpr int NB = decltype(nbc)::value;
-    return dispatch_vc(vec_ok(X, ldx, n) && bz.zx % 16 == 0, center, [&](auto vv, auto cc) {
+    const int mode = load_mode(X, ldx, n, bz.zx);
+    return dispatch_vc(mode == 1 ? 1 : 0, center, [&](auto vv, auto cc) {
       hipLaunchKernelGGL((k_gram_mx2<NB, decltype(vv)::value, decltype(cc)::value>),
                          dim3(unsigned(chunks), 1, unsigned(bz.nz)), dim3(NT2), 0, st, X, ldx, m, n, mean_io, var, tpc,
                          part, bz.zx, bz.zb);
@@ -630,7 +654,7 @@ int launch_zgram_mx2(hipStream_t st, const float* X, int64_t ldx, int64_t m, int
                      double* part, const PassBatch& bz) {
   return dispatch_nb(n, [&](auto nbc) {
     constexpr int NB = decltype(nbc)::value;
-    return dispatch_vc(vec_ok(X, ldx, n) && bz.zx % 16 == 0, center, [&](auto vv, auto cc) {
+    return dispatch_vc(load_mode(X, ldx, n, bz.zx), center, [&](auto vv, auto cc) {
       hipLaunchKernelGGL((k_zgram_mx2<NB, decltype(vv)::value, decltype(cc)::value>),
                          dim3(unsigned(chunks), 1, unsigned(bz.nz)), dim3(NT2), 0, st, X, ldx, m, n, mean, var,
                          (const h8*)Wf, wscale, tpc, part, bz.zx, bz.zb);
@@ -646,7 +670,7 @@ int launch_gemm_mx2(hipStream_t st, const float* X, int64_t ldx, int64_t m, int 
   if (mx2_nb(nc) < 0 || mx2_nb(nc) > mx2_nb(n)) return -1;
   return dispatch_nb(n, [&](auto nbc) {
     constexpr int NB = decltype(nbc)::value;
-    return dispatch_vc(vec_ok(X, ldx, n) && bz.zx % 16 == 0, center, [&](auto vv, auto cc) {
+    return dispatch_vc(load_mode(X, ldx, n, bz.zx), center, [&](auto vv, auto cc) {
       hipLaunchKernelGGL((k_gemm_mx2<NB, decltype(vv)::value, decltype(cc)::value>),
                          dim3(unsigned(chunks), 1, unsigned(bz.nz)), dim3(NT2), 0, st, X, ldx, m, n, mean, var,
                          (const h8*)Wh, (const h8*)Wl, wscale, nc, tpc, C, ldc, bz.zx, bz.zb, bz.zc);
