@@ -10,6 +10,8 @@ singular subspace (sigma >= 1e-3 sigma_1(f)) within 1e-3 rad of the oracle's.
 P is Re(U): the complex phase of each mode is arbitrary (LAPACK's choice in the
 reference), so P is compared after aligning each mode's phase to the oracle's.
 """
+import ctypes as C
+
 import numpy as np
 import pytest
 import torch
@@ -94,6 +96,35 @@ def test_spod_modes_layout_bitwise(M, nB, nf, out, rng):
                                     1 if out == torch.float64 else 0))
     torch.cuda.synchronize()
     assert torch.equal(P, ref)
+
+
+def test_uniform_group_over_budget_falls_back(rng):
+    """A uniform group whose per-problem workspace slices exceed the context's
+    HBM budget runs problem by problem instead, with the same results."""
+    from pyloworder_amd import _lib
+    k, m, n = 7, 3000, 15
+    A = torch.from_numpy((rng.standard_normal((k, m, n)) + 1j * rng.standard_normal((k, m, n)))
+                         .astype(np.complex64)).cuda()
+    U = torch.empty_like(A)
+    S = torch.empty((k, n), dtype=torch.float32, device="cuda")
+    V = torch.empty((k, n, n), dtype=torch.complex64, device="cuda")
+    lib = _lib.load()
+    need = lib.pc_workspace_query(0, _lib.PC_C64, m, n, 0, _lib.PC_PREC_MIXED)
+    assert need > 0
+    ctx = _lib.Context(torch.cuda.current_device(), 3 * need)  # room for 3 slices, not 7
+    probs = (_lib.PCProblem * k)()
+    for i in range(k):
+        probs[i] = _lib.PCProblem(A[i].data_ptr(), n, m, n, 0, 0, U[i].data_ptr(), n, S[i].data_ptr(),
+                                  V[i].data_ptr(), n)
+    torch.cuda.synchronize()
+    try:
+        ctx.check(lib.pc_grouped_tsqr_svd(ctx.h, _lib.PC_C64, C.cast(probs, C.c_void_p), k, _lib.PC_PREC_MIXED))
+        torch.cuda.synchronize()
+    finally:
+        ctx.close()
+    for i in range(k):
+        U1, S1, V1 = pl.math.tsqr_svd(A[i], precision="mixed")
+        assert torch.equal(S[i], S1) and torch.equal(U[i], U1) and torch.equal(V[i], V1), i
 
 
 def test_grouped_rejects_complex_centring(rng):
