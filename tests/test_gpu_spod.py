@@ -149,7 +149,7 @@ def test_grouped_errors_name_the_problem(rng):
 
 @pytest.mark.parametrize("M,N,nDFT,nolap", [(700, 400, 0, 0), (300, 3000, 1024, 512), (257, 500, 99, 30),
                                             (64, 200, 2, 1), (33, 4200, 2048, 1024), (21, 4100, 4096, 0),
-                                            (130, 700, 256, 100)])
+                                            (130, 700, 256, 100), (50, 2048, 256, 128), (41, 5000, 256, 0)])
 def test_spod_blocks_vs_oracle(M, N, nDFT, nolap):
     X, t = _flow(M, N, seed=M + N)
     Qo, fo, nbo = oracle.spod_blocks(X, t, nDFT, nolap)
