@@ -405,7 +405,6 @@ __global__ __launch_bounds__(1024) void k_null_build(int64_t m, int n, const int
   if (d == 0) return;
   int g = n - d;
   __shared__ int pick[512];
-  __shared__ double sh[32];
   // rank candidates by leverage (ties by index), keep the first d
   for (int c = threadIdx.x; c < ncand; c += 1024) {
     double v = key_asc(lev[c]);
