@@ -163,10 +163,15 @@ int pc_truncation_residual(const double* S, int64_t n, double r, int64_t* N);
 
 /* ---- batched (grouped) solves --------------------------------------------
  * The per-frequency loop of pyLOM/SPOD/wrapper.py:81-86 (tsqr_svd per block)
- * and any list of independent POD problems in one call: every problem is
- * validated first, the problems run spread over internal streams (forked
- * from and joined back to the context stream) with one workspace slice each,
- * and the statuses are read once at the end; an error names the problem.
+ * and any list of independent POD problems in one call.  Every problem is
+ * validated first; the statuses are read once at the end, and an error names
+ * the problem.  A uniform complex group (one shape, leading dimensions and
+ * centring; A, U, S and VT at constant strides; n <= 32) runs every kernel as
+ * one launch over all problems on the context stream, one workspace slice per
+ * problem (falling back to the path below when the slices exceed the HBM
+ * budget).  Other groups are taken from a queue, most expensive first, by
+ * internal streams forked from and joined back to the context stream: one per
+ * HIP hardware queue (GPU_MAX_HW_QUEUES, 4 when unset, at most 16 used).
  * Results are identical, bit for bit, to pc_tsqr_svd on each problem alone. */
 typedef struct {
   const void* A; int64_t lda; int64_t m; int64_t n; int32_t center; int32_t pad_;
