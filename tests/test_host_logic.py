@@ -1,0 +1,77 @@
+"""Host-side logic that needs no GPU: the output layout of grouped solves
+(stacked tensors, so that a uniform group runs batched), the SPOD block
+geometry (pyLOM/SPOD/wrapper.py:45-52) and the bench's choice of roofline."""
+import os
+import sys
+from pathlib import Path
+from types import SimpleNamespace
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from pyloworder_amd import SPOD
+from pyloworder_amd.vmmath import _group_outputs
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def _args(shapes, dtype=torch.complex64):
+    return [SimpleNamespace(t=torch.zeros(s, dtype=dtype)) for s in shapes]
+
+
+def test_group_outputs_stacked_for_equal_shapes():
+    k, m, n = 5, 40, 7
+    outs = _group_outputs(_args([(m, n)] * k), torch.complex64)
+    U0, S0, V0 = outs[0]
+    for i, (U, S, V) in enumerate(outs):
+        assert U.shape == (m, n) and S.shape == (n,) and V.shape == (n, n)
+        assert S.dtype == torch.float32 and U.dtype == torch.complex64
+        # one storage per output kind, constant strides between the problems
+        assert U.untyped_storage().data_ptr() == U0.untyped_storage().data_ptr()
+        assert U.data_ptr() - U0.data_ptr() == i * m * n * U.element_size()
+        assert S.data_ptr() - S0.data_ptr() == i * n * S.element_size()
+        assert V.data_ptr() - V0.data_ptr() == i * n * n * V.element_size()
+        assert U.stride() == (n, 1) and V.stride() == (n, 1)
+
+
+def test_group_outputs_separate_for_uneven_shapes():
+    outs = _group_outputs(_args([(40, 7), (30, 7), (40, 5)], torch.float32), torch.float32)
+    ptrs = {o[0].untyped_storage().data_ptr() for o in outs}
+    assert len(ptrs) == 3
+    assert [tuple(o[0].shape) for o in outs] == [(40, 7), (30, 7), (40, 5)]
+    assert all(o[1].dtype == torch.float32 for o in outs)
+
+
+@pytest.mark.parametrize("N,nDFT,nolap", [(2048, 256, 128), (640, 0, 0), (4200, 2048, 1024), (500, 99, 30),
+                                          (5000, 256, 0), (200, 2, 1)])
+def test_spod_geometry_matches_oracle(N, nDFT, nolap):
+    X = np.zeros((3, N))
+    t = np.arange(N) * 0.01
+    _, f, nb = oracle.spod_blocks(X, t, nDFT, nolap)
+    d, o, b = SPOD._sizes(N, nDFT, nolap)
+    assert b == nb and f.shape[0] == int(np.ceil(d / 2)) + 1
+
+
+def test_bench_roofline_takes_the_binding_bound():
+    # bench sets GPU_MAX_HW_QUEUES on import; keep this process's value
+    before = os.environ.get("GPU_MAX_HW_QUEUES")
+    sys.path.insert(0, str(ROOT))
+    try:
+        import bench
+    finally:
+        if before is None:
+            os.environ.pop("GPU_MAX_HW_QUEUES", None)
+        else:
+            os.environ["GPU_MAX_HW_QUEUES"] = before
+    m, n = 8_000_000, 256
+    # P2 (k_zgram_mx2): 3.87 TFLOP on 8.19 GB, above the 312 flop/B ridge: MFMA-bound
+    name, r = bench.roofline_for({"gram": (3.4, 0), "zgram": (5.6, 0), "gemm": (5.1, 0)}, m, n, 4)
+    assert name == "zgram" and r["bound"] == "mfma" and r["unit"] == "TFLOP/s"
+    assert r["algorithmic_flops"] == (4 * m * n * n + 3 * 36 * 2048 * m)
+    assert abs(r["frac"] - r["achieved"] / r["peak"]) < 1e-3
+    # P3 (k_gemm_mx2): 3.15 TFLOP on 16.4 GB: HBM-bound
+    name, r = bench.roofline_for({"gram": (3.4, 0), "zgram": (5.0, 0), "gemm": (5.1, 0)}, m, n, 4)
+    assert name == "gemm" and r["bound"] == "hbm" and r["unit"] == "GB/s"
+    assert r["algorithmic_bytes"] == 2 * m * n * 4
