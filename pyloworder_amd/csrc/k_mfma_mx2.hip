@@ -402,9 +402,14 @@ __global__ __launch_bounds__(NT2, 1) void k_gemm_mx2(const float* __restrict__ X
   __shared__ __align__(16) char Fs[RT::FBYTES];
   __shared__ int rexp[RT::TR];
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63, hl = l >> 5, c = l & 31;
-  const int J = w;
-  // every wave owns a column block when NB = 256: keep that test out of the code
-  const bool own = NCBLK >= NW2 || J < NCBLK;
+  // wave w computes column block J = w mod NCBLK for the 32-row blocks
+  // R = w / NCBLK, + NW2 / NCBLK, ... of each tile: with fewer column blocks
+  // than waves, the waves sharing a column block split the tile's rows.  Every
+  // wave owns a column block when NB = 256: keep that test out of the code.
+  constexpr int RSTEP = NCBLK >= NW2 ? 1 : NW2 / NCBLK;
+  const int J = NCBLK >= NW2 ? w : w % NCBLK;
+  const int R0 = NCBLK >= NW2 ? 0 : w / NCBLK;
+  const bool own = NCBLK >= NW2 || R0 < RT::NR;
   h8 bh[KSW], bl[KSW];
   float isc = 1.0f;
   if (own) {
@@ -443,7 +448,7 @@ __global__ __launch_bounds__(NT2, 1) void k_gemm_mx2(const float* __restrict__ X
       const int64_t nrow = m - row0 < RT::TR ? m - row0 : RT::TR;
       const rsrc_t cs = make_rsrc(C + row0 * ldc, uint32_t(((nrow - 1) * ldc + nc) * 4));
 #pragma unroll
-      for (int R = 0; R < RT::NR; R++) {
+      for (int R = R0; R < RT::NR; R += RSTEP) {
         f16v acc = {};
 #pragma unroll
         for (int ks = 0; ks < KSW; ks++) {
