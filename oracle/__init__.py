@@ -28,7 +28,8 @@ Pinning (how each function is checked, see tests/test_oracle.py):
     plus its last sketch; the update reproduces a low-rank history exactly
     (Q2 B2 = [A1, A2] when rank(A1) + rank(A2) <= r, no power steps); a test
     shows the sketch depends on the QR sign convention (``positive`` selects
-    diag(R) >= 0, the MI355X convention).
+    diag(R) >= 0 instead of LAPACK's signs, which the reference and the
+    device path both use).
   * spod_run: a single-frequency travelling wave whose spectrum peaks at the
     known bin, Parseval-type energy identities, and a hand-built 2-block case.
 """

@@ -187,9 +187,10 @@ def randomized_qr(blocks, r: int, q: int, seed: int = -1, omega=None):
 
 
 def _qr_signs(blocks, positive):
-    """tsqr of the blocks; with ``positive`` the column signs are chosen so
-    that diag(R) >= 0 (the convention of the MI355X tsqr).  The reference
-    keeps LAPACK's signs (pyLOM/vmmath/svd.py:49-118 via np.linalg.qr)."""
+    """tsqr of the blocks, LAPACK's signs as the reference
+    (pyLOM/vmmath/svd.py:49-118 via np.linalg.qr); with ``positive`` the
+    column signs are flipped so that diag(R) >= 0, a second convention that
+    the oracle tests use to show the sketch depends on the choice."""
     Qi, R = tsqr(blocks)
     if positive:
         d = np.where(np.diag(R) < 0, -1.0, 1.0)

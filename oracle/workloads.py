@@ -1,8 +1,8 @@
 """Seeded synthetic inputs of the five POD workloads, numpy form (test infra).
 
-The GPU-side generators used by bench.py live in pyloworder_amd.workloads and
-draw from torch; these numpy twins build the small and scaled-down cases that
-tests compare against the oracle.
+bench.py generates the full-size inputs on the device with torch (its
+generate / waves_device / uneven_problems / pool_device); these numpy twins
+build the small and scaled-down cases that tests compare against the oracle.
 """
 from __future__ import annotations
 

@@ -84,6 +84,11 @@ extern "C" int pc_synchronize(pc_ctx* ctx) {
 
 int pc_ws_reserve(pc_ctx* ctx, int64_t bytes) {
   if (bytes <= ctx->ws_bytes) return PC_OK;
+  // an entry point holding the front of the buffer (ws_skip > 0) reserves its
+  // nested needs up front: growing here would free memory still in use
+  if (ctx->ws_skip > 0 && ctx->ws)
+    return pc_fail(ctx, PC_ERR_INVALID, "internal: workspace must grow (%lld -> %lld bytes) while its front is held",
+                   (long long)ctx->ws_bytes, (long long)bytes);
   if (ctx->budget > 0 && bytes > ctx->budget)
     return pc_fail(ctx, PC_ERR_NOMEM, "workspace of %lld bytes exceeds the HBM budget of %lld bytes",
                    (long long)bytes, (long long)ctx->budget);

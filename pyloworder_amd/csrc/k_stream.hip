@@ -79,17 +79,23 @@ __global__ __launch_bounds__(256) void k_vecmat_c(const T* __restrict__ v, const
 template <class T>
 __global__ __launch_bounds__(256) void k_transpose(const T* __restrict__ A, int64_t lda, int64_t m, int64_t n,
                                                    T* __restrict__ B, int64_t ldb) {
+  // 32 x 32 tiles, flattened (row tiles fastest) and taken grid-stride, so
+  // neither dimension is limited by the grid
   __shared__ T tile[32][33];
-  int64_t bi = int64_t(blockIdx.y) * 32, bj = int64_t(blockIdx.x) * 32;
-  int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
-  for (int r = ty; r < 32; r += 8) {
-    int64_t i = bi + r, j = bj + tx;
-    if (i < m && j < n) tile[r][tx] = A[i * lda + j];
-  }
-  __syncthreads();
-  for (int r = ty; r < 32; r += 8) {
-    int64_t j = bj + r, i = bi + tx;
-    if (i < m && j < n) B[j * ldb + i] = tile[tx][r];
+  const int64_t tm = ceil_div(m, 32), tiles = tm * ceil_div(n, 32);
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (int64_t tt = blockIdx.x; tt < tiles; tt += gridDim.x) {
+    const int64_t bi = (tt % tm) * 32, bj = (tt / tm) * 32;
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {
+      int64_t i = bi + r, j = bj + tx;
+      if (i < m && j < n) tile[r][tx] = A[i * lda + j];
+    }
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {
+      int64_t j = bj + r, i = bi + tx;
+      if (i < m && j < n) B[j * ldb + i] = tile[tx][r];
+    }
   }
 }
 
@@ -245,8 +251,7 @@ extern "C" int pc_transpose(pc_ctx* ctx, pc_dtype dt, const void* A, int64_t lda
   PC_CHECK(ctx, check_mat(ctx, A, lda, m, n, "pc_transpose A"));
   PC_CHECK(ctx, check_mat(ctx, B, ldb, n, m, "pc_transpose B"));
   if (m == 0 || n == 0) return PC_OK;
-  if (ceil_div(m, 32) > 65535) return pc_fail(ctx, PC_ERR_SHAPE, "pc_transpose: m too large");
-  dim3 g(unsigned(ceil_div(n, 32)), unsigned(ceil_div(m, 32)));
+  dim3 g(unsigned(std::min<int64_t>(ceil_div(m, 32) * ceil_div(n, 32), int64_t(1) << 20)));
   switch (dt) {
     case PC_F64: hipLaunchKernelGGL(k_transpose<double>, g, dim3(256), 0, ctx->stream, (const double*)A, lda, m, n, (double*)B, ldb); break;
     case PC_F32: hipLaunchKernelGGL(k_transpose<float>, g, dim3(256), 0, ctx->stream, (const float*)A, lda, m, n, (float*)B, ldb); break;
@@ -270,8 +275,8 @@ extern "C" int pc_diff_norms(pc_ctx* ctx, pc_dtype dt, const void* A, int64_t ld
   if (dt == PC_C64) { base = PC_F32; nn *= 2; la *= 2; lb *= 2; }
   if (dt == PC_C128) { base = PC_F64; nn *= 2; la *= 2; lb *= 2; }
   const int nb = 1024;
-  PC_CHECK(ctx, pc_ws_reserve(ctx, (2 * nb + 2) * 8));
-  double* part = reinterpret_cast<double*>(ctx->ws);
+  PC_CHECK(ctx, pc_ws_reserve(ctx, ctx->ws_skip + (2 * nb + 2) * 8));
+  double* part = reinterpret_cast<double*>(ctx->ws + ctx->ws_skip);
   double* res = part + 2 * nb;
   if (base == PC_F64)
     hipLaunchKernelGGL(k_diff_norms<double>, dim3(nb), dim3(256), 0, ctx->stream, (const double*)A, la, (const double*)B, lb, mm, nn, part);

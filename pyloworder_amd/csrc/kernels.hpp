@@ -40,3 +40,6 @@ void launch_row_stats_f32(hipStream_t st, const float* X, int64_t ldx, int64_t m
 // (k_mfma_mx.hip)
 int launch_pack_w_mx(hipStream_t st, const double* W, int ldw, int n, int nc, int NB, int NCB, _Float16* Wh,
                      _Float16* Wl, float* wscale, double* Wr, int ldwr, int nz = 1, int64_t zb = 0);
+// numpy RandomState(seed).random_sample stream (MT19937, genrand_res53) into
+// count float32 / float64 values (k_rng.hip); -1 for other dtypes
+int launch_mt_omega(hipStream_t st, uint32_t seed, int64_t count, pc_dtype dt, void* out);

@@ -180,6 +180,11 @@ PC_DEV void add_elem(float* p, double v) { *p = float(double(*p) + v); }
 PC_DEV void add_elem(cdouble* p, cdouble v) { p->x += v.x; p->y += v.y; }
 PC_DEV void add_elem(cfloat* p, cdouble v) { p->x = float(p->x + v.x); p->y = float(p->y + v.y); }
 
+// nnull[0] = n - rank: the columns of Q the null-space completion builds
+__global__ void k_null_count(const int* rank, int n, int* nnull) {
+  if (threadIdx.x == 0) nnull[0] = n - rank[0];
+}
+
 // QR output: R (n x n upper, rows >= rank zero); marks the n - rank null columns of Q
 template <class T, class TO>
 __global__ __launch_bounds__(256) void k_store_r(const T* Rm, int ldi, int n, TO* R, int64_t ldr, const int* rank,
@@ -211,6 +216,76 @@ __global__ __launch_bounds__(256) void k_qr_apply_phase(T* Rm, int ldr, T* Qm, i
   const int i = int(idx / n), j = int(idx % n);
   Rm[int64_t(i) * ldr + j] = conj_(ph[i]) * Rm[int64_t(i) * ldr + j];
   Qm[int64_t(i) * ldq + j] = Qm[int64_t(i) * ldq + j] * ph[j];
+}
+
+// ---------------------------------------------------------------------------
+// LAPACK's sign convention for the QR (numpy.linalg.qr = ?geqrf + ?orgqr, the
+// leaf QR of the reference's tsqr, pyLOM/vmmath/svd.py:28-36, 62).  ?larfg
+// takes beta = -sign(Re alpha) ||x|| at step k, so R_H = D R+ and Q_H = Q+ D,
+// where Q+ R+ is the QR with a positive real diagonal and D = diag(+-1).  The
+// Householder reflectors of A are those of Q+ (column k of A is R+_kk > 0 times
+// column k of Q+ plus earlier columns), and the alpha of step k of the
+// Householder QR of an orthonormal Q+ is the k-th pivot of the LU of its top
+// n x n block Q1 with the diagonal shifted by s_k = -sign(Re pivot_k)
+// (Ballard, Demmel, Grigori, Jacquelin, Nguyen, Solomonik, "Reconstructing
+// Householder vectors from tall-skinny QR", IPDPS 2014).  So D_k = s_k, read
+// from the first n rows of Q alone.  A column already reduced (|alpha| = 1,
+// ||x|| = 0) takes no reflection in ?larfg (tau = 0, beta = alpha): then
+// D_k = sign(alpha).
+// ---------------------------------------------------------------------------
+// Q1[i][:] = row i of the final Q (i < n): y_i W plus the null-space fix of row i
+template <class T, class TS>
+__global__ __launch_bounds__(256) void k_q1_rows(const TS* X, int64_t ldx, int n, const T* W, int ldw,
+                                                 const int* nnull, const int* jsel, const T* cnull, T* Q1) {
+  const int i = blockIdx.x;
+  __shared__ T yrow[512];
+  for (int j = threadIdx.x; j < n; j += 256) yrow[j] = load_elem<T, TS>(X, int64_t(i) * ldx, j);
+  __syncthreads();
+  const int d = nnull[0], g = n - d;
+  for (int k = threadIdx.x; k < n; k += 256) {
+    T u = T(0.0);
+    for (int j = 0; j < n; j++) u += yrow[j] * W[int64_t(j) * ldw + k];
+    if (k >= g)
+      for (int a = 0; a < d; a++)
+        if (jsel[a] == i) u += cnull[int64_t(a) * d + (k - g)];
+    Q1[int64_t(i) * n + k] = u;
+  }
+}
+
+// D (n signs) from Q1 (n x n, overwritten by its shifted LU); one workgroup.
+// The shift makes every pivot at least 1 in modulus, so no pivoting is needed.
+template <class T>
+__global__ __launch_bounds__(1024) void k_lapack_signs(T* Q1, int n, T* D) {
+  constexpr double kReduced = 64 * 2.220446049250313e-16;
+  for (int k = 0; k < n; k++) {
+    const T a = Q1[int64_t(k) * n + k];
+    const double re = real_(a);
+    const T s = T(re >= 0.0 ? -1.0 : 1.0);
+    if (threadIdx.x == 0) {
+      const bool reduced = fabs(sqrt(abs2_(a)) - 1.0) <= kReduced && abs2_(a - T(re)) <= kReduced * kReduced;
+      D[k] = reduced ? T(re >= 0.0 ? 1.0 : -1.0) : s;
+    }
+    const T ip = inv_(a - s);
+    const int w = n - k - 1;
+    for (int64_t idx = threadIdx.x; idx < int64_t(w) * w; idx += 1024) {
+      const int i = k + 1 + int(idx / w), j = k + 1 + int(idx % w);
+      Q1[int64_t(i) * n + j] -= Q1[int64_t(i) * n + k] * ip * Q1[int64_t(k) * n + j];
+    }
+    __syncthreads();
+  }
+}
+
+// R := D R (rows), W := W D (columns), null fix entries of column g + b by D_{g+b}
+template <class T>
+__global__ __launch_bounds__(256) void k_apply_signs(const T* D, int n, T* R, int ldr, T* W, int ldw,
+                                                     const int* nnull, T* cnull) {
+  const int64_t idx = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (idx >= int64_t(n) * n) return;
+  const int i = int(idx / n), j = int(idx % n);
+  R[int64_t(i) * ldr + j] = R[int64_t(i) * ldr + j] * real_(D[i]);
+  W[int64_t(i) * ldw + j] = W[int64_t(i) * ldw + j] * real_(D[j]);
+  const int d = nnull[0], g = n - d;
+  if (idx < int64_t(d) * d) cnull[idx] = cnull[idx] * real_(D[g + int(idx % d)]);
 }
 
 template <class T> struct SmallBufs {
@@ -384,6 +459,27 @@ void launch_null_lev<cdouble>(hipStream_t st, const Plan& p, const void* A, int6
   else
     hipLaunchKernelGGL((k_null_lev<cdouble, cfloat>), dim3(ncand, 1, nz), dim3(256), 0, st, (const cfloat*)A, lda, p.m, p.n,
                        center, mean, var, b.Wf, p.n, b.nnull, ncand, b.lev, b.urows, zb, zx);
+}
+
+template <class T>
+void launch_q1_rows(hipStream_t st, const Plan& p, const void* A, int64_t lda, SmallBufs<T>& b);
+template <>
+void launch_q1_rows<double>(hipStream_t st, const Plan& p, const void* A, int64_t lda, SmallBufs<double>& b) {
+  if (p.dt == PC_F64)
+    hipLaunchKernelGGL((k_q1_rows<double, double>), dim3(p.n), dim3(256), 0, st, (const double*)A, lda, p.n, b.Wf, p.n,
+                       (const int*)b.nnull, (const int*)b.jsel, (const double*)b.cnull, b.tmp);
+  else
+    hipLaunchKernelGGL((k_q1_rows<double, float>), dim3(p.n), dim3(256), 0, st, (const float*)A, lda, p.n, b.Wf, p.n,
+                       (const int*)b.nnull, (const int*)b.jsel, (const double*)b.cnull, b.tmp);
+}
+template <>
+void launch_q1_rows<cdouble>(hipStream_t st, const Plan& p, const void* A, int64_t lda, SmallBufs<cdouble>& b) {
+  if (p.dt == PC_C128)
+    hipLaunchKernelGGL((k_q1_rows<cdouble, cdouble>), dim3(p.n), dim3(256), 0, st, (const cdouble*)A, lda, p.n, b.Wf,
+                       p.n, (const int*)b.nnull, (const int*)b.jsel, (const cdouble*)b.cnull, b.tmp);
+  else
+    hipLaunchKernelGGL((k_q1_rows<cdouble, cfloat>), dim3(p.n), dim3(256), 0, st, (const cfloat*)A, lda, p.n, b.Wf,
+                       p.n, (const int*)b.nnull, (const int*)b.jsel, (const cdouble*)b.cnull, b.tmp);
 }
 
 // select the d lowest-leverage candidates, build the completion:
@@ -811,8 +907,9 @@ static int core_t(pc_ctx* ctx, const Plan& p, const void* A, int64_t lda, int ce
     hipLaunchKernelGGL((k_qr_phase<T>), dim3(nblk(n)), dim3(256), 0, st, b.J, n, n, b.tmp);
     hipLaunchKernelGGL((k_qr_apply_phase<T>), dim3(nblk(nn)), dim3(256), 0, st, b.J, n, b.UM, n, n,
                        (const T*)b.tmp);
-    hipLaunchKernelGGL((k_store_r<T, typename VtType<T, ST>::type>), dim3(nblk(nn)), dim3(256), 0, st, b.J, n, n,
-                       (typename VtType<T, ST>::type*)out.VT, out.ldvt, b.rank, b.nnull);
+    // R is stored after the LAPACK signs are known (below); the null-space
+    // completion needs the count of null columns now
+    hipLaunchKernelGGL(k_null_count, dim3(1), dim3(64), 0, st, (const int*)b.rank, n, b.nnull);
   }
   // W = W1[:, perm] * (R2^-1 * UM)
   hipLaunchKernelGGL((k_gemm_small<T>), gg, dim3(256), 0, st, 0, 0, n, n, n, T(1.0), b.R2i, n, (const int*)nullptr,
@@ -824,6 +921,16 @@ static int core_t(pc_ctx* ctx, const Plan& p, const void* A, int64_t lda, int ce
   launch_null_lev<T>(st, p, A, lda, (p.mixed && center == 1) ? 3 : center, mean, var, b, ncand);
   hipLaunchKernelGGL((k_null_build<T>), dim3(1), dim3(1024), 0, st, p.m, n, b.nnull, ncand, b.lev, b.urows, b.Wf, n,
                      b.jsel, b.cnull, nwork, b.status);
+  if (mode == CORE_QR) {
+    // LAPACK's signs (numpy.linalg.qr): D from the first n rows of Q, then
+    // R = D R+, Q = Q+ D through W and the null fix; Q1 in tmp, D in G2
+    launch_q1_rows<T>(st, p, A, lda, b);
+    hipLaunchKernelGGL((k_lapack_signs<T>), dim3(1), dim3(1024), 0, st, b.tmp, n, b.G2);
+    hipLaunchKernelGGL((k_apply_signs<T>), dim3(nblk(nn)), dim3(256), 0, st, (const T*)b.G2, n, b.J, n, b.Wf, n,
+                       (const int*)b.nnull, b.cnull);
+    hipLaunchKernelGGL((k_store_r<T, typename VtType<T, ST>::type>), dim3(nblk(nn)), dim3(256), 0, st, b.J, n, n,
+                       (typename VtType<T, ST>::type*)out.VT, out.ldvt, b.rank, b.nnull);
+  }
   // ---- P3
   to_operand<T>(st, p, b.Wf, nc, b, true);
   PC_CHECK(ctx, (launch_p3<T, ST>(ctx, p, A, lda, center, mean, var, b, out)));

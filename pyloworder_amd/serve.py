@@ -112,9 +112,10 @@ class DeviceExecutor:
         if req.kind == POD:
             return P.run(req.X, remove_mean=self.remove_mean, precision=self.precision)
         k = req.modes or basis.U.shape[1]
+        # projections and reconstructions run FP64 MFMA on the stored data
         if req.kind == PROJECT:
-            return P.project(basis.U[:, :k], req.X, precision=self.precision)
-        return P.reconstruct(basis.U[:, :k], basis.S[:k], basis.V[:k, :], precision=self.precision)
+            return P.project(basis.U[:, :k], req.X, precision="fp64")
+        return P.reconstruct(basis.U[:, :k], basis.S[:k], basis.V[:k, :], precision="fp64")
 
     def make_stream(self):
         import torch

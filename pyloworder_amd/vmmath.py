@@ -66,7 +66,6 @@ def tsqr_svd(Ai, precision: str = "fp64", center: int = 0):
     return a.give(U), a.give(S), a.give(V)
 
 
-@cr("math.tsqr_svd_grouped")
 def _group_outputs(args, dt):
     """Output tensors (U, S, V) per problem.  Problems of one shape get views of
     three stacked tensors, so S and V sit at constant strides: the library then
@@ -86,6 +85,7 @@ def _group_outputs(args, dt):
     return out
 
 
+@cr("math.tsqr_svd_grouped")
 def tsqr_svd_grouped(As, precision: str = "fp64", center: int = 0):
     """tsqr_svd of every matrix of ``As`` in one grouped call (pc_grouped_tsqr_svd).
 
@@ -181,7 +181,9 @@ class GroupedGraph:
 
 @cr("math.tsqr")
 def tsqr(Ai, precision: str = "fp64"):
-    """Qi (m, n), R (n, n) — pyLOM/vmmath/svd.py:49-118.  R is unique up to row signs."""
+    """Qi (m, n), R (n, n) — pyLOM/vmmath/svd.py:49-118.  The signs are LAPACK's
+    (?geqrf, as numpy.linalg.qr in the reference's one-rank tsqr): R_kk is
+    real with the sign -sign(Re alpha_k) of the Householder step."""
     a = Arg(Ai, "Ai")
     m, n = a.t.shape
     Q = empty((m, n), a.t.dtype, a.t)
@@ -226,30 +228,41 @@ def reference_omega(n: int, r: int, seed: int, dtype) -> np.ndarray:
     return rs.random_sample((n, r)).astype(dtype)
 
 
-def _omega_tensor(a: Arg, r: int, seed: int, omega):
-    n = a.t.shape[1]
+def _omega_tensor(a: Arg, r: int, omega):
+    """The caller's sketch matrix as a contiguous device tensor, or None: the
+    library then draws the reference's Omega on the device from the seed."""
     if omega is None:
-        np_dt = {torch.float32: np.float32, torch.float64: np.float64,
-                 torch.complex64: np.complex64, torch.complex128: np.complex128}[a.t.dtype]
-        om = reference_omega(n, r, seed, np_dt)
-        return torch.from_numpy(om).to(a.t.device)
+        return None
+    n = a.t.shape[1]
     o = Arg(omega, "omega", dtype=a.t.dtype).t
     if tuple(o.shape) != (n, r):
         raise ValueError(f"omega must be ({n}, {r})")
     return o.contiguous()
 
 
+def _seed(seed: int) -> int:
+    """seed < 0 draws a time-based seed (pyLOM/vmmath/svd.py:131, 264); numpy's
+    legacy seeding takes 0 <= seed < 2**32."""
+    seed = int(time.time()) if seed < 0 else int(seed)
+    if seed > 0xFFFFFFFF:
+        raise ValueError("Seed must be between 0 and 2**32 - 1")
+    return seed
+
+
 @cr("math.randomized_qr")
 def randomized_qr(Ai, r: int, q: int, seed: int = -1, omega=None, precision: str = "fp64"):
-    """Qi (m, r), B (r, n) — pyLOM/vmmath/svd.py:120-144 (same Omega stream)."""
+    """Qi (m, r), B (r, n) — pyLOM/vmmath/svd.py:120-144.  Without ``omega`` the
+    library draws the reference's Omega (numpy's seeded MT19937 stream) on
+    the device."""
     a = Arg(Ai, "Ai")
     m, n = a.t.shape
-    om = _omega_tensor(a, r, seed, omega)
+    om = _omega_tensor(a, r, omega)
     Q = empty((m, r), a.t.dtype, a.t)
     B = empty((r, n), a.t.dtype, a.t)
     ctx = _ctx()
-    ctx.check(ctx.lib.pc_randomized_qr(ctx.h, pc_dtype(a.t), a.ptr(), a.ld, m, n, r, q, max(seed, 0), om.data_ptr(),
-                                       Q.data_ptr(), Q.stride(0), B.data_ptr(), B.stride(0), _prec(precision)))
+    ctx.check(ctx.lib.pc_randomized_qr(ctx.h, pc_dtype(a.t), a.ptr(), a.ld, m, n, r, q, _seed(seed),
+                                       None if om is None else om.data_ptr(), Q.data_ptr(), Q.stride(0), B.data_ptr(),
+                                       B.stride(0), _prec(precision)))
     return a.give(Q), a.give(B)
 
 
@@ -258,14 +271,14 @@ def randomized_svd(Ai, r: int, q: int, seed: int = -1, omega=None, precision: st
     """Ui (m, r), S (r,), V (r, n) — pyLOM/vmmath/svd.py:254-273."""
     a = Arg(Ai, "Ai")
     m, n = a.t.shape
-    om = _omega_tensor(a, r, seed, omega)
+    om = _omega_tensor(a, r, omega)
     U = empty((m, r), a.t.dtype, a.t)
     S = empty((r,), real_dtype(a.t.dtype), a.t)
     V = empty((r, n), a.t.dtype, a.t)
     ctx = _ctx()
-    ctx.check(ctx.lib.pc_randomized_svd(ctx.h, pc_dtype(a.t), a.ptr(), a.ld, m, n, r, q, max(seed, 0),
-                                        om.data_ptr(), U.data_ptr(), U.stride(0), S.data_ptr(), V.data_ptr(),
-                                        V.stride(0), _prec(precision)))
+    ctx.check(ctx.lib.pc_randomized_svd(ctx.h, pc_dtype(a.t), a.ptr(), a.ld, m, n, r, q, _seed(seed),
+                                        None if om is None else om.data_ptr(), U.data_ptr(), U.stride(0),
+                                        S.data_ptr(), V.data_ptr(), V.stride(0), _prec(precision)))
     return a.give(U), a.give(S), a.give(V)
 
 
@@ -279,13 +292,14 @@ def _dev(x, dtype=None):
 
 
 def _gemm_into(opa, A, opb, B, C, beta, precision):
-    """C = op(A) op(B) + beta C on the device (pc_gemm); row-major operands,
-    op 1 = transpose.  Shapes are taken from C and op(A)."""
+    """C = op(A) op(B) + beta C on the device (pc_gemm, FP64 MFMA whatever
+    ``precision`` asks of the QRs); row-major operands, op 1 = transpose.
+    Shapes are taken from C and op(A)."""
     M, N = C.shape
     K = A.shape[0] if opa else A.shape[1]
     ctx = _ctx()
     ctx.check(ctx.lib.pc_gemm(ctx.h, pc_dtype(C), opa, opb, M, N, K, 1.0, A.data_ptr(), A.stride(0), B.data_ptr(),
-                              B.stride(0), float(beta), C.data_ptr(), C.stride(0), _prec(precision)))
+                              B.stride(0), float(beta), C.data_ptr(), C.stride(0), _lib.PC_PREC_FP64))
     return C
 
 

@@ -123,26 +123,37 @@ def test_zero_and_constant_columns(rng):
     assert metrics.recon_rmse(Y, _np(U), _np(S), _np(V)) <= 1e-12
 
 
-def test_tsqr_qr(rng):
-    A = rng.standard_normal((5000, 30)) @ np.diag(np.logspace(0, -5, 30))
+@pytest.mark.parametrize("dt", [np.float64, np.float32, np.complex128, np.complex64])
+def test_tsqr_lapack_signs(dt, rng):
+    """tsqr returns the reference's Q and R (pyLOM/vmmath/svd.py:49-118 at one
+    rank = numpy.linalg.qr, LAPACK ?geqrf signs) elementwise, not only up to
+    signs: R to 1e-9 relative of max|R| (FP64 math), Q to 1e-8."""
+    cx = np.dtype(dt).kind == "c"
+    A = rng.standard_normal((5000, 30)) + (1j * rng.standard_normal((5000, 30)) if cx else 0)
+    A = (A @ np.diag(np.logspace(0, -5, 30))).astype(dt)
     Q, R = pl.math.tsqr(A)
     Q, R = _np(Q), _np(R)
+    Ao = A.astype(np.complex128 if cx else np.float64)
+    Qo, Ro = oracle.tsqr([Ao])
     assert np.abs(np.tril(R, -1)).max() == 0.0
-    assert np.all(np.diag(R) >= 0.0)  # the QR sign convention: diag(R) >= 0
-    assert np.abs(Q @ R - A).max() <= 1e-12 * np.abs(A).max()
-    assert metrics.orthogonality(Q) <= 1e-10
-    Qo, Ro = oracle.tsqr(oracle.split_rows(A, 3))
-    assert np.allclose(np.abs(R), np.abs(Ro), rtol=1e-9, atol=1e-12 * np.abs(Ro).max())
+    tol = 1e-9 if np.dtype(dt).itemsize >= 16 or dt == np.float64 else 2e-6
+    assert np.abs(R - Ro).max() <= tol * np.abs(Ro).max()
+    assert np.abs(Q - Qo[0]).max() <= max(tol, 1e-8) * 10
+    assert np.abs(Q @ R - Ao).max() <= (1e-12 if tol < 1e-8 else 1e-6) * np.abs(Ao).max()
+    assert metrics.orthogonality(Q) <= (1e-10 if tol < 1e-8 else 1e-5)
+    # the tree R of 3 simulated ranks equals it up to row signs
+    Qt, Rt = oracle.tsqr(oracle.split_rows(Ao, 3))
+    assert np.allclose(np.abs(R), np.abs(Rt), rtol=1e-5, atol=tol * np.abs(Rt).max())
 
 
-def test_tsqr_complex_real_nonnegative_diagonal(rng):
-    A = (rng.standard_normal((3000, 12)) + 1j * rng.standard_normal((3000, 12))) @ np.diag(np.logspace(0, -3, 12))
+def test_tsqr_already_reduced_columns_keep_lapack_sign():
+    """A = [diag(d); 0] is reduced from the start: LAPACK takes no reflection
+    (tau = 0) and keeps R_kk = d_k, negative entries included."""
+    d = np.array([3.0, -2.0, 1.5, -0.5])
+    A = np.vstack([np.diag(d), np.zeros((60, 4))])
     Q, R = pl.math.tsqr(A)
-    Q, R = _np(Q), _np(R)
-    d = np.diag(R)
-    assert np.abs(d.imag).max() <= 1e-12 * np.abs(d).max() and np.all(d.real >= 0.0)
-    assert np.abs(Q @ R - A).max() <= 1e-12 * np.abs(A).max()
-    assert metrics.orthogonality(Q) <= 1e-10
+    Qo, Ro = np.linalg.qr(A)
+    assert np.allclose(_np(R), Ro, atol=1e-14) and np.allclose(_np(Q), Qo, atol=1e-14)
 
 
 def test_dense_svd_square_and_wide(rng):
@@ -170,6 +181,10 @@ def test_randomized_qr_shapes():
     assert Q.shape == (5000, 8) and B.shape == (8, 40)
     assert metrics.orthogonality(Q) <= 1e-10
     assert np.allclose(Q.T @ X, B, atol=1e-10 * np.abs(B).max())
+    # the reference's Q and B themselves (same Omega stream, LAPACK QR signs)
+    Qo, Bo = oracle.randomized_qr([X], 8, 2, seed=3)
+    assert np.abs(Q - Qo[0]).max() <= 1e-8
+    assert np.abs(B - Bo).max() <= 1e-8 * np.abs(Bo).max()
 
 
 def test_matmul_variants(rng):
@@ -250,26 +265,28 @@ def test_centered_full_width_rank_deficient(precision, dt):
 @pytest.mark.parametrize("q", [0, 2])
 def test_streaming_qr_matches_oracle(q, rng):
     """init_qr_streaming / update_qr_streaming (pyLOM/vmmath/svd.py:175-225)
-    against the oracle with the same Omega and the QR sign convention of this
-    build (diag(R) >= 0), on full-rank data with a decaying spectrum (every
-    sketch has full rank r): sketches and Q2 B2 to 1e-8 relative."""
+    against the oracle with the same Omega and the reference's QR signs
+    (LAPACK's, as numpy.linalg.qr), on full-rank data with a decaying spectrum:
+    the accumulated sketch Yo, Q2 and B2 elementwise to 1e-8 relative."""
     m, r = 6000, 12
     A1 = rng.standard_normal((m, 40)) @ np.diag(np.logspace(0, -2, 40))
     A2 = rng.standard_normal((m, 30)) @ np.diag(np.logspace(0, -2, 30))
     om1 = rng.random((40, r)); om2 = rng.random((30, r))
     Q, B, Y = pl.math.init_qr_streaming(A1, r, q, omega=om1)
-    Qo, Bo, Yo_ = oracle.init_qr_streaming([A1], r, q, omega=om1, positive=True)
+    Qo, Bo, Yo_ = oracle.init_qr_streaming([A1], r, q, omega=om1)
     assert np.abs(Y - Yo_[0]).max() <= 1e-8 * np.abs(Yo_[0]).max()
-    assert np.abs(Q @ B - Qo[0] @ Bo).max() <= 1e-8 * np.abs(A1).max()
+    assert np.abs(Q - Qo[0]).max() <= 1e-8
+    assert np.abs(B - Bo).max() <= 1e-8 * np.abs(Bo).max()
     Yd = torch.from_numpy(Y.copy()).cuda()
     Q2, B2, Y2 = pl.math.update_qr_streaming(A2, torch.from_numpy(Q).cuda(), torch.from_numpy(B).cuda(), Yd, r, q,
                                               omega=om2)
     assert Y2 is Yd  # updated in place on the device
-    Q2o, B2o, Y2o = oracle.update_qr_streaming([A2], Qo, Bo, [Yo_[0].copy()], r, q, omega=om2, positive=True)
+    Q2o, B2o, Y2o = oracle.update_qr_streaming([A2], Qo, Bo, [Yo_[0].copy()], r, q, omega=om2)
     Q2, B2, Y2 = (_np(t) for t in (Q2, B2, Y2))
     assert B2.shape == (r, 70)
     assert np.abs(Y2 - Y2o[0]).max() <= 1e-8 * np.abs(Y2o[0]).max()
-    assert np.abs(Q2 @ B2 - Q2o[0] @ B2o).max() <= 1e-8 * np.abs(A1).max()
+    assert np.abs(Q2 - Q2o[0]).max() <= 1e-8
+    assert np.abs(B2 - B2o).max() <= 1e-8 * np.abs(B2o).max()
     assert np.abs(Q2.T @ Q2 - np.eye(r)).max() <= 1e-10
 
 

@@ -59,6 +59,23 @@ def test_randomized_reference_omega():
     assert metrics.max_principal_angle(U[:, :k], d["U"][:, :k], k) <= 1e-7
 
 
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_randomized_device_omega_is_the_reference_stream(dt):
+    """omega = None draws Omega on the device from the seed: numpy's
+    RandomState(seed).random_sample((n, r)) (pyLOM/vmmath/svd.py:131-133)
+    bit for bit, so every output equals the call given numpy's Omega exactly."""
+    d = load("randomized")
+    X = d["X"].astype(dt)
+    r, q, seed = int(d["r"]), int(d["q"]), int(d["seed"])
+    for sd in (seed, 0, 2**32 - 1):
+        om = pl.math.reference_omega(X.shape[1], r, sd, dt)
+        Q1, B1 = pl.math.randomized_qr(X, r, q, seed=sd)
+        Q2, B2 = pl.math.randomized_qr(X, r, q, seed=sd, omega=om)
+        assert np.array_equal(Q1, Q2) and np.array_equal(B1, B2)
+    with pytest.raises(ValueError):
+        pl.math.randomized_qr(X, r, q, seed=2**32)
+
+
 def test_spod_fixture():
     d = load("spod")
     Q, f, nB = pl.SPOD.blocks(d["X"], d["t"], int(d["nDFT"]), int(d["nolap"]))

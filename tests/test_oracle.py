@@ -265,8 +265,8 @@ def test_update_qr_streaming_recovers_low_rank_history():
 
 def test_update_qr_streaming_sign_convention_changes_the_sketch():
     """Yo accumulates (A A^T)^q A Omega whose columns carry the QR's signs: the
-    two conventions give different sketches (the reason this path is compared
-    to the oracle under one stated convention)."""
+    two conventions give different sketches (the reason the device tsqr
+    reproduces LAPACK's signs, and the streaming path is pinned elementwise)."""
     rng = np.random.default_rng(23)
     A1 = rng.standard_normal((200, 10))
     A2 = rng.standard_normal((200, 8))
