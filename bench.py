@@ -50,6 +50,10 @@ def parse():
                          "the FETCH_SIZE / WRITE_SIZE counters")
     ap.add_argument("--cpu-rows", type=int, default=400_000)
     ap.add_argument("--mode", default="both", choices=["both", "mixed", "fp64"])
+    ap.add_argument("--no-extras", action="store_true",
+                    help="pod workload: skip the randomized, grouped and serving summaries")
+    ap.add_argument("--rand-r", type=int, default=74, help="randomized sketch width (64 modes + 10)")
+    ap.add_argument("--rand-q", type=int, default=3, help="randomized power iterations")
     ap.add_argument("--rate", type=float, default=200.0, help="serve: Poisson arrival rate, requests/s")
     ap.add_argument("--duration", type=float, default=20.0, help="serve: arrival window, s")
     ap.add_argument("--slots", type=int, default=4, help="serve: requests in flight")
@@ -116,6 +120,55 @@ class Solver:
         cnt = (C.c_int64 * 5)()
         self.ctx.check(self.ctx.lib.pc_profile_read(self.ctx.h, ms, cnt, 5))
         return list(ms), list(cnt)
+
+
+class RandSolver:
+    """Direct C-ABI randomized_svd (pyLOM/vmmath/svd.py:254-273), outputs
+    preallocated; Omega drawn on the device from the seed."""
+
+    def __init__(self, X, r, q, seed=2024, precision="mixed"):
+        import torch
+        from pyloworder_amd import _lib
+        from pyloworder_amd._tensor import pc_dtype, real_dtype
+        self.ctx = _lib.context()
+        self.X = X
+        m, n = X.shape
+        self.m, self.n, self.r, self.q, self.seed = m, n, r, q, seed
+        self.U = torch.empty((m, r), dtype=X.dtype, device="cuda")
+        self.S = torch.empty((r,), dtype=real_dtype(X.dtype), device="cuda")
+        self.V = torch.empty((r, n), dtype=X.dtype, device="cuda")
+        self.dt = pc_dtype(X)
+        self.prec = _lib.PC_PREC_MIXED if precision == "mixed" else _lib.PC_PREC_FP64
+
+    def __call__(self):
+        c = self.ctx
+        c.check(c.lib.pc_randomized_svd(c.h, self.dt, self.X.data_ptr(), self.X.stride(0), self.m, self.n, self.r,
+                                        self.q, self.seed, None, self.U.data_ptr(), self.U.stride(0),
+                                        self.S.data_ptr(), self.V.data_ptr(), self.V.stride(0), self.prec))
+
+
+def recon_rmse_device(X, U, S, V, center):
+    """Relative reconstruction RMSE (pyLOM/vmmath/stats.py:16-34) of the
+    (row-centred) data against U diag(S) V, computed on the device in FP64
+    row chunks."""
+    import torch
+    num = den = 0.0
+    step = 1 << 20
+    for r0 in range(0, X.shape[0], step):
+        Y = X[r0:r0 + step].double()
+        if center:
+            Y -= Y.mean(dim=1, keepdim=True)
+        R = (U[r0:r0 + step].double() * S.double()) @ V.double()
+        num += float(((Y - R) ** 2).sum())
+        den += float((Y ** 2).sum())
+    return float(np.sqrt(num / den))
+
+
+def randomized_floor(m, n, r, q, b):
+    """Algorithmic bytes of the reference's randomized_svd (2q + 2 passes over
+    A, q + 1 sketches written and read, U written) and its HBM floor."""
+    nbytes = (2 * q + 2) * m * n * b + 2 * (q + 1) * m * r * b + m * r * b
+    return nbytes, nbytes / (HBM_PEAK_GBS * 1e9)
 
 
 def time_solves(solver, steps, warmup):
@@ -268,8 +321,9 @@ def uneven_problems(count, seed=77):
     return out
 
 
-def run_grouped(a):
-    """SPOD (nDFT 256, 50% overlap) of 200000 x 2048 float32 + 64 uneven PODs."""
+def run_grouped(a, precisions=("mixed", "fp64"), emit=True, separate=True):
+    """SPOD (nDFT 256, 50% overlap) of 200000 x 2048 float32 + 64 uneven PODs.
+    Prints the JSON line when ``emit``, and returns it."""
     import ctypes as C
     import torch
     import pyloworder_amd as pl
@@ -284,7 +338,7 @@ def run_grouped(a):
     real_bytes = sum(3 * P.shape[0] * P.shape[1] * 4 for P in probs_X)
     res = {}
     sync = torch.cuda.synchronize
-    for prec in ("mixed", "fp64"):
+    for prec in precisions:
         # full SPOD.run (blocks + grouped per-frequency SVDs + sort/P layout)
         # results are released inside the loops so that each call reuses the
         # caching allocator's blocks instead of allocating beside a live result
@@ -339,19 +393,30 @@ def run_grouped(a):
         sync()
         t_real = (time.perf_counter() - t0) / a.steps
         # the same 64 problems as separate calls, for comparison
-        t0 = time.perf_counter()
+        t_single = float("nan")
+        if separate:
+            t0 = time.perf_counter()
+            for P_, (U_, S_, V_) in zip(probs_X, outs):
+                ctx.check(ctx.lib.pc_tsqr_svd(ctx.h, _lib.PC_F32, P_.data_ptr(), P_.stride(0), P_.shape[0],
+                                              P_.shape[1], 1, U_.data_ptr(), U_.stride(0), S_.data_ptr(),
+                                              V_.data_ptr(), V_.stride(0), pc))
+            sync()
+            t_single = time.perf_counter() - t0
+        # reconstruction RMSE of every real problem (centred rows, as solved)
+        rm = []
         for P_, (U_, S_, V_) in zip(probs_X, outs):
-            ctx.check(ctx.lib.pc_tsqr_svd(ctx.h, _lib.PC_F32, P_.data_ptr(), P_.stride(0), P_.shape[0], P_.shape[1],
-                                          1, U_.data_ptr(), U_.stride(0), S_.data_ptr(), V_.data_ptr(),
-                                          V_.stride(0), pc))
-        sync()
-        t_single = time.perf_counter() - t0
+            Yc = P_.double() - P_.double().mean(dim=1, keepdim=True)
+            R_ = (U_.double() * S_.double()) @ V_.double()
+            rm.append(float(torch.linalg.norm(Yc - R_) / torch.linalg.norm(Yc)))
+            del Yc, R_
         del outs
         tot = t_spod + t_real
         res[prec] = {"spod_run_ms": round(t_spod * 1e3, 2), "spod_svd_stage_ms": round(t_blocksvd * 1e3, 2),
                      "spod_svd_stage_graph_ms": round(t_graph * 1e3, 2),
                      "spod_svd_stage_gbs": round(spod_bytes / t_blocksvd / 1e9, 1),
-                     "real64_grouped_ms": round(t_real * 1e3, 2), "real64_separate_calls_ms": round(t_single * 1e3, 2),
+                     "real64_grouped_ms": round(t_real * 1e3, 2),
+                     "real64_separate_calls_ms": None if t_single != t_single else round(t_single * 1e3, 2),
+                     "real64_max_recon_rmse": max(rm),
                      "real64_gbs": round(real_bytes / t_real / 1e9, 1),
                      "solves_per_s": round((nf + len(probs_X)) / tot, 1),
                      "gbs": round((spod_bytes + real_bytes) / tot / 1e9, 1)}
@@ -364,8 +429,12 @@ def run_grouped(a):
            "config": {"workload": "SPOD.run nDFT=256 nolap=128 on 200000 x 2048 float32 + 64 real POD problems "
                                   "m log-U[5e4, 2e6], n U[16, 256]", "nf": nf, "nBlks": nB,
                       "algorithmic_GB": round((spod_bytes + real_bytes) / 1e9, 2)},
-           "mixed": res["mixed"], "fp64": res["fp64"]}
-    print(json.dumps(out), flush=True)
+           "mixed": res.get("mixed"), "fp64": res.get("fp64")}
+    if emit:
+        print(json.dumps(out), flush=True)
+    del X, probs_X
+    torch.cuda.empty_cache()
+    return out
 
 
 def pool_device(m, n, decay, seed):
@@ -385,7 +454,7 @@ def pool_device(m, n, decay, seed):
     return X
 
 
-def run_serve(a):
+def run_serve(a, emit=True):
     """Poisson arrivals of POD solves (50 %), projections (25 %) and
     reconstructions (25 %) on resident bases; m log-uniform in [1e4, 4e6],
     n log-uniform in [8, 256]; float32 mixed path; 200 GB admission budget."""
@@ -481,7 +550,11 @@ def run_serve(a):
                       "workspace_reserve_per_slot_GB": round(reserve / 1e9, 3)},
            "p50_ms": summ["p50_ms"], "p99_ms": summ["p99_ms"], "serve": summ,
            "algorithmic_gbs": round(gbs, 1), "late_submissions": late}
-    print(json.dumps(out), flush=True)
+    if emit:
+        print(json.dumps(out), flush=True)
+    del pools
+    torch.cuda.empty_cache()
+    return out
 
 
 def run_spod(a):
@@ -584,12 +657,60 @@ def main():
     Ur = sref.U[:, :k].double()
     P = Um - Ur @ (Ur.T @ Um)
     ang = float(torch.linalg.matrix_norm(P, ord=2).clamp(max=1.0).arcsin())
-    del sref, Um, Ur, P
+    del Um, Ur, P
     acc = {"sigma_relerr_gated": float(rel[gated].max()), "n_gated": k,
            "sigma_relerr_all": float(rel[nz].max()), "sigma_normwise": float(np.abs(Sm - Sr).max() / Sr[0]),
-           "max_principal_angle_gated": ang}
+           "max_principal_angle_gated": ang,
+           "recon_rmse": recon_rmse_device(X32, smx.U, smx.S, smx.V, True),
+           "recon_rmse_fp64_math": recon_rmse_device(X32, sref.U, sref.S, sref.V, True)}
     log(f"[bench] mixed accuracy {acc}")
+    rand = None
+    if not a.no_extras:
+        # randomized_svd r = 74 (64 modes + 10), q = 3 on the same float32 data
+        log(f"[bench] randomized mixed r={a.rand_r} q={a.rand_q}")
+        rs = RandSolver(X32, a.rand_r, a.rand_q)
+        msr, latr = time_solves(rs, a.steps, a.warmup)
+        kk = min(64, a.rand_r)
+        Sr_ = rs.S.double().cpu().numpy()
+        relr = np.abs(Sr_[:kk] - Sr[:kk]) / Sr[:kk]
+        Ur = sref.U[:, :kk].double()
+        Pm = rs.U[:, :kk].double() - Ur @ (Ur.T @ rs.U[:, :kk].double())
+        angr = float(torch.linalg.matrix_norm(Pm, ord=2).clamp(max=1.0).arcsin())
+        del Ur, Pm
+        nbytes, floor_s = randomized_floor(m, n, a.rand_r, a.rand_q, 4)
+        rand = {"r": a.rand_r, "q": a.rand_q, "kept_modes": kk, "ms_per_solve": round(msr, 3),
+                "solves_per_s": round(1e3 / msr, 3), "p50_ms": round(latr[len(latr) // 2], 3),
+                "p99_ms": round(latr[min(len(latr) - 1, int(0.99 * len(latr)))], 3),
+                "algorithmic_bytes": nbytes, "hbm_floor_ms": round(floor_s * 1e3, 3),
+                "floor_frac": round(floor_s * 1e3 / msr, 4), "algorithmic_gbs": round(nbytes / (msr * 1e-3) / 1e9, 1),
+                "method": "two mixed Gram passes give A = Q_Y M; the reference's randomized SVD (same Omega, "
+                          "q power steps) of the 256 x 256 FP64 factor M; one mixed product pass U = A W",
+                "sigma_relerr_kept_vs_full_svd": float(relr.max()),
+                "max_principal_angle_kept_vs_full_svd": angr,
+                "recon_rmse": recon_rmse_device(X32, rs.U, rs.S, rs.V, False)}
+        log(f"[bench] randomized: {rand}")
+        del rs
+    del sref
     value = 3 * m * n * 4 / (msmx * 1e-3) / 1e9
+    grouped = served = None
+    if not a.no_extras:
+        del smx, X32
+        torch.cuda.empty_cache()
+        ga = argparse.Namespace(**vars(a))
+        ga.steps, ga.warmup = min(a.steps, 3), 1
+        log("[bench] grouped summary (mixed)")
+        g = run_grouped(ga, precisions=("mixed",), emit=False, separate=False)
+        grouped = {"config": g["config"], "solves_per_s": g["value"], **g["mixed"]}
+        sa = argparse.Namespace(**vars(a))
+        sa.duration = 10.0
+        log("[bench] serving summary")
+        sv = run_serve(sa, emit=False)
+        sm = sv["serve"]
+        served = {"config": sv["config"], "solves_per_s": sv["value"], "p50_ms": sv["p50_ms"],
+                  "p99_ms": sv["p99_ms"], "requests_per_s": sm.get("requests_per_s"),
+                  "requests": sm.get("requests"), "rejected": sm.get("rejected"), "errors": sm.get("errors"),
+                  "by_kind": {k: sm.get(k) for k in ("pod", "project", "reconstruct") if k in sm},
+                  "algorithmic_gbs": sv["algorithmic_gbs"]}
     cpu = None
     if not a.no_cpu:
         log("[bench] cpu baseline")
@@ -619,6 +740,9 @@ def main():
         "accuracy_mixed_vs_fp64": acc,
         "fp64": res64,
         "mixed_vs_fp64_speedup": round(res64["ms_per_solve"] / msmx, 3) if res64 else None,
+        "randomized": rand,
+        "grouped": grouped,
+        "serving": served,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -233,22 +233,23 @@ __global__ __launch_bounds__(256) void k_qr_apply_phase(T* Rm, int ldr, T* Qm, i
 // ||x|| = 0) takes no reflection in ?larfg (tau = 0, beta = alpha): then
 // D_k = sign(alpha).
 // ---------------------------------------------------------------------------
-// Q1[i][:] = row i of the final Q (i < n): y_i W plus the null-space fix of row i
+// Q1[i][:] = row i of the final Q (i < gridDim.x): y_i W (K = n inputs, cols
+// outputs) plus the null-space fix of row i (nnull may be NULL: no fix)
 template <class T, class TS>
-__global__ __launch_bounds__(256) void k_q1_rows(const TS* X, int64_t ldx, int n, const T* W, int ldw,
+__global__ __launch_bounds__(256) void k_q1_rows(const TS* X, int64_t ldx, int n, int cols, const T* W, int ldw,
                                                  const int* nnull, const int* jsel, const T* cnull, T* Q1) {
   const int i = blockIdx.x;
   __shared__ T yrow[512];
   for (int j = threadIdx.x; j < n; j += 256) yrow[j] = load_elem<T, TS>(X, int64_t(i) * ldx, j);
   __syncthreads();
-  const int d = nnull[0], g = n - d;
-  for (int k = threadIdx.x; k < n; k += 256) {
+  const int d = nnull ? nnull[0] : 0, g = cols - d;
+  for (int k = threadIdx.x; k < cols; k += 256) {
     T u = T(0.0);
     for (int j = 0; j < n; j++) u += yrow[j] * W[int64_t(j) * ldw + k];
     if (k >= g)
       for (int a = 0; a < d; a++)
         if (jsel[a] == i) u += cnull[int64_t(a) * d + (k - g)];
-    Q1[int64_t(i) * n + k] = u;
+    Q1[int64_t(i) * cols + k] = u;
   }
 }
 
@@ -466,19 +467,19 @@ void launch_q1_rows(hipStream_t st, const Plan& p, const void* A, int64_t lda, S
 template <>
 void launch_q1_rows<double>(hipStream_t st, const Plan& p, const void* A, int64_t lda, SmallBufs<double>& b) {
   if (p.dt == PC_F64)
-    hipLaunchKernelGGL((k_q1_rows<double, double>), dim3(p.n), dim3(256), 0, st, (const double*)A, lda, p.n, b.Wf, p.n,
+    hipLaunchKernelGGL((k_q1_rows<double, double>), dim3(p.n), dim3(256), 0, st, (const double*)A, lda, p.n, p.n, b.Wf, p.n,
                        (const int*)b.nnull, (const int*)b.jsel, (const double*)b.cnull, b.tmp);
   else
-    hipLaunchKernelGGL((k_q1_rows<double, float>), dim3(p.n), dim3(256), 0, st, (const float*)A, lda, p.n, b.Wf, p.n,
+    hipLaunchKernelGGL((k_q1_rows<double, float>), dim3(p.n), dim3(256), 0, st, (const float*)A, lda, p.n, p.n, b.Wf, p.n,
                        (const int*)b.nnull, (const int*)b.jsel, (const double*)b.cnull, b.tmp);
 }
 template <>
 void launch_q1_rows<cdouble>(hipStream_t st, const Plan& p, const void* A, int64_t lda, SmallBufs<cdouble>& b) {
   if (p.dt == PC_C128)
-    hipLaunchKernelGGL((k_q1_rows<cdouble, cdouble>), dim3(p.n), dim3(256), 0, st, (const cdouble*)A, lda, p.n, b.Wf,
+    hipLaunchKernelGGL((k_q1_rows<cdouble, cdouble>), dim3(p.n), dim3(256), 0, st, (const cdouble*)A, lda, p.n, p.n, b.Wf,
                        p.n, (const int*)b.nnull, (const int*)b.jsel, (const cdouble*)b.cnull, b.tmp);
   else
-    hipLaunchKernelGGL((k_q1_rows<cdouble, cfloat>), dim3(p.n), dim3(256), 0, st, (const cfloat*)A, lda, p.n, b.Wf,
+    hipLaunchKernelGGL((k_q1_rows<cdouble, cfloat>), dim3(p.n), dim3(256), 0, st, (const cfloat*)A, lda, p.n, p.n, b.Wf,
                        p.n, (const int*)b.nnull, (const int*)b.jsel, (const cdouble*)b.cnull, b.tmp);
 }
 
@@ -665,7 +666,58 @@ __global__ __launch_bounds__(256) void k_unembed(const double* P, int NB, int n,
 // ---------------------------------------------------------------------------
 // core pipeline
 // ---------------------------------------------------------------------------
-enum CoreMode { CORE_SVD = 0, CORE_QR = 1 };
+enum CoreMode { CORE_SVD = 0, CORE_QR = 1, CORE_RAND = 2 };
+
+// Randomized QR / SVD through the factor of the two Gram passes (mixed
+// precision, real float32 storage): A = Q_Y M with Q_Y = A W1[:, perm] R2^-1
+// orthonormal and M (n x n) FP64.  The reference's algorithm
+// (pyLOM/vmmath/svd.py:120-144, 254-273) uses A only through A Omega, A^T Q
+// and Q^T A; with A = Q_Y M every sketch is Q_Y times the sketch of M, every
+// tsqr factor of Q_Y Y_M is Q_Y times that of Y_M, and Q^T A = Q_M^T M.  So
+// randomized_svd(A) = (Q_Y U_M, S_M, V_M) with (U_M, S_M, V_M) the randomized
+// SVD of M, same Omega and power steps: computed in FP64 on the device,
+// then U = A (W1[:, perm] R2^-1 U_M) in one product pass.  The QR's signs
+// are LAPACK's on the m x r Q, read from its first r rows.
+struct RandArgs {
+  int64_t r;
+  int q;
+  uint64_t seed;
+  const void* omega;                       // n x r (ld r), storage type, or NULL: drawn from seed
+  void* Q; int64_t ldq; void* B; int64_t ldb;                 // randomized_qr outputs (or NULL)
+  void* U; int64_t ldu; void* S; void* VT; int64_t ldvt;      // randomized_svd outputs (or NULL)
+};
+static int randomized_core(pc_ctx* ctx, pc_dtype dt, const void* A, int64_t lda, int64_t m, int64_t n, int64_t r,
+                           int q, uint64_t seed, const void* omega, void* Qout, int64_t ldq, void* Bout, int64_t ldb,
+                           void* U, int64_t ldu, void* S, void* VT, int64_t ldvt, int precision);
+static int64_t rand_need(pc_dtype dt, int64_t m, int64_t n, int64_t r, bool draw);
+// bytes of the buffers core_rand_tail takes from the cursor
+inline int64_t rand_tail_bytes(int64_t n, int64_t r) {
+  const int64_t e[] = {n * r * 4, n * r * 8, n * r * 8, r * n * 8, r * 8, n * n * 8, n * r * 8, r * r * 8, r * 8};
+  int64_t t = 0;
+  for (int64_t b : e) t += round_up(b, 256) + 256;
+  return t;
+}
+
+template <class TS>
+__global__ __launch_bounds__(256) void k_widen(const TS* in, int64_t count, double* out) {
+  const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (i < count) out[i] = double(in[i]);
+}
+// out[i][j] = in[i][j] * (D ? D[i] : 1) for rows x cols, narrowed to the storage type
+template <class TO>
+__global__ __launch_bounds__(256) void k_narrow_rows(const double* in, int ldi, int rows, int cols, const double* D,
+                                                     TO* out, int64_t ldo) {
+  const int64_t idx = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (idx >= int64_t(rows) * cols) return;
+  const int i = int(idx / cols), j = int(idx % cols);
+  out[int64_t(i) * ldo + j] = TO(in[int64_t(i) * ldi + j] * (D ? D[i] : 1.0));
+}
+// W[:, k] *= D_k (k < cols)
+__global__ __launch_bounds__(256) void k_scale_cols(double* W, int ldw, int rows, int cols, const double* D) {
+  const int64_t idx = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (idx >= int64_t(rows) * cols) return;
+  W[int64_t(idx / cols) * ldw + idx % cols] *= D[idx % cols];
+}
 
 // status word of a solve -> error code and message
 static int status_error(pc_ctx* ctx, int s0) {
@@ -810,7 +862,12 @@ static int launch_p3(pc_ctx* ctx, const Plan& p, const void* A, int64_t lda, int
 }
 
 template <class T, class ST>
-static int core_t(pc_ctx* ctx, const Plan& p, const void* A, int64_t lda, int center, int mode, CoreOut out) {
+static int core_rand_tail(pc_ctx* ctx, const Plan& p, const void* A, int64_t lda, double* mean, double* var,
+                          SmallBufs<double>& b, WsCursor& cur, const RandArgs& ra);
+
+template <class T, class ST>
+static int core_t(pc_ctx* ctx, const Plan& p, const void* A, int64_t lda, int center, int mode, CoreOut out,
+                  const RandArgs* ra = nullptr) {
   hipStream_t st = ctx->stream;
   int n = p.n;
   int64_t nn = int64_t(n) * n;
@@ -826,6 +883,8 @@ static int core_t(pc_ctx* ctx, const Plan& p, const void* A, int64_t lda, int ce
     // not move, so the group reserves the largest problem's need up front
     if (ctx->gstat && base + need + 4096 > ctx->ws_bytes)
       return pc_fail(ctx, PC_ERR_INVALID, "grouped workspace smaller than a member's need");
+    if (mode == CORE_RAND)  // the factor's randomized SVD and the product pass, nested in this front
+      need += rand_tail_bytes(n, ra->r) + rand_need(PC_F64, n, n, ra->r, false) + 4096;
     PC_CHECK(ctx, pc_ws_reserve(ctx, base + need + 4096));
   }
   WsCursor cur{ctx->ws + base, 0, ctx->ws_bytes - base};
@@ -857,6 +916,12 @@ static int core_t(pc_ctx* ctx, const Plan& p, const void* A, int64_t lda, int ce
     hipLaunchKernelGGL((k_center_right<T>), dim3(nblk(n)), dim3(256), 0, st, b.Hc, n, n);
   hipLaunchKernelGGL((k_pad_identity<T>), dim3(1), dim3(256), 0, st, b.R2, n, n, b.rank);
   run_trinv<T>(st, b.R2, n, b.R2i);
+  if (mode == CORE_RAND) {
+    if constexpr (std::is_same<T, double>::value && std::is_same<ST, float>::value)
+      return core_rand_tail(ctx, p, A, lda, mean, var, b, cur, *ra);
+    else
+      return pc_fail(ctx, PC_ERR_UNSUPPORTED, "randomized through the factor: float32 storage only");
+  }
   int nc = n;
   bool gram_steps_;
   int jb_, jnb2_;
