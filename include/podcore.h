@@ -98,6 +98,19 @@ int pc_row_center(pc_ctx* ctx, pc_dtype dtype, const void* X, int64_t ldx, int64
  * U is m x n (ldu), S n, VT n x n (ldvt, rows = right singular vectors).   */
 int pc_tsqr_svd(pc_ctx* ctx, pc_dtype dtype, const void* A, int64_t lda, int64_t m, int64_t n,
                 int center, void* U, int64_t ldu, void* S, void* VT, int64_t ldvt, int precision);
+/* No counterpart in the reference: pc_tsqr_svd for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments, kernels and results as
+ * pc_tsqr_svd, but the workspace is the library's static device scratch of
+ * 128 MiB per GPU (held until the process ends) instead of the context's
+ * hipMalloc'd buffer; the call makes no hipMalloc / hipFree and never grows a
+ * buffer.  A problem needing more scratch (pc_workspace_query(0, ...)
+ * > 128 MiB: n <= 64 fits up to about 6 million rows, n = 128 about 390,000,
+ * n = 256 about 29,000) returns PC_ERR_NOMEM with no work queued.  Every call on one GPU shares that GPU's scratch:
+ * calls that overlap on one GPU (other streams, other threads, other
+ * contexts) race; the scratch keeps the last call's contents.               */
+int pc_tsqr_svd_noalloc(pc_ctx* ctx, pc_dtype dtype, const void* A, int64_t lda, int64_t m, int64_t n, int center,
+                        void* U, int64_t ldu, void* S, void* VT, int64_t ldvt, int precision);
 /* tsqr  pyLOM/vmmath/svd.py:49-118; src/svd.c:416-527 (s), 565-676 (d),
  *       714-825 (c), 863-974 (z).  Also serves qr (svd.py:28-36, svd.c:237-407).
  * Q m x n (ldq), R n x n upper triangular (ldr).                            */

@@ -39,6 +39,7 @@ SIGNATURES = {
     "pc_row_stats": (ip, [vp, ip, vp, i64, i64, i64, vp, vp]),
     "pc_row_center": (ip, [vp, ip, vp, i64, i64, i64, vp, vp, vp, i64]),
     "pc_tsqr_svd": (ip, [vp, ip, vp, i64, i64, i64, ip, vp, i64, vp, vp, i64, ip]),
+    "pc_tsqr_svd_noalloc": (ip, [vp, ip, vp, i64, i64, i64, ip, vp, i64, vp, vp, i64, ip]),
     "pc_tsqr": (ip, [vp, ip, vp, i64, i64, i64, vp, i64, vp, i64, ip]),
     "pc_svd": (ip, [vp, ip, vp, i64, i64, i64, vp, i64, vp, vp, i64]),
     "pc_randomized_qr": (ip, [vp, ip, vp, i64, i64, i64, i64, ip, C.c_uint64, vp, vp, i64, vp, i64, ip]),

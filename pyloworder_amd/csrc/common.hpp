@@ -86,6 +86,7 @@ struct pc_ctx {
   char* ws = nullptr;     // device workspace
   int64_t ws_bytes = 0;
   int64_t ws_skip = 0;    // bytes at the front of ws held by the calling entry point
+  bool ws_fixed = false;  // ws is borrowed for one call (pc_tsqr_svd_noalloc): never reallocated
   int* dflag = nullptr;   // device status words (error flags, ranks)
   int* hflag = nullptr;   // pinned host mirror
   int* gstat = nullptr;   // grouped solve: where the current problem leaves its status words
@@ -126,6 +127,9 @@ int pc_fail(pc_ctx* ctx, int code, const char* fmt, ...);
 
 // grows ctx->ws to at least `bytes` (sync + realloc when growing)
 int pc_ws_reserve(pc_ctx* ctx, int64_t bytes);
+// takes, once per device and with that device current, the address of the
+// library's static device scratch (solve.hip)
+int pc_scratch_attach(int device);
 
 // bump allocator over the workspace (256-byte aligned)
 struct WsCursor {

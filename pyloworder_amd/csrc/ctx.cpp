@@ -34,6 +34,10 @@ extern "C" int pc_ctx_create(int device, int64_t budget, uint32_t flags, pc_ctx*
   if (hipHostMalloc(&c->hflag, 64 * sizeof(int)) != hipSuccess) {
     (void)hipFree(c->dflag); delete c; return pc_fail(nullptr, PC_ERR_HIP, "hipHostMalloc failed");
   }
+  if (pc_scratch_attach(device) != PC_OK) {
+    (void)hipFree(c->dflag); (void)hipHostFree(c->hflag); delete c;
+    return pc_fail(nullptr, PC_ERR_HIP, "pc_ctx_create: address of the static scratch not available");
+  }
   *out = c;
   return PC_OK;
 }
@@ -84,6 +88,9 @@ extern "C" int pc_synchronize(pc_ctx* ctx) {
 
 int pc_ws_reserve(pc_ctx* ctx, int64_t bytes) {
   if (bytes <= ctx->ws_bytes) return PC_OK;
+  if (ctx->ws_fixed)
+    return pc_fail(ctx, PC_ERR_NOMEM, "the call needs %lld bytes of workspace, its fixed scratch holds %lld",
+                   (long long)bytes, (long long)ctx->ws_bytes);
   // an entry point holding the front of the buffer (ws_skip > 0) reserves its
   // nested needs up front: growing here would free memory still in use
   if (ctx->ws_skip > 0 && ctx->ws)

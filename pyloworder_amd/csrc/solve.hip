@@ -1168,3 +1168,68 @@ extern "C" int pc_tsqr(pc_ctx* ctx, pc_dtype dt, const void* A, int64_t lda, int
 }
 
 #include "ops.inc"
+
+// ---------------------------------------------------------------------------
+// Allocation-free tall-skinny SVD: the workspace of pc_tsqr_svd taken from a
+// file-scope device array instead of the context's (grown with hipMalloc).
+// Each GPU holds its own copy of `scratch` at its own address; the address is
+// taken once per device at context creation (pc_scratch_attach) and only that
+// address is handed to kernels.  Calls on one GPU share the array, so calls
+// that overlap on one GPU race.
+// ---------------------------------------------------------------------------
+constexpr int64_t kScratchBytes = int64_t(128) << 20;
+static __device__ __align__(256) char scratch[kScratchBytes];
+constexpr int kScratchDevices = 64;
+static void* g_scratch_ptr[kScratchDevices] = {};
+static std::mutex g_scratch_mu;
+
+int pc_scratch_attach(int device) {
+  if (device < 0 || device >= kScratchDevices) return PC_ERR_INVALID;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device]) return PC_OK;
+  void* ptr = nullptr;
+  if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return PC_ERR_HIP;
+  g_scratch_ptr[device] = ptr;
+  return PC_OK;
+}
+
+extern "C" int pc_tsqr_svd_noalloc(pc_ctx* ctx, pc_dtype dt, const void* A, int64_t lda, int64_t m, int64_t n,
+                                   int center, void* U, int64_t ldu, void* S, void* VT, int64_t ldvt, int precision) {
+  PC_CHECK(ctx, check_common(ctx, dt, m, n));
+  PC_CHECK(ctx, pc_check_mat(ctx, A, lda, m, n, "pc_tsqr_svd_noalloc A"));
+  PC_CHECK(ctx, pc_check_mat(ctx, U, ldu, m, n, "pc_tsqr_svd_noalloc U"));
+  PC_CHECK(ctx, pc_check_mat(ctx, VT, ldvt, n, n, "pc_tsqr_svd_noalloc VT"));
+  if (!S) return pc_fail(ctx, PC_ERR_INVALID, "pc_tsqr_svd_noalloc: S is NULL");
+  if (center < 0 || center > 2) return pc_fail(ctx, PC_ERR_INVALID, "bad center mode %d", center);
+  if (center && dtype_complex(dt)) return pc_fail(ctx, PC_ERR_UNSUPPORTED, "row centring is for real data");
+  if (ctx->gstat) return pc_fail(ctx, PC_ERR_INVALID, "pc_tsqr_svd_noalloc: not inside a grouped solve");
+  Plan p;
+  PC_CHECK(ctx, make_plan(ctx, dt, m, n, precision, p));
+  const int64_t need = p.cplx ? core_need<cdouble>(p) : core_need<double>(p);
+  if (need > kScratchBytes)
+    return pc_fail(ctx, PC_ERR_NOMEM,
+                   "pc_tsqr_svd_noalloc: m = %lld, n = %lld needs %lld bytes of scratch, more than its %lld "
+                   "(pc_tsqr_svd takes it)", (long long)m, (long long)n, (long long)need, (long long)kScratchBytes);
+  void* sp;
+  {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    sp = ctx->device >= 0 && ctx->device < kScratchDevices ? g_scratch_ptr[ctx->device] : nullptr;
+  }
+  if (!sp) return pc_fail(ctx, PC_ERR_HIP, "pc_tsqr_svd_noalloc: no static scratch on device %d", ctx->device);
+  // the context's own workspace stays where it is (work queued on it is
+  // unaffected); this call runs on the static scratch and never grows it
+  char* ws0 = ctx->ws;
+  const int64_t bytes0 = ctx->ws_bytes, skip0 = ctx->ws_skip;
+  const bool fixed0 = ctx->ws_fixed;
+  ctx->ws = static_cast<char*>(sp);
+  ctx->ws_bytes = kScratchBytes;
+  ctx->ws_skip = 0;
+  ctx->ws_fixed = true;
+  CoreOut o{U, ldu, S, VT, ldvt};
+  const int rc = core_dispatch(ctx, dt, A, lda, m, n, center, CORE_SVD, o, precision);
+  ctx->ws = ws0;
+  ctx->ws_bytes = bytes0;
+  ctx->ws_skip = skip0;
+  ctx->ws_fixed = fixed0;
+  return rc;
+}
