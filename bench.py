@@ -457,12 +457,13 @@ def pool_device(m, n, decay, seed):
 def run_serve(a, emit=True):
     """Poisson arrivals of POD solves (50 %), projections (25 %) and
     reconstructions (25 %) on resident bases; m log-uniform in [1e4, 4e6],
-    n log-uniform in [8, 256]; float32 mixed path; 200 GB admission budget."""
+    n log-uniform in [8, 512]; float32, mixed path for n <= 256 and FP64 MFMA
+    beyond (and for project / reconstruct); 200 GB admission budget."""
     import threading
     import torch
     from pyloworder_amd import serve
     from pyloworder_amd.serve import POD, PROJECT, RECONSTRUCT, Request, Server
-    M_MAX, N_MAX = 4_000_000, 256
+    M_MAX, N_MAX = 4_000_000, 512
     log(f"[bench] serve: generating 3 input pools {M_MAX} x {N_MAX} float32")
     pools = [pool_device(M_MAX, N_MAX, d, 100 + i) for i, d in enumerate((0.9, 0.935, 0.97))]
     rng = np.random.default_rng(12)
@@ -543,9 +544,12 @@ def run_serve(a, emit=True):
     out = {"metric": "serving: solves/s and p50/p99 latency (POD / project / reconstruct stream)",
            "value": summ["solves_per_s"], "unit": "solves/s", "n_gpus": 1, "steps": len(arrivals), "warmup": 2 * a.slots,
            "ms_per_step": round(wall * 1e3 / max(1, summ["requests"]), 3), "higher_is_better": True, "scaling": "weak",
-           "vs_baseline": None, "dtype": "f32 storage, fp16x2 MFMA / FP64 small", "data": "synthetic (device-generated)",
+           "vs_baseline": None,
+           "dtype": "f32 storage; POD n <= 256: fp16x2 MFMA / FP64 small, n > 256: FP64 MFMA; project / reconstruct: "
+                    "FP64 MFMA",
+           "data": "synthetic (device-generated)",
            "config": {"workload": "Poisson arrivals, 50% POD.run / 25% project / 25% reconstruct on resident bases, "
-                                  "m log-U[1e4, 4e6], n log-U[8, 256], decay in {0.9, 0.935, 0.97}, float32 mixed",
+                                  "m log-U[1e4, 4e6], n log-U[8, 512], decay in {0.9, 0.935, 0.97}, float32",
                       "rate_per_s": a.rate, "duration_s": a.duration, "slots": a.slots, "budget_GB": 200,
                       "workspace_reserve_per_slot_GB": round(reserve / 1e9, 3)},
            "p50_ms": summ["p50_ms"], "p99_ms": summ["p99_ms"], "serve": summ,

@@ -144,7 +144,8 @@ __global__ __launch_bounds__(256) void k_reduce_gram_cplx(const double* __restri
 // ---------------------------------------------------------------------------
 template <class T>
 __global__ __launch_bounds__(NT) void k_chol_shift(const T* G, int ldg, int n, double shift_rel, T* R, int ldr,
-                                                   int* status, int64_t zb = 0) {
+                                                   int* status, int64_t zb = 0, const int* only_if = nullptr) {
+  if (only_if && !only_if[0]) return;  // fallback of the multi-workgroup factorisation: it succeeded
   G = zoff(G, zb);
   R = zoff(R, zb);
   status = zoff(status, zb);
@@ -208,16 +209,18 @@ __global__ __launch_bounds__(NT) void k_chol_shift(const T* G, int ldg, int n, d
 // Inverse of an upper-triangular matrix: W = R^{-1}; column j solved by the
 // waves in turn (back substitution, dot products over the solved part).
 // ---------------------------------------------------------------------------
+// With a grid of several workgroups the columns are spread over them (each
+// column is one wave's independent back substitution: same results).
 template <class T>
 __global__ __launch_bounds__(NT) void k_trinv_upper(const T* R, int ldr, int n, T* W, int ldw, int64_t zb = 0) {
   R = zoff(R, zb);
   W = zoff(W, zb);
-  for (int64_t idx = threadIdx.x; idx < int64_t(n) * n; idx += NT) {
+  for (int64_t idx = int64_t(blockIdx.x) * NT + threadIdx.x; idx < int64_t(n) * n; idx += int64_t(gridDim.x) * NT) {
     int i = int(idx / n), j = int(idx % n);
     if (i > j) W[int64_t(i) * ldw + j] = T(0.0);
   }
   int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  for (int j = wv; j < n; j += NT / 64) {
+  for (int j = blockIdx.x * (NT / 64) + wv; j < n; j += gridDim.x * (NT / 64)) {
     // W[:, j]: W[j][j] = 1/R[j][j]; W[i][j] = -(sum_{k=i+1..j} R[i][k] W[k][j]) / R[i][i]
     T rjj = R[int64_t(j) * ldr + j];
     T wjj = inv_(rjj);

@@ -153,6 +153,12 @@ void launch_row_stats_f32(hipStream_t st, const float* X, int64_t ldx, int64_t m
   hipLaunchKernelGGL((k_row_stats<float, double>), g, dim3(256), 0, st, X, ldx, m, n, mean, var);
 }
 
+void launch_row_stats_f64(hipStream_t st, const double* X, int64_t ldx, int64_t m, int64_t n, double* mean,
+                          double* var) {
+  dim3 g = dim3(unsigned(ceil_div(m, 4)));
+  hipLaunchKernelGGL((k_row_stats<double, double>), g, dim3(256), 0, st, X, ldx, m, n, mean, var);
+}
+
 int pc_check_mat(pc_ctx* ctx, const void* p, int64_t ld, int64_t m, int64_t n, const char* what) {
   return check_mat(ctx, p, ld, m, n, what);
 }

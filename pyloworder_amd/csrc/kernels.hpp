@@ -35,6 +35,8 @@ int launch_gemm_mx2(hipStream_t st, const float* X, int64_t ldx, int64_t m, int 
 // FP64 row mean / variance of float32 rows (k_stream.hip)
 void launch_row_stats_f32(hipStream_t st, const float* X, int64_t ldx, int64_t m, int64_t n, double* mean,
                           double* var);
+void launch_row_stats_f64(hipStream_t st, const double* X, int64_t ldx, int64_t m, int64_t n, double* mean,
+                          double* var);
 // pack an FP64 n x nc matrix (ld ldw) into fp16 hi (and lo) fragment order with
 // per-column power-of-two scales; also writes the FP64 value of hi*2^-s (Wr)
 // (k_mfma_mx.hip)
@@ -43,3 +45,12 @@ int launch_pack_w_mx(hipStream_t st, const double* W, int ldw, int n, int nc, in
 // numpy RandomState(seed).random_sample stream (MT19937, genrand_res53) into
 // count float32 / float64 values (k_rng.hip); -1 for other dtypes
 int launch_mt_omega(hipStream_t st, uint32_t seed, int64_t count, pc_dtype dt, void* out);
+// wide problems (256 < real columns <= 2048), FP64 MFMA (k_wide.hip): stripe-pair
+// Gram into NBp x NBp slabs per row chunk, and the K-looped tall product
+int wide_stripe();
+int wide_tile_rows();
+int launch_wgram_f64(hipStream_t st, pc_dtype dt, const void* X, int64_t ldx, int64_t m, int n, int center,
+                     const double* mean, const double* var, int chunks, int64_t tpc, double* part, int NBp);
+int launch_wgemm_f64(hipStream_t st, pc_dtype dt, const void* X, int64_t ldx, int64_t m, int K, int center,
+                     const double* mean, const double* var, const double* W, int64_t ldw, int N, int chunks,
+                     int64_t tpc, void* C, int64_t ldc, int out_f64);

@@ -21,6 +21,7 @@
 #include "kernels.hpp"
 #include "k_small.hip"
 #include "k_dense.hip"
+#include "k_mwg.hip"
 
 int pc_check_mat(pc_ctx* ctx, const void* p, int64_t ld, int64_t m, int64_t n, const char* what);
 
@@ -118,37 +119,79 @@ void launch_block_jacobi(pc_ctx* ctx, T* H, int ldh, int rows, int cols, T* J, i
 }
 
 
-// Small dense factorisations: blocked MFMA kernels for real n <= 256, the
-// generic single-workgroup kernels otherwise (complex, larger n).
+// Small dense factorisations: blocked MFMA kernels for real n <= 256 (one
+// workgroup), one launch per step over many workgroups for larger single
+// problems (k_mwg.hip), the generic single-workgroup kernels otherwise
+// (complex n <= 128, batched groups).
 template <class T>
-void run_chol_shift(hipStream_t st, const T* G, int n, double shift, T* R, int* status, int nz = 1, int64_t zb = 0) {
+inline bool use_fast(int n, int nz) { return std::is_same<T, double>::value && n <= dense::NMAX && nz == 1; }
+inline bool use_mw(int n, int nz) { return n > 128 && nz == 1; }
+
+// work: n x n of T, dsc: 1 double, flag: 1 int (multi-workgroup path only)
+template <class T>
+void run_chol_shift(hipStream_t st, const T* G, int n, double shift, T* R, int* status, int nz = 1, int64_t zb = 0,
+                    T* work = nullptr, double* dsc = nullptr, int* flag = nullptr) {
   if constexpr (std::is_same<T, double>::value)
-    if (n <= dense::NMAX && nz == 1) {
+    if (use_fast<T>(n, nz)) {
       hipLaunchKernelGGL(k_chol_fast, dim3(1), dim3(NT), 0, st, G, n, n, shift, R, n, status);
       return;
     }
-  hipLaunchKernelGGL((k_chol_shift<T>), dim3(1, 1, nz), dim3(NT), 0, st, G, n, n, shift, R, n, status, zb);
+  if (use_mw(n, nz) && work && dsc && flag) {
+    hipLaunchKernelGGL((k_chol_mw_tau<T>), dim3(1), dim3(1024), 0, st, G, n, n, shift, dsc, flag);
+    hipLaunchKernelGGL((k_chol_mw_copy<T>), dim3(mw_grid(int64_t(n) * n, MW)), dim3(MW), 0, st, G, n, n,
+                       (const double*)dsc, work);
+    for (int k = 0; k < n - 1; k++)
+      hipLaunchKernelGGL((k_chol_mw_step<T>), dim3(mw_grid(n - k - 1, MW / 64)), dim3(MW), 0, st, work, n, k,
+                         (const double*)dsc, flag);
+    hipLaunchKernelGGL((k_chol_mw_finish<T>), dim3(mw_grid(int64_t(n) * n, MW)), dim3(MW), 0, st, (const T*)work, n,
+                       R, n, (const int*)flag);
+    // the last pivot, and any attempt that failed, is checked by the generic kernel
+    hipLaunchKernelGGL((k_chol_check_last<T>), dim3(1), dim3(64), 0, st, (const T*)work, n, (const double*)dsc, flag);
+    hipLaunchKernelGGL((k_chol_shift<T>), dim3(1), dim3(NT), 0, st, G, n, n, shift, R, n, status, int64_t(0),
+                       (const int*)flag);
+    return;
+  }
+  hipLaunchKernelGGL((k_chol_shift<T>), dim3(1, 1, nz), dim3(NT), 0, st, G, n, n, shift, R, n, status, zb,
+                     (const int*)nullptr);
 }
 template <class T>
 void run_trinv(hipStream_t st, const T* R, int n, T* W, int nz = 1, int64_t zb = 0) {
   if constexpr (std::is_same<T, double>::value)
-    if (n <= dense::NMAX && nz == 1) {
+    if (use_fast<T>(n, nz)) {
       hipLaunchKernelGGL(k_trinv_cols, dim3(n), dim3(64), 0, st, R, n, n, W, n);
       return;
     }
+  if (use_mw(n, nz)) {  // columns spread over the workgroups
+    hipLaunchKernelGGL((k_trinv_upper<T>), dim3(unsigned(ceil_div(n, NT / 64))), dim3(NT), 0, st, R, n, n, W, n,
+                       int64_t(0));
+    return;
+  }
   hipLaunchKernelGGL((k_trinv_upper<T>), dim3(1, 1, nz), dim3(NT), 0, st, R, n, n, W, n, zb);
 }
+// state: 1 int (multi-workgroup path)
 template <class T>
 void run_pchol(hipStream_t st, const T* G, int n, double tol, T* A, T* R, int* perm, int* rank, int* status,
-               int nz = 1, int64_t zb = 0) {
+               int nz = 1, int64_t zb = 0, int* state = nullptr) {
   if constexpr (std::is_same<T, double>::value)
-    if (n <= dense::NMAX && nz == 1) {
+    if (use_fast<T>(n, nz)) {
       hipLaunchKernelGGL(k_pchol_fast, dim3(1), dim3(NT), 0, st, G, n, n, tol, A, R, n, perm, rank, status);
       return;
     }
+  if (use_mw(n, nz) && state) {
+    hipLaunchKernelGGL((k_pchol_mw_copy<T>), dim3(mw_grid(int64_t(n) * n, MW)), dim3(MW), 0, st, G, n, n, A, R, n,
+                       perm, rank, state);
+    for (int k = 0; k < n; k++) {
+      hipLaunchKernelGGL((k_pchol_mw_pivot<T>), dim3(1), dim3(1024), 0, st, G, n, A, n, k, tol, R, n, perm, rank,
+                         state, status);
+      if (k + 1 < n)
+        hipLaunchKernelGGL((k_pchol_mw_update<T>), dim3(mw_grid(n - k - 1, MW / 64)), dim3(MW), 0, st, A, n, k,
+                           (const T*)R, n, (const int*)state);
+    }
+    return;
+  }
   hipLaunchKernelGGL((k_pchol<T>), dim3(1, 1, nz), dim3(NT), 0, st, G, n, n, tol, A, R, n, perm, rank, status, zb);
 }
-// square Householder QR (n x n), Q and R explicit
+// square Householder QR (n x n), Q and R explicit; work: 2 n^2 + n of T
 template <class T>
 void run_house_qr_sq(hipStream_t st, const T* A, int n, T* Q, T* R, T* work) {
   if constexpr (std::is_same<T, double>::value)
@@ -156,6 +199,24 @@ void run_house_qr_sq(hipStream_t st, const T* A, int n, T* Q, T* R, T* work) {
       hipLaunchKernelGGL(k_house_qr_fast, dim3(1), dim3(NT), 0, st, A, n, n, Q, n, R, n, work);
       return;
     }
+  if (use_mw(n, 1)) {
+    T* Wc = work;
+    T* Qc = work + int64_t(n) * n;
+    T* tau = Qc + int64_t(n) * n;
+    hipLaunchKernelGGL((k_hqr_init<T>), dim3(mw_grid(2 * int64_t(n) * n, MW)), dim3(MW), 0, st, A, n, n, n, n, Wc, Qc);
+    for (int k = 0; k < n; k++) {
+      hipLaunchKernelGGL((k_hqr_reflect<T>), dim3(1), dim3(MW), 0, st, Wc, n, k, tau);
+      if (k + 1 < n)
+        hipLaunchKernelGGL((k_hqr_apply<T>), dim3(mw_grid(n - k - 1, MW / 64)), dim3(MW), 0, st, Wc, n, n, k,
+                           (const T*)tau);
+    }
+    for (int k = n - 1; k >= 0; k--)
+      hipLaunchKernelGGL((k_hqr_qapply<T>), dim3(mw_grid(n - k, MW / 64)), dim3(MW), 0, st, (const T*)Wc, n, n, k,
+                         (const T*)tau, Qc);
+    hipLaunchKernelGGL((k_hqr_out<T>), dim3(mw_grid(2 * int64_t(n) * n, MW)), dim3(MW), 0, st, (const T*)Wc,
+                       (const T*)Qc, n, n, n, Q, n, R, n);
+    return;
+  }
   hipLaunchKernelGGL((k_house_qr<T>), dim3(1), dim3(NT), 0, st, A, n, n, n, Q, n, R, n, work);
 }
 // workspace of the small dense stage (T elements): house QR and null completion
@@ -239,7 +300,8 @@ template <class T, class TS>
 __global__ __launch_bounds__(256) void k_q1_rows(const TS* X, int64_t ldx, int n, int cols, const T* W, int ldw,
                                                  const int* nnull, const int* jsel, const T* cnull, T* Q1) {
   const int i = blockIdx.x;
-  __shared__ T yrow[512];
+  HIP_DYNAMIC_SHARED(char, q1_smem)  // n values of T
+  T* yrow = reinterpret_cast<T*>(q1_smem);
   for (int j = threadIdx.x; j < n; j += 256) yrow[j] = load_elem<T, TS>(X, int64_t(i) * ldx, j);
   __syncthreads();
   const int d = nnull ? nnull[0] : 0, g = cols - d;
@@ -296,12 +358,16 @@ template <class T> struct SmallBufs {
   int *perm3, *rank3;  // pivoted Cholesky of M^H M (Jacobi preconditioner)
   int* jstate;       // block Jacobi: done flag + per-sweep rotation flags
   double* jdstate;   // block Jacobi: negligible squared column norm
+  double* dsc;       // multi-workgroup Cholesky: the shift
   double* Wpad;  // NB x NB real (embedded) operand for the FP64 streaming kernels
   // null completion
   double* lev;
   T* urows;
   int* jsel;
   T* cnull;
+  // wide problems: Z = Yc W1 (m x nr FP64, ld ldz), in U when U is FP64
+  double* Zw;
+  int64_t ldz;
   // mixed operands
   _Float16 *Wh, *Wl;
   float* wscale;
@@ -322,7 +388,11 @@ struct Plan {
   int tr3, chunks3;  // P2/P3 kernels (mixed uses other tiles)
   int64_t tpc3;
   int nparts;  // partial slabs per P1 launch
+  bool wide;   // 256 < real columns <= 2048: stripe-pair FP64 kernels, Z materialised
 };
+
+constexpr int kWideMaxReal = 2048;            // real columns of the widest problem
+constexpr int64_t kWideSlabBytes = int64_t(256) << 20;  // partial Gram slabs of a wide pass
 
 int make_plan(pc_ctx* ctx, pc_dtype dt, int64_t m, int64_t n, int precision, Plan& p) {
   p.dt = dt;
@@ -344,9 +414,27 @@ int make_plan(pc_ctx* ctx, pc_dtype dt, int64_t m, int64_t n, int precision, Pla
     p.tr = f64_tile_rows();
     p.tr3 = f64_tile_rows();
   }
-  if (p.NB < 0)
-    return pc_fail(ctx, PC_ERR_SHAPE, "n = %lld is larger than this build supports (%s)", (long long)n,
-                   p.cplx ? "complex: 128" : "256");
+  p.wide = false;
+  if (p.NB < 0 && p.mixed)
+    return pc_fail(ctx, PC_ERR_UNSUPPORTED, "mixed precision takes n <= %s (n = %lld); use FP64",
+                   p.cplx ? "128 complex" : "256", (long long)n);
+  if (p.NB < 0) {
+    if (p.nr > kWideMaxReal)
+      return pc_fail(ctx, PC_ERR_SHAPE, "n = %lld is larger than this build supports (%s)", (long long)n,
+                     p.cplx ? "complex: 1024" : "2048");
+    // wide: columns in stripes of 128, Gram slabs capped at kWideSlabBytes (the
+    // chunk count still depends on (m, n) only)
+    p.wide = true;
+    p.NB = int(round_up(p.nr, wide_stripe()));
+    p.tr = p.tr3 = wide_tile_rows();
+    const int64_t cap = std::max<int64_t>(1, kWideSlabBytes / (int64_t(p.NB) * p.NB * 8));
+    p.chunks = int(std::min<int64_t>(stream_chunks(m, p.tr), cap));
+    p.tpc = ceil_div(ceil_div(m, p.tr), p.chunks);
+    p.chunks3 = stream_chunks(m, p.tr3);
+    p.tpc3 = ceil_div(ceil_div(m, p.tr3), p.chunks3);
+    p.nparts = p.chunks;
+    return PC_OK;
+  }
   p.chunks = stream_chunks(m, p.tr);
   p.tpc = ceil_div(ceil_div(m, p.tr), p.chunks);
   p.chunks3 = stream_chunks(m, p.tr3);
@@ -372,6 +460,7 @@ int64_t carve(WsCursor& c, const Plan& p, SmallBufs<T>& b, double** mean, double
   b.perm3 = c.take<int>(n); b.rank3 = c.take<int>(4);
   b.jstate = c.take<int>(kBjacMaxSweeps + 8);
   b.jdstate = c.take<double>(4);
+  b.dsc = c.take<double>(4);
   b.Wpad = c.take<double>(int64_t(p.NB) * p.NB);
   int ncand = int(std::min<int64_t>(p.m, 4 * int64_t(n) + 16));
   b.lev = c.take<double>(ncand);
@@ -382,6 +471,9 @@ int64_t carve(WsCursor& c, const Plan& p, SmallBufs<T>& b, double** mean, double
   b.Wl = c.take<_Float16>(int64_t(p.NB) * p.NB);
   b.wscale = c.take<float>(p.NB);
   b.Wr = c.take<double>(int64_t(p.NB) * p.NB);
+  b.Zw = nullptr;
+  b.ldz = p.nr;
+  if (p.wide && dtype_single(p.dt)) b.Zw = c.take<double>(p.m * int64_t(p.nr));
   return c.off;
 }
 
@@ -404,7 +496,8 @@ __global__ __launch_bounds__(256) void k_null_lev(const TS* X, int64_t ldx, int6
   if (d == 0) return;
   int c = blockIdx.x;
   int64_t row = ncand >= m ? c : (int64_t(c) * m) / ncand;
-  __shared__ T yrow[512];
+  HIP_DYNAMIC_SHARED(char, lev_smem)  // n values of T
+  T* yrow = reinterpret_cast<T*>(lev_smem);
   for (int j = threadIdx.x; j < n; j += 256) {
     T v = load_elem<T, TS>(X, row * ldx, j);
     if (center == 3) {
@@ -444,10 +537,10 @@ void launch_null_lev<double>(hipStream_t st, const Plan& p, const void* A, int64
                              const double* var, SmallBufs<double>& b, int ncand, int nz,
                              int64_t zb, int64_t zx) {
   if (p.dt == PC_F64)
-    hipLaunchKernelGGL((k_null_lev<double, double>), dim3(ncand, 1, nz), dim3(256), 0, st, (const double*)A, lda, p.m, p.n,
+    hipLaunchKernelGGL((k_null_lev<double, double>), dim3(ncand, 1, nz), dim3(256), size_t(p.n) * sizeof(double), st, (const double*)A, lda, p.m, p.n,
                        center, mean, var, b.Wf, p.n, b.nnull, ncand, b.lev, b.urows, zb, zx);
   else
-    hipLaunchKernelGGL((k_null_lev<double, float>), dim3(ncand, 1, nz), dim3(256), 0, st, (const float*)A, lda, p.m, p.n,
+    hipLaunchKernelGGL((k_null_lev<double, float>), dim3(ncand, 1, nz), dim3(256), size_t(p.n) * sizeof(double), st, (const float*)A, lda, p.m, p.n,
                        center, mean, var, b.Wf, p.n, b.nnull, ncand, b.lev, b.urows, zb, zx);
 }
 template <>
@@ -455,10 +548,10 @@ void launch_null_lev<cdouble>(hipStream_t st, const Plan& p, const void* A, int6
                               const double* var, SmallBufs<cdouble>& b, int ncand, int nz,
                               int64_t zb, int64_t zx) {
   if (p.dt == PC_C128)
-    hipLaunchKernelGGL((k_null_lev<cdouble, cdouble>), dim3(ncand, 1, nz), dim3(256), 0, st, (const cdouble*)A, lda, p.m,
+    hipLaunchKernelGGL((k_null_lev<cdouble, cdouble>), dim3(ncand, 1, nz), dim3(256), size_t(p.n) * sizeof(cdouble), st, (const cdouble*)A, lda, p.m,
                        p.n, center, mean, var, b.Wf, p.n, b.nnull, ncand, b.lev, b.urows, zb, zx);
   else
-    hipLaunchKernelGGL((k_null_lev<cdouble, cfloat>), dim3(ncand, 1, nz), dim3(256), 0, st, (const cfloat*)A, lda, p.m, p.n,
+    hipLaunchKernelGGL((k_null_lev<cdouble, cfloat>), dim3(ncand, 1, nz), dim3(256), size_t(p.n) * sizeof(cdouble), st, (const cfloat*)A, lda, p.m, p.n,
                        center, mean, var, b.Wf, p.n, b.nnull, ncand, b.lev, b.urows, zb, zx);
 }
 
@@ -467,19 +560,19 @@ void launch_q1_rows(hipStream_t st, const Plan& p, const void* A, int64_t lda, S
 template <>
 void launch_q1_rows<double>(hipStream_t st, const Plan& p, const void* A, int64_t lda, SmallBufs<double>& b) {
   if (p.dt == PC_F64)
-    hipLaunchKernelGGL((k_q1_rows<double, double>), dim3(p.n), dim3(256), 0, st, (const double*)A, lda, p.n, p.n, b.Wf, p.n,
+    hipLaunchKernelGGL((k_q1_rows<double, double>), dim3(p.n), dim3(256), size_t(p.n) * sizeof(double), st, (const double*)A, lda, p.n, p.n, b.Wf, p.n,
                        (const int*)b.nnull, (const int*)b.jsel, (const double*)b.cnull, b.tmp);
   else
-    hipLaunchKernelGGL((k_q1_rows<double, float>), dim3(p.n), dim3(256), 0, st, (const float*)A, lda, p.n, p.n, b.Wf, p.n,
+    hipLaunchKernelGGL((k_q1_rows<double, float>), dim3(p.n), dim3(256), size_t(p.n) * sizeof(double), st, (const float*)A, lda, p.n, p.n, b.Wf, p.n,
                        (const int*)b.nnull, (const int*)b.jsel, (const double*)b.cnull, b.tmp);
 }
 template <>
 void launch_q1_rows<cdouble>(hipStream_t st, const Plan& p, const void* A, int64_t lda, SmallBufs<cdouble>& b) {
   if (p.dt == PC_C128)
-    hipLaunchKernelGGL((k_q1_rows<cdouble, cdouble>), dim3(p.n), dim3(256), 0, st, (const cdouble*)A, lda, p.n, p.n, b.Wf,
+    hipLaunchKernelGGL((k_q1_rows<cdouble, cdouble>), dim3(p.n), dim3(256), size_t(p.n) * sizeof(cdouble), st, (const cdouble*)A, lda, p.n, p.n, b.Wf,
                        p.n, (const int*)b.nnull, (const int*)b.jsel, (const cdouble*)b.cnull, b.tmp);
   else
-    hipLaunchKernelGGL((k_q1_rows<cdouble, cfloat>), dim3(p.n), dim3(256), 0, st, (const cfloat*)A, lda, p.n, p.n, b.Wf,
+    hipLaunchKernelGGL((k_q1_rows<cdouble, cfloat>), dim3(p.n), dim3(256), size_t(p.n) * sizeof(cdouble), st, (const cfloat*)A, lda, p.n, p.n, b.Wf,
                        p.n, (const int*)b.nnull, (const int*)b.jsel, (const cdouble*)b.cnull, b.tmp);
 }
 
@@ -501,7 +594,7 @@ __global__ __launch_bounds__(1024) void k_null_build(int64_t m, int n, const int
   int d = nnull[0];
   if (d == 0) return;
   int g = n - d;
-  __shared__ int pick[512];
+  HIP_DYNAMIC_SHARED(int, pick)  // d <= n candidate indices
   // rank candidates by leverage (ties by index), keep the first d
   for (int c = threadIdx.x; c < ncand; c += 1024) {
     double v = key_asc(lev[c]);
@@ -744,7 +837,22 @@ static int launch_streams(pc_ctx* ctx, const Plan& p, const void* A, int64_t lda
   int64_t ldr = p.cplx ? 2 * lda : lda;
   hipEvent_t ev0;
   pc_prof_begin(ctx, &ev0);
-  if (p.mixed) {
+  if (p.wide) {
+    // P1: FP64 row statistics, then the stripe-pair Gram of Yc; P2: Z = Yc W1
+    // written once, then its stripe-pair Gram
+    if (pass == 1 && center) {
+      if (sdt == PC_F64) launch_row_stats_f64(st, (const double*)A, ldr, p.m, p.nr, mean, var);
+      else launch_row_stats_f32(st, (const float*)A, ldr, p.m, p.nr, mean, var);
+    }
+    if (pass == 1) {
+      rc = launch_wgram_f64(st, sdt, A, ldr, p.m, p.nr, center, mean, var, p.chunks, p.tpc, part, p.NB);
+    } else {
+      rc = launch_wgemm_f64(st, sdt, A, ldr, p.m, p.nr, center, mean, var, b.Wpad, p.NB, p.nr, p.chunks3, p.tpc3,
+                            b.Zw, b.ldz, 1);
+      if (rc == 0)
+        rc = launch_wgram_f64(st, PC_F64, b.Zw, b.ldz, p.m, p.nr, 0, nullptr, nullptr, p.chunks, p.tpc, part, p.NB);
+    }
+  } else if (p.mixed) {
     // center 1: the kernels shift each row by its first element and P1 records
     // that shift in `mean`; center 2: exact FP64 statistics first
     if (pass == 1 && center == 2) {
@@ -846,7 +954,10 @@ static int launch_p3(pc_ctx* ctx, const Plan& p, const void* A, int64_t lda, int
   pc_dtype sdt = p.cplx ? (p.dt == PC_C64 ? PC_F32 : PC_F64) : p.dt;
   int64_t ldr = p.cplx ? 2 * lda : lda;
   int64_t ldu = p.cplx ? 2 * out.ldu : out.ldu;
-  if (p.mixed)
+  if (p.wide)
+    rc = launch_wgemm_f64(st, sdt, A, ldr, p.m, p.nr, center, mean, var, b.Wpad, p.NB, p.nr, p.chunks3, p.tpc3,
+                          out.U, ldu, 0);
+  else if (p.mixed)
     rc = launch_gemm_mx2(st, (const float*)A, ldr, p.m, p.nr, center, mean, var, b.Wh, b.Wl, b.wscale, p.nr,
                          p.chunks3, p.tpc3, (float*)out.U, ldu, bz);
   else
@@ -897,15 +1008,20 @@ static int core_t(pc_ctx* ctx, const Plan& p, const void* A, int64_t lda, int ce
   // ---- P1
   PC_CHECK(ctx, launch_streams<T>(ctx, p, A, lda, center, mean, var, part, b, 1));
   reduce_gram<T>(st, p, part, p.nparts, b.G1);
-  run_chol_shift<T>(st, b.G1, n, p.mixed ? kShiftMx : kShiftF64, b.R1, b.status);
+  // rank[1] / rank[2]: state words of the multi-workgroup factorisations
+  run_chol_shift<T>(st, b.G1, n, p.mixed ? kShiftMx : kShiftF64, b.R1, b.status, 1, 0, b.Awk, b.dsc, b.rank + 2);
   run_trinv<T>(st, b.R1, n, b.W1);
   to_operand<T>(st, p, b.W1, n, b, false);
   if (p.mixed) rounded_back<T>(st, p, b, b.W1);  // W1 := the operand actually applied
   run_trinv<T>(st, b.W1, n, b.R1h);
   // ---- P2
+  if (p.wide && !b.Zw) {  // FP64 storage: Z lives in U until P3 overwrites it
+    b.Zw = static_cast<double*>(out.U);
+    b.ldz = p.cplx ? 2 * out.ldu : out.ldu;
+  }
   PC_CHECK(ctx, launch_streams<T>(ctx, p, A, lda, center, mean, var, part, b, 2));
   reduce_gram<T>(st, p, part, p.mixed ? p.chunks3 : p.chunks, b.G2);
-  run_pchol<T>(st, b.G2, n, p.mixed ? kTolMx : kTolF64, b.Awk, b.R2, b.perm, b.rank, b.status);
+  run_pchol<T>(st, b.G2, n, p.mixed ? kTolMx : kTolF64, b.Awk, b.R2, b.perm, b.rank, b.status, 1, 0, b.rank + 1);
   // M = R2 (rows < rank) * R1h[perm, :], stored column-major into Hc
   dim3 gg(unsigned(ceil_div(n, 32)), unsigned(ceil_div(n, 32)));
   hipLaunchKernelGGL((k_gemm_small<T>), gg, dim3(256), 0, st, 0, 0, n, n, n, T(1.0), b.R2, n, (const int*)nullptr,
@@ -946,7 +1062,7 @@ static int core_t(pc_ctx* ctx, const Plan& p, const void* A, int64_t lda, int ce
     hipLaunchKernelGGL((k_copy2d<T>), dim3(nblk(nn)), dim3(256), 0, st, b.Hc, n, n, n, b.tmp, n, 1);
     hipLaunchKernelGGL((k_gemm_small<T>), gg, dim3(256), 0, st, 2, 0, n, n, n, T(1.0), b.tmp, n, (const int*)nullptr,
                        b.tmp, n, (const int*)nullptr, T(0.0), b.G1, n, 0);
-    run_pchol<T>(st, b.G1, n, 1e-30, b.Awk, b.R1, b.perm3, b.rank3, b.status);
+    run_pchol<T>(st, b.G1, n, 1e-30, b.Awk, b.R1, b.perm3, b.rank3, b.status, 1, 0, b.rank3 + 1);
     hipLaunchKernelGGL((k_copy2d<T>), dim3(nblk(nn)), dim3(256), 0, st, b.R1, n, n, n, b.Awk, n, 2);
     run_house_qr_sq<T>(st, b.Awk, n, b.UM, b.G1, nwork);
     hipLaunchKernelGGL((k_permute_rows<T>), dim3(nblk(nn)), dim3(256), 0, st, b.UM, n, n, (const int*)b.perm3, b.Awk,
@@ -984,7 +1100,7 @@ static int core_t(pc_ctx* ctx, const Plan& p, const void* A, int64_t lda, int ce
   // null completion (kernels return at once when nnull == 0)
   int ncand = int(std::min<int64_t>(p.m, 4 * int64_t(n) + 16));
   launch_null_lev<T>(st, p, A, lda, (p.mixed && center == 1) ? 3 : center, mean, var, b, ncand);
-  hipLaunchKernelGGL((k_null_build<T>), dim3(1), dim3(1024), 0, st, p.m, n, b.nnull, ncand, b.lev, b.urows, b.Wf, n,
+  hipLaunchKernelGGL((k_null_build<T>), dim3(1), dim3(1024), size_t(n) * sizeof(int), st, p.m, n, b.nnull, ncand, b.lev, b.urows, b.Wf, n,
                      b.jsel, b.cnull, nwork, b.status);
   if (mode == CORE_QR) {
     // LAPACK's signs (numpy.linalg.qr): D from the first n rows of Q, then
@@ -1104,7 +1220,7 @@ static int core_batch_t(pc_ctx* ctx, const Plan& p, const pc_problem* pr, int64_
                      b.tmp, n, (const int*)nullptr, T(0.0), b.Wf, n, 0, zb);
   const int ncand = int(std::min<int64_t>(p.m, 4 * int64_t(n) + 16));
   launch_null_lev<T>(st, p, pr[0].A, lda, (p.mixed && center == 1) ? 3 : center, mean, var, b, ncand, nz, zb, d.zA);
-  hipLaunchKernelGGL((k_null_build<T>), dim3(1, 1, z), dim3(1024), 0, st, p.m, n, b.nnull, ncand, b.lev, b.urows,
+  hipLaunchKernelGGL((k_null_build<T>), dim3(1, 1, z), dim3(1024), size_t(n) * sizeof(int), st, p.m, n, b.nnull, ncand, b.lev, b.urows,
                      b.Wf, n, b.jsel, b.cnull, nwork, b.status, zb);
   to_operand<T>(st, p, b.Wf, n, b, true, nz, zb);
   // ---- P3 (and the null-space fix of each U)
@@ -1139,7 +1255,9 @@ static int check_common(pc_ctx* ctx, pc_dtype dt, int64_t m, int64_t n) {
   if (dt < PC_F32 || dt > PC_C128) return pc_fail(ctx, PC_ERR_INVALID, "bad dtype %d", int(dt));
   if (n <= 0 || m <= 0) return pc_fail(ctx, PC_ERR_SHAPE, "empty matrix (m=%lld, n=%lld)", (long long)m, (long long)n);
   if (m < n) return pc_fail(ctx, PC_ERR_SHAPE, "tall-skinny solve needs m >= n (m=%lld, n=%lld)", (long long)m, (long long)n);
-  if (n > 512) return pc_fail(ctx, PC_ERR_SHAPE, "n = %lld > 512 is not supported", (long long)n);
+  if (n > (dtype_complex(dt) ? kWideMaxReal / 2 : kWideMaxReal))
+    return pc_fail(ctx, PC_ERR_SHAPE, "n = %lld > %d is not supported", (long long)n,
+                   dtype_complex(dt) ? kWideMaxReal / 2 : kWideMaxReal);
   PC_HIP(ctx, hipSetDevice(ctx->device));
   return PC_OK;
 }

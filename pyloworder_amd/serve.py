@@ -80,11 +80,27 @@ class DeviceExecutor:
     def _esz(X) -> int:
         return X.element_size()
 
+    # the mixed path takes n <= 256 (float32); wider POD requests run FP64
+    MIXED_MAX_N = 256
+
+    def precision_for(self, X) -> str:
+        """Arithmetic of a POD solve of X: the executor's, except that mixed
+        requests wider than the mixed kernels take FP64 (same results
+        contract, FP64 MFMA)."""
+        if self.precision == "mixed" and X.shape[1] > self.MIXED_MAX_N:
+            return "fp64"
+        return self.precision
+
     def workspace_bound(self, m_max: int, n_max: int, dtype=None) -> int:
-        """Workspace one slot's context may hold: that of the largest solve."""
-        from ._lib import PC_F32
+        """Workspace one slot's context may hold: that of the largest solve of
+        either precision a request may take."""
+        from ._lib import PC_F32, PC_PREC_FP64
         lib = self._lib.load()
-        return int(max(lib.pc_workspace_query(0, PC_F32 if dtype is None else dtype, m_max, n_max, 0, self._prec), 0))
+        dt = PC_F32 if dtype is None else dtype
+        need = [lib.pc_workspace_query(0, dt, m_max, min(n_max, self.MIXED_MAX_N), 0, self._prec)]
+        if n_max > self.MIXED_MAX_N or self._prec == PC_PREC_FP64:
+            need.append(lib.pc_workspace_query(0, dt, m_max, n_max, 0, PC_PREC_FP64))
+        return int(max(max(need), 0))
 
     def bytes_needed(self, req: Request, basis: Optional[Basis]) -> int:
         """Device bytes of the request's outputs (the slot's workspace is
@@ -110,7 +126,7 @@ class DeviceExecutor:
     def run(self, req: Request, basis: Optional[Basis]):
         from . import POD as P
         if req.kind == POD:
-            return P.run(req.X, remove_mean=self.remove_mean, precision=self.precision)
+            return P.run(req.X, remove_mean=self.remove_mean, precision=self.precision_for(req.X))
         k = req.modes or basis.U.shape[1]
         # projections and reconstructions run FP64 MFMA on the stored data
         if req.kind == PROJECT:
