@@ -135,6 +135,57 @@ __global__ __launch_bounds__(WNT) void k_wgemm_f64(const TS* __restrict__ X, int
   }
 }
 
+// P[chunk] (PBp x QBp) += A^T B over the chunk's rows, A m x p, B m x q: one
+// (row chunk, stripe I of A, stripe J of B) per workgroup
+template <class TA, class TB>
+__global__ __launch_bounds__(WNT) void k_wxgram_f64(const TA* __restrict__ A, int64_t lda, int p,
+                                                    const TB* __restrict__ B, int64_t ldb, int q, int64_t m, int kq,
+                                                    int64_t tpc, double* __restrict__ part, int PBp, int QBp) {
+  __shared__ double S[WTR][2 * WS + 2];
+  const int I = blockIdx.y / kq, J = blockIdx.y % kq;
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63, kr = l >> 4, cc = l & 15;
+  d4 acc[8];
+#pragma unroll
+  for (int s = 0; s < 8; s++) acc[s] = d4{0, 0, 0, 0};
+  const int64_t ntiles = ceil_div(m, WTR);
+  const int64_t t0 = int64_t(blockIdx.x) * tpc;
+  const int64_t t1 = t0 + tpc < ntiles ? t0 + tpc : ntiles;
+  for (int64_t t = t0; t < t1; t++) {
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < WTR * 2 * WS; idx += WNT) {
+      const int r = idx / (2 * WS), c = idx % (2 * WS);
+      const int64_t row = t * WTR + r;
+      double v = 0.0;
+      if (c < WS) {
+        const int col = I * WS + c;
+        if (row < m && col < p) v = double(A[row * lda + col]);
+      } else {
+        const int col = J * WS + (c - WS);
+        if (row < m && col < q) v = double(B[row * ldb + col]);
+      }
+      S[r][c] = v;
+    }
+    __syncthreads();
+#pragma unroll 2
+    for (int kk = 0; kk < WTR / 4; kk++) {
+      const double* srow = S[kk * 4 + kr];
+#pragma unroll
+      for (int s = 0; s < 8; s++) {
+        const int b = w * 8 + s, bi = b >> 3, bj = b & 7;
+        acc[s] = mfma64(srow[bi * 16 + cc], srow[WS + bj * 16 + cc], acc[s]);
+      }
+    }
+  }
+  double* P = part + int64_t(blockIdx.x) * PBp * QBp;
+#pragma unroll
+  for (int s = 0; s < 8; s++) {
+    const int b = w * 8 + s, bi = b >> 3, bj = b & 7;
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+      P[int64_t(I * WS + bi * 16 + 4 * r + kr) * QBp + J * WS + bj * 16 + cc] = acc[s][r];
+  }
+}
+
 }  // namespace
 
 int wide_stripe() { return WS; }
@@ -170,6 +221,24 @@ int launch_wgemm_f64(hipStream_t st, pc_dtype dt, const void* X, int64_t ldx, in
   else if (dt == PC_F32)
     hipLaunchKernelGGL((k_wgemm_f64<float, float>), g, dim3(WNT), 0, st, (const float*)X, ldx, m, K, center, mean,
                        var, W, ldw, N, tpc, (float*)C, ldc);
+  else
+    return -1;
+  return 0;
+}
+
+// partial slabs (chunks x PBp x QBp) of A^T B, A m x p and B m x q of one real
+// storage type; PBp, QBp multiples of the stripe width
+int launch_wxgram_f64(hipStream_t st, pc_dtype dt, const void* A, int64_t lda, int p, const void* B, int64_t ldb,
+                      int q, int64_t m, int chunks, int64_t tpc, double* part, int PBp, int QBp) {
+  const int kp = int(ceil_div(p, WS)), kq = int(ceil_div(q, WS));
+  if (kp * WS > PBp || kq * WS > QBp || int64_t(kp) * kq > 65535) return -1;
+  const dim3 g(unsigned(chunks), unsigned(kp * kq));
+  if (dt == PC_F64)
+    hipLaunchKernelGGL((k_wxgram_f64<double, double>), g, dim3(WNT), 0, st, (const double*)A, lda, p,
+                       (const double*)B, ldb, q, m, kq, tpc, part, PBp, QBp);
+  else if (dt == PC_F32)
+    hipLaunchKernelGGL((k_wxgram_f64<float, float>), g, dim3(WNT), 0, st, (const float*)A, lda, p, (const float*)B,
+                       ldb, q, m, kq, tpc, part, PBp, QBp);
   else
     return -1;
   return 0;

@@ -54,3 +54,5 @@ int launch_wgram_f64(hipStream_t st, pc_dtype dt, const void* X, int64_t ldx, in
 int launch_wgemm_f64(hipStream_t st, pc_dtype dt, const void* X, int64_t ldx, int64_t m, int K, int center,
                      const double* mean, const double* var, const double* W, int64_t ldw, int N, int chunks,
                      int64_t tpc, void* C, int64_t ldc, int out_f64);
+int launch_wxgram_f64(hipStream_t st, pc_dtype dt, const void* A, int64_t lda, int p, const void* B, int64_t ldb,
+                      int q, int64_t m, int chunks, int64_t tpc, double* part, int PBp, int QBp);

@@ -97,3 +97,46 @@ def test_wide_mixed_is_refused_with_a_reason():
         pl.math.tsqr_svd(X, precision="mixed")
     with pytest.raises(pl.PodCoreError):
         pl.math.tsqr_svd(np.ones((3000, 2049)))
+
+
+def test_wide_randomized_svd_fp64_vs_oracle():
+    """randomized_svd (pyLOM/vmmath/svd.py:254-273) at n = 512, r = 200, q = 2
+    against the oracle with the same seeded Omega: gated singular values 1e-9,
+    their left subspace 1e-7 rad."""
+    X = workloads.lowrank(30000, 512, seed=12, decay=0.97, top=1e3, noise=1e-6)
+    U, S, V = pl.math.randomized_svd(X, 200, 2, seed=7)
+    Uo, So, Vo = oracle.randomized_svd([X], 200, 2, seed=7)
+    e = metrics.sigma_errors(_np(S), So)
+    assert e["maxrel_gated"] <= 1e-9, e
+    assert metrics.max_principal_angle(_np(U), np.concatenate(Uo), min(e["n_gated"], 150)) <= 1e-7
+    Q, B = pl.math.randomized_qr(X, 200, 2, seed=7)
+    Qo, Bo = oracle.randomized_qr([X], 200, 2, seed=7)
+    assert np.abs(_np(Q) - Qo[0]).max() <= 1e-8 and np.abs(_np(B) - Bo).max() <= 1e-8 * np.abs(Bo).max()
+
+
+def test_mixed_randomized_wide_sketch():
+    """mixed randomized_svd with r = 192 > 128 (the factor's FP64 randomized
+    solve takes any r <= n)."""
+    X = workloads.lowrank(60000, 256, seed=13, decay=0.96, top=1e3, noise=1e-6 * np.sqrt(8e6 / 60000),
+                          dtype=np.float32)
+    U, S, V = pl.math.randomized_svd(X, 192, 1, seed=3, precision="mixed")
+    Uo, So, Vo = oracle.randomized_svd([X.astype(np.float64)], 192, 1, seed=3)
+    e = metrics.sigma_errors(_np(S), So)
+    assert e["maxrel_gated"] <= 1e-5, e
+
+
+def test_wide_products():
+    rng = np.random.default_rng(14)
+    U = np.linalg.qr(rng.standard_normal((20000, 300)))[0]
+    S = np.linspace(3.0, 1.0, 300)
+    V = rng.standard_normal((300, 700))
+    Xr = pl.POD.reconstruct(U, S, V)
+    ref = (U * S) @ V
+    assert np.abs(_np(Xr) - ref).max() <= 1e-12 * np.abs(ref).max()
+    Cm = pl.POD.project(U, ref)
+    assert np.abs(_np(Cm) - S[:, None] * V).max() <= 1e-11 * np.abs(V).max() * 3
+    A = rng.standard_normal((5000, 600))
+    B = rng.standard_normal((600, 400))
+    assert np.abs(_np(pl.math.matmul(A, B)) - A @ B).max() <= 1e-11 * np.abs(A @ B).max()
+    AtB = pl.math.matmul(torch.from_numpy(A).cuda().T, torch.from_numpy(rng.standard_normal((5000, 300))).cuda())
+    assert tuple(AtB.shape) == (600, 300)
