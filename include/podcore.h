@@ -230,6 +230,37 @@ int pc_spod_blocks(pc_ctx* ctx, const float* X, int64_t ldx, int64_t M, int64_t 
 int pc_spod_modes(pc_ctx* ctx, const void* U, int64_t M, int64_t nBlks, int64_t nf, const int64_t* order, void* P,
                   int out_f64);
 
+/* ---- DMD ---------------------------------------------------------------------
+ * eigen  pyLOM/vmmath/maths.py:161-183 (numpy.linalg.eig = LAPACK ?geev):
+ *   eigenvalues wr + i wi and right eigenvectors V (n x n complex128, ldv) of
+ *   the real n x n matrix A (FP64, row-major).  Pairs come with the positive
+ *   imaginary part first; vectors have unit 2-norm, a pair's largest
+ *   component is real (?geev's normalisation).  The order of the eigenvalues
+ *   follows the real Schur form this solver reaches, which can differ from
+ *   LAPACK's.  Synchronous (status read).                                   */
+int pc_eig(pc_ctx* ctx, const double* A, int64_t lda, int64_t n, double* wr, double* wi, void* V, int64_t ldv);
+/* The projected operator of DMD.run (pyLOM/DMD/wrapper.py:85-88):
+ *   Atilde = (U^T Y1) VT^T diag(1/S), U m x N, Y1 = Y[:, 1:] m x ns,
+ *   S N, VT N x ns (the truncated SVD of Y[:, :-1]); Atilde N x N, FP64.    */
+int pc_dmd_operator(pc_ctx* ctx, const double* U, int64_t ldu, const double* Y1, int64_t ldy, int64_t m, int64_t N,
+                    int64_t ns, const double* S, const double* VT, int64_t ldvt, double* Atilde, int64_t ldat);
+/* Modes, amplitudes and order of DMD.run (pyLOM/DMD/wrapper.py:90-117,
+ * _order_modes :18-46): from (wr, wi, W) = eig(Atilde) (W N x N complex128,
+ * ld N), the amplitudes b of Jovanovic et al. (2014) (P = (W^H W) o
+ * conj(Vand Vand^H), q = conj(diag(Vand G^H W)), G = diag(S) VT, b = P^-1 q),
+ * then everything ordered by decreasing |b| with each conjugate pair
+ * positive-imaginary first: mu_re / mu_im (N), b (N complex128) and the modes
+ * Phi = Y1 VT^T diag(1/S) W / mu (m x N complex128, ldphi), one pass over
+ * Y1.  Synchronous (status read).                                         */
+int pc_dmd_modes(pc_ctx* ctx, const double* Y1, int64_t ldy, int64_t m, int64_t ns, int64_t N, const double* S,
+                 const double* VT, int64_t ldvt, const double* wr, const double* wi, const void* W, double* mu_re,
+                 double* mu_im, void* Phi, int64_t ldphi, void* b);
+/* reconstruction_jovanovic  pyLOM/DMD/wrapper.py:141-147:
+ *   X = Re(Phi diag(b) Vand(t)), Vand[k][j] = mu_k^{t_j} (t: nt times on the
+ *   device), X m x nt FP64 (ldx).                                          */
+int pc_dmd_reconstruct(pc_ctx* ctx, const void* Phi, int64_t ldphi, int64_t m, int64_t N, const double* mu_re,
+                       const double* mu_im, const void* b, const double* t, int64_t nt, double* X, int64_t ldx);
+
 #ifdef __cplusplus
 }
 #endif

@@ -20,6 +20,8 @@ __all__ = [
     "transpose", "matmul", "matmulp", "vecmat", "compute_truncation_residual",
     "energy", "RMSE", "pod_run", "pod_truncate", "pod_reconstruct",
     "pod_project", "hammwin", "spod_blocks", "spod_run", "init_qr_streaming", "update_qr_streaming",
+    "eigen", "vandermonde", "vandermonde_time", "dmd_order_modes", "dmd_run", "dmd_frequency_damping",
+    "dmd_reconstruction_jovanovic",
 ]
 
 
@@ -429,3 +431,92 @@ def spod_run(X, t, nDFT=0, nolap=0, remove_mean=True):
         L[ifreq, :] = np.abs(S * S)
     order = np.argsort(L[:, 0])[::-1]
     return L[order, :], P[:, order], f[order]
+
+
+# ---------------------------------------------------------------------------
+# DMD  (pyLOM/DMD/wrapper.py, the helpers of pyLOM/vmmath/maths.py)
+# ---------------------------------------------------------------------------
+def eigen(A):
+    """(real, imag, vecs) of A — pyLOM/vmmath/maths.py:161-183 (numpy.linalg.eig)."""
+    w, vecs = np.linalg.eig(A)
+    return np.real(w).copy(), np.imag(w).copy(), vecs
+
+
+def vandermonde(real, imag, m: int, n: int):
+    """Vand[:, j] = (real + i imag)**j, j < n — pyLOM/vmmath/maths.py:203-222."""
+    lam = real + 1j * imag
+    V = np.zeros((m, n), dtype=np.complex128)
+    for icol in range(n):
+        V[:, icol] = lam ** icol
+    return V
+
+
+def vandermonde_time(real, imag, m: int, time):
+    """Vand[:, j] = (real + i imag)**t_j — pyLOM/vmmath/maths.py:225-246."""
+    lam = real + 1j * imag
+    V = np.zeros((m, len(time)), dtype=np.complex128)
+    for it, t in enumerate(time):
+        V[:, it] = lam ** t
+    return V
+
+
+def dmd_order_modes(muReal, muImag, Phi, bJov):
+    """pyLOM/DMD/wrapper.py:18-46: modes by decreasing |b|, then inside each
+    conjugate pair the positive imaginary part first (imaginary parts of the
+    pair's entries exchanged as the reference does)."""
+    order = np.flip(np.abs(bJov).argsort(kind="stable"))
+    muReal, muImag, Phi, bJov = muReal[order].copy(), muImag[order].copy(), Phi[:, order].copy(), bJov[order].copy()
+    skip = False
+    for ii in range(muImag.shape[0]):
+        if skip:
+            skip = False
+            continue
+        if muImag[ii] < 0:
+            muImag[ii] = muImag[ii + 1]
+            muImag[ii + 1] = -muImag[ii]
+            bJov.imag[ii] = bJov.imag[ii + 1]
+            bJov.imag[ii + 1] = -bJov.imag[ii]
+            Phi.imag[:, ii] = Phi.imag[:, ii + 1]
+            Phi.imag[:, ii + 1] = -Phi.imag[:, ii + 1]
+            skip = True
+            continue
+        if muImag[ii] > 0:
+            skip = True
+    return muReal, muImag, Phi, bJov
+
+
+def dmd_run(X, r, remove_mean: bool = True):
+    """pyLOM/DMD/wrapper.py:49-117 on one rank: SVD of the first n-1
+    snapshots, truncation, the projected operator
+    Atilde = U^T Y[:, 1:] VT^T diag(1/S), its eigen-decomposition, the modes
+    Phi = Y[:, 1:] VT^T diag(1/S) w / mu and the amplitudes of Jovanovic et
+    al. (2014), ordered by amplitude.  Returns (muReal, muImag, Phi, bJov)."""
+    X = np.asarray(X, dtype=np.float64)
+    Y = subtract_mean(X, temporal_mean(X)) if remove_mean else X.copy()
+    U, S, VT = np.linalg.svd(Y[:, :-1], full_matrices=False)
+    N = int(r) if r >= 1 else compute_truncation_residual(S, r)
+    U, S, VT = U[:, :N], S[:N], VT[:N, :]
+    aux1 = U.T @ Y[:, 1:]
+    aux2 = ((1.0 / S)[:, None] * VT).T
+    Atilde = aux1 @ aux2
+    muReal, muImag, w = eigen(Atilde)
+    Phi = (Y[:, 1:] @ VT.T @ np.diag(1.0 / S) @ w) / (muReal + muImag * 1j)
+    Vand = vandermonde(muReal, muImag, muReal.shape[0], Y.shape[1] - 1)
+    P = (w.conj().T @ w) * np.conj(Vand @ Vand.conj().T)
+    Pl = np.linalg.cholesky(P)
+    G = np.diag(S) @ VT
+    q = np.conj(np.diag(Vand @ G.conj().T @ w))
+    bJov = np.linalg.inv(Pl.conj().T) @ (np.linalg.inv(Pl) @ q)
+    return dmd_order_modes(muReal, muImag, Phi, bJov)
+
+
+def dmd_frequency_damping(real, imag, dt):
+    """pyLOM/DMD/wrapper.py:120-130: delta = log|mu| / dt, omega = arg(mu) / dt."""
+    mod = np.sqrt(real * real + imag * imag)
+    return np.log(mod) / dt, np.arctan2(imag, real) / dt
+
+
+def dmd_reconstruction_jovanovic(Phi, real, imag, t, bJov):
+    """pyLOM/DMD/wrapper.py:141-147: Re(Phi diag(b) Vand(t))."""
+    V = vandermonde_time(real, imag, real.shape[0], t)
+    return (Phi @ (np.diag(bJov) @ V)).real

@@ -1286,6 +1286,7 @@ extern "C" int pc_tsqr(pc_ctx* ctx, pc_dtype dt, const void* A, int64_t lda, int
 }
 
 #include "ops.inc"
+#include "dmd.inc"
 
 // ---------------------------------------------------------------------------
 // Allocation-free tall-skinny SVD: the workspace of pc_tsqr_svd taken from a

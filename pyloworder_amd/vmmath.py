@@ -26,7 +26,7 @@ from .cr import cr
 __all__ = [
     "qr", "svd", "tsqr", "tsqr_svd", "tsqr_svd_grouped", "GroupedGraph", "randomized_qr", "randomized_svd", "init_qr_streaming", "update_qr_streaming", "matmul", "matmulp",
     "vecmat", "transpose", "temporal_mean", "subtract_mean", "temporal_variance", "norm_variance",
-    "compute_truncation_residual", "RMSE", "energy", "reference_omega",
+    "compute_truncation_residual", "RMSE", "energy", "reference_omega", "eigen",
 ]
 
 _PREC = {"fp64": _lib.PC_PREC_FP64, "mixed": _lib.PC_PREC_MIXED}
@@ -530,3 +530,31 @@ def energy(original, rec) -> float:
     """1 - sum (o - r)^2 / sum o^2 — pyLOM/vmmath/truncation.py:41-60."""
     s1, s2, _ = _diff_norms(original, rec)
     return float(1.0 - s1 / s2)
+
+
+# ---------------------------------------------------------------------------
+# eigen  (pyLOM/vmmath/maths.py:161-183)
+# ---------------------------------------------------------------------------
+@cr("math.eigen")
+def eigen(A):
+    """(real, imag, vecs) of a real square matrix — pyLOM/vmmath/maths.py:161-183.
+
+    Computed on the device (Hessenberg reduction, Francis double-shift QR,
+    back substitution) with numpy.linalg.eig's conventions: conjugate pairs
+    positive imaginary part first, unit-norm vectors, a pair's largest
+    component real.  ``vecs`` is complex (real when every eigenvalue is real,
+    as numpy returns it).  The eigenvalue order follows this solver's Schur
+    form and can differ from LAPACK's.
+    """
+    a = Arg(A, "A", dtype=torch.float64)
+    n, n2 = a.t.shape
+    if n != n2:
+        raise ValueError("eigen: A must be square")
+    wr = empty((n,), torch.float64, a.t)
+    wi = empty((n,), torch.float64, a.t)
+    V = empty((n, n), torch.complex128, a.t)
+    ctx = _ctx()
+    ctx.check(ctx.lib.pc_eig(ctx.h, a.ptr(), a.ld, n, wr.data_ptr(), wi.data_ptr(), V.data_ptr(), V.stride(0)))
+    if not bool((wi != 0).any()):
+        V = V.real.contiguous()
+    return a.give(wr), a.give(wi), a.give(V)

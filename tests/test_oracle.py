@@ -277,3 +277,54 @@ def test_update_qr_streaming_sign_convention_changes_the_sketch():
         _, _, Yo = oracle.update_qr_streaming([A2], Q, B, [Y[0].copy()], 4, 1, omega=om2, positive=positive)
         outs.append(Yo[0])
     assert not np.allclose(outs[0], outs[1])
+
+
+def _dynamics(m=400, n=60, seed=3):
+    """Snapshots of exact linear dynamics: two oscillating pairs and one
+    decaying real mode, X[:, t] = sum_k Re(a_k phi_k mu_k^t) (rank 5)."""
+    rng = np.random.default_rng(seed)
+    mus = [0.98 * np.exp(0.3j), 0.9 * np.exp(1.1j), 0.7]
+    amps = [3.0, 1.5, 0.8]
+    X = np.zeros((m, n))
+    t = np.arange(n)
+    for mu, a in zip(mus, amps):
+        phi = rng.standard_normal(m) + (1j * rng.standard_normal(m) if np.iscomplex(mu) else 0)
+        X += np.real(a * np.outer(phi, mu ** t))
+    return X, mus
+
+
+def test_dmd_recovers_known_eigenvalues_and_reconstructs():
+    """Exact rank-5 dynamics: DMD's eigenvalues are the generating ones
+    (conjugate pairs included), and Re(Phi diag(b) Vand) reproduces every
+    snapshot used (Jovanovic amplitudes minimise that residual, here 0)."""
+    X, mus = _dynamics()
+    muR, muI, Phi, b = oracle.dmd_run(X, 5, remove_mean=False)
+    got = np.sort_complex(muR + 1j * muI)
+    want = np.sort_complex(np.array([mus[0], np.conj(mus[0]), mus[1], np.conj(mus[1]), mus[2]]))
+    assert np.abs(got - want).max() < 1e-10
+    Xr = oracle.dmd_reconstruction_jovanovic(Phi, muR, muI, np.arange(X.shape[1] - 1), b)
+    assert np.abs(Xr - X[:, :-1]).max() < 1e-9 * np.abs(X).max()
+    # ordering: |b| descending, each conjugate pair positive imaginary part first
+    assert np.all(np.diff(np.abs(b)) <= 1e-9 * np.abs(b).max())
+    for i in range(len(muI) - 1):
+        if muI[i] != 0 and abs(muR[i] - muR[i + 1]) < 1e-12:
+            assert muI[i] > 0 and abs(muI[i] + muI[i + 1]) < 1e-12
+
+
+def test_dmd_order_modes_hand_case():
+    """pyLOM/DMD/wrapper.py:18-46 on a hand case: |b| = 1, 5, 5, 3 with the
+    pair listed negative-imaginary first."""
+    muR = np.array([0.1, 0.5, 0.5, 0.9])
+    muI = np.array([0.0, -0.2, 0.2, 0.0])
+    b = np.array([1.0, 3 - 4j, 3 + 4j, 3.0])
+    Phi = np.array([[1.0, 1 - 1j, 1 + 1j, 2.0]])
+    r, i, P, bb = oracle.dmd_order_modes(muR, muI, Phi, b)
+    assert np.allclose(r, [0.5, 0.5, 0.9, 0.1]) and np.allclose(i, [0.2, -0.2, 0.0, 0.0])
+    assert np.allclose(bb, [3 + 4j, 3 - 4j, 3.0, 1.0]) and np.allclose(P, [[1 + 1j, 1 - 1j, 2.0, 1.0]])
+
+
+def test_dmd_frequency_damping_closed_form():
+    delta, omega, dt = np.array([-0.1, 0.0, 0.05]), np.array([2.0, -1.0, 0.5]), 0.01
+    mu = np.exp((delta + 1j * omega) * dt)
+    d, w = oracle.dmd_frequency_damping(mu.real, mu.imag, dt)
+    assert np.allclose(d, delta, atol=1e-12) and np.allclose(w, omega, atol=1e-12)
