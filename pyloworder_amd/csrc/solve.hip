@@ -27,7 +27,7 @@ int pc_check_mat(pc_ctx* ctx, const void* p, int64_t ld, int64_t m, int64_t n, c
 
 namespace {
 
-constexpr double kShiftF64 = 1e-8;   // tau / trace(G1), FP64 path
+constexpr double kShiftF64 = 1e-13;  // tau / max(diag G1), FP64 path (G1 rounding ~ n u: 3e-14 at n = 256)
 constexpr double kShiftMx = 1e-6;    // tau / trace(G1), mixed path
 constexpr double kTolF64 = 1e-15;    // pivoted Cholesky rank tolerance (relative to max diag)
 constexpr double kTolMx = 1e-7;

@@ -303,3 +303,23 @@ def test_streaming_qr_recovers_low_rank_history(rng):
     Q2, B2, _ = pl.math.update_qr_streaming(A2, Q, B, Y, r, 0)
     X = np.hstack((A1, A2))
     assert np.abs(Q2 @ B2 - X).max() <= 1e-9 * np.abs(X).max()
+
+
+@pytest.mark.parametrize("logk", [8, 10, 12, 14])
+def test_conditioning_matches_lapack(logk):
+    """Graded spectra s = logspace(0, -logk) on 20000 x 64: the FP64 path's
+    singular values against the exact ones, held to what LAPACK reaches on
+    the same matrix (numpy.linalg.svd, ~u kappa relative on sigma_min): at
+    most 10 times LAPACK's largest relative error (and 1e-13 at least).
+    With graded columns (A = U diag(s)) both resolve every sigma to ~1e-15."""
+    rng = np.random.default_rng(5)
+    m, n = 20000, 64
+    Uo = np.linalg.qr(rng.standard_normal((m, n)))[0]
+    Vo = np.linalg.qr(rng.standard_normal((n, n)))[0]
+    s = np.logspace(0, -logk, n)
+    A = (Uo * s) @ Vo.T
+    S = _np(pl.math.tsqr_svd(A)[1])
+    lap = np.max(np.abs(np.linalg.svd(A, compute_uv=False) - s) / s)
+    assert np.max(np.abs(S - s) / s) <= max(10 * lap, 1e-13), (np.max(np.abs(S - s) / s), lap)
+    S2 = _np(pl.math.tsqr_svd(Uo * s)[1])
+    assert np.max(np.abs(S2 - s) / s) <= 1e-14
