@@ -230,6 +230,16 @@ int pc_spod_blocks(pc_ctx* ctx, const float* X, int64_t ldx, int64_t M, int64_t 
 int pc_spod_modes(pc_ctx* ctx, const void* U, int64_t M, int64_t nBlks, int64_t nf, const int64_t* order, void* P,
                   int out_f64);
 
+/* ---- data assembly ---------------------------------------------------------
+ * Dataset.X  pyLOM/dataset.py:252-292: the snapshot matrix of the given
+ * fields, rows interleaved per point: row p * nc + c (nc = sum of ndim) is
+ * component idim of field f for point p, i.e. row idim + ndim[f] * p of
+ * field f (src[f]: ndim[f] * npoints rows of nt values, leading dimension
+ * ld[f]).  X: npoints * nc rows of nt values (ldx).  Host arrays of device
+ * pointers; at most 32 fields; float32 or float64.                          */
+int pc_dataset_x(pc_ctx* ctx, pc_dtype dtype, int64_t nfields, const void* const* src, const int64_t* ndim,
+                 const int64_t* ld, int64_t npoints, int64_t nt, void* X, int64_t ldx);
+
 /* ---- DMD ---------------------------------------------------------------------
  * eigen  pyLOM/vmmath/maths.py:161-183 (numpy.linalg.eig = LAPACK ?geev):
  *   eigenvalues wr + i wi and right eigenvectors V (n x n complex128, ldv) of

@@ -21,7 +21,7 @@ __all__ = [
     "energy", "RMSE", "pod_run", "pod_truncate", "pod_reconstruct",
     "pod_project", "hammwin", "spod_blocks", "spod_run", "init_qr_streaming", "update_qr_streaming",
     "eigen", "vandermonde", "vandermonde_time", "dmd_order_modes", "dmd_run", "dmd_frequency_damping",
-    "dmd_reconstruction_jovanovic",
+    "dmd_reconstruction_jovanovic", "dataset_x",
 ]
 
 
@@ -520,3 +520,20 @@ def dmd_reconstruction_jovanovic(Phi, real, imag, t, bJov):
     """pyLOM/DMD/wrapper.py:141-147: Re(Phi diag(b) Vand(t))."""
     V = vandermonde_time(real, imag, real.shape[0], t)
     return (Phi @ (np.diag(bJov) @ V)).real
+
+
+def dataset_x(fields: dict, npoints: int, names=None):
+    """pyLOM/dataset.py:252-292 for one temporal dimension: fields[name] =
+    (ndim, value (ndim * npoints, ...)); rows of X interleaved per point."""
+    names = list(names) if names else list(fields)
+    nf = sum(fields[n][0] for n in names)
+    v0 = np.asarray(fields[names[0]][1])
+    X = np.zeros((nf * npoints,) + v0.shape[1:], dtype=np.float64)
+    ifield = 0
+    for name in names:
+        ndim, value = fields[name]
+        value = np.asarray(value)
+        for idim in range(ndim):
+            X[ifield:nf * npoints:nf] = value[idim:ndim * npoints:ndim]
+            ifield += 1
+    return X

@@ -11,8 +11,9 @@ from . import vmmath as math
 from . import POD
 from . import SPOD
 from . import DMD
+from .dataset import Dataset
 from .cr import cr_info, cr_start, cr_stop
 
 __version__ = "0.1.0"
 
-__all__ = ["math", "vmmath", "POD", "SPOD", "DMD", "LinAlgError", "PodCoreError", "cr_info", "cr_start", "cr_stop"]
+__all__ = ["math", "vmmath", "POD", "SPOD", "DMD", "Dataset", "LinAlgError", "PodCoreError", "cr_info", "cr_start", "cr_stop"]

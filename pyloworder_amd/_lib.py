@@ -59,6 +59,7 @@ SIGNATURES = {
     "pc_grouped_graph_launch": (ip, [vp, vp]),
     "pc_graph_destroy": (ip, [vp]),
     "pc_eig": (ip, [vp, vp, i64, i64, vp, vp, vp, i64]),
+    "pc_dataset_x": (ip, [vp, ip, i64, C.POINTER(vp), C.POINTER(i64), C.POINTER(i64), i64, i64, vp, i64]),
     "pc_dmd_operator": (ip, [vp, vp, i64, vp, i64, i64, i64, i64, vp, vp, i64, vp, i64]),
     "pc_dmd_modes": (ip, [vp, vp, i64, i64, i64, i64, vp, vp, i64, vp, vp, vp, vp, vp, vp, i64, vp]),
     "pc_dmd_reconstruct": (ip, [vp, vp, i64, i64, i64, vp, vp, vp, vp, i64, vp, i64]),

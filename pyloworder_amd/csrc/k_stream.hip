@@ -147,6 +147,31 @@ int check_mat(pc_ctx* ctx, const void* p, int64_t ld, int64_t m, int64_t n, cons
 }  // namespace
 
 // FP64 statistics of float32 rows (the variance-normalised mixed path)
+// Dataset.X row interleave (pyLOM/dataset.py:252-292): output row
+// r = p nf + c (point p, field component c) is row idim + ndim_f p of field f,
+// where component c is component idim of field f in the given field order.
+constexpr int kMaxFields = 32;
+struct FieldSet {
+  const void* src[kMaxFields];
+  int64_t ld[kMaxFields];
+  int field_of[kMaxFields * 8];  // component -> field
+  int idim_of[kMaxFields * 8];   // component -> its index inside the field
+  int ndim[kMaxFields];
+};
+template <class T>
+__global__ __launch_bounds__(256) void k_interleave(FieldSet fs, int nf, int64_t npoints, int64_t nt, T* X,
+                                                    int64_t ldx) {
+  const int64_t rows = npoints * nf;
+  for (int64_t r = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6); r < rows; r += int64_t(gridDim.x) * 4) {
+    const int64_t p = r / nf;
+    const int c = int(r - p * nf);
+    const int f = fs.field_of[c];
+    const T* s = static_cast<const T*>(fs.src[f]) + (int64_t(fs.idim_of[c]) + int64_t(fs.ndim[f]) * p) * fs.ld[f];
+    T* o = X + r * ldx;
+    for (int64_t j = threadIdx.x & 63; j < nt; j += 64) o[j] = s[j];
+  }
+}
+
 void launch_row_stats_f32(hipStream_t st, const float* X, int64_t ldx, int64_t m, int64_t n, double* mean,
                           double* var) {
   dim3 g = dim3(unsigned(ceil_div(m, 4)));
@@ -293,5 +318,38 @@ extern "C" int pc_diff_norms(pc_ctx* ctx, pc_dtype dt, const void* A, int64_t ld
   PC_HIP(ctx, hipMemcpyAsync(ctx->hflag, res, 16, hipMemcpyDeviceToHost, ctx->stream));
   PC_HIP(ctx, hipStreamSynchronize(ctx->stream));
   memcpy(out, ctx->hflag, 16);
+  return PC_OK;
+}
+
+extern "C" int pc_dataset_x(pc_ctx* ctx, pc_dtype dt, int64_t nfields, const void* const* src, const int64_t* ndim,
+                            const int64_t* ld, int64_t npoints, int64_t nt, void* X, int64_t ldx) {
+  if (!ctx) return pc_fail(nullptr, PC_ERR_INVALID, "pc_dataset_x: ctx is NULL");
+  if (!src || !ndim || !ld || !X) return pc_fail(ctx, PC_ERR_INVALID, "pc_dataset_x: NULL argument");
+  if (nfields <= 0 || nfields > kMaxFields) return pc_fail(ctx, PC_ERR_SHAPE, "pc_dataset_x: 1..%d fields", kMaxFields);
+  if (dt != PC_F32 && dt != PC_F64) return pc_fail(ctx, PC_ERR_UNSUPPORTED, "pc_dataset_x: real fields only");
+  FieldSet fs;
+  int nc = 0;
+  for (int64_t f = 0; f < nfields; f++) {
+    if (!src[f] || ndim[f] < 1 || nc + ndim[f] > kMaxFields * 8 || ld[f] < nt)
+      return pc_fail(ctx, PC_ERR_INVALID, "pc_dataset_x: field %lld: bad pointer, ndim or leading dimension",
+                     (long long)f);
+    fs.src[f] = src[f];
+    fs.ld[f] = ld[f];
+    fs.ndim[f] = int(ndim[f]);
+    for (int d = 0; d < ndim[f]; d++) {
+      fs.field_of[nc] = int(f);
+      fs.idim_of[nc] = d;
+      nc++;
+    }
+  }
+  PC_CHECK(ctx, check_mat(ctx, X, ldx, npoints * nc, nt, "pc_dataset_x X"));
+  if (npoints == 0 || nt == 0) return PC_OK;
+  PC_HIP(ctx, hipSetDevice(ctx->device));
+  const dim3 g(unsigned(std::min<int64_t>(ceil_div(npoints * nc, 4), 1 << 20)));
+  if (dt == PC_F64)
+    hipLaunchKernelGGL(k_interleave<double>, g, dim3(256), 0, ctx->stream, fs, nc, npoints, nt, (double*)X, ldx);
+  else
+    hipLaunchKernelGGL(k_interleave<float>, g, dim3(256), 0, ctx->stream, fs, nc, npoints, nt, (float*)X, ldx);
+  PC_HIP(ctx, hipGetLastError());
   return PC_OK;
 }

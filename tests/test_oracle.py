@@ -328,3 +328,14 @@ def test_dmd_frequency_damping_closed_form():
     mu = np.exp((delta + 1j * omega) * dt)
     d, w = oracle.dmd_frequency_damping(mu.real, mu.imag, dt)
     assert np.allclose(d, delta, atol=1e-12) and np.allclose(w, omega, atol=1e-12)
+
+
+def test_dataset_x_hand_case():
+    """pyLOM/dataset.py:252-292: two points, a scalar field P and a 2-component
+    field U (u, v per point), X rows = p0:(P, u, v), p1:(P, u, v)."""
+    P = np.array([[1.0, 2.0], [3.0, 4.0]])                       # points 0, 1
+    U = np.array([[10.0, 11.0], [20.0, 21.0], [30.0, 31.0], [40.0, 41.0]])  # (u0, v0, u1, v1)
+    X = oracle.dataset_x({"P": (1, P), "U": (2, U)}, 2)
+    assert np.array_equal(X, [[1, 2], [10, 11], [20, 21], [3, 4], [30, 31], [40, 41]])
+    X2 = oracle.dataset_x({"P": (1, P), "U": (2, U)}, 2, names=["U"])
+    assert np.array_equal(X2, U)
