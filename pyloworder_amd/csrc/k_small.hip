@@ -565,9 +565,7 @@ __global__ __launch_bounds__(NT) void k_bjac_gstep(double* H, int ldh, int rows,
   __shared__ double G[GC * GLD], G2[GC * GLD];
   __shared__ double Q[GC * GLD], Q2[GC * GLD];
   __shared__ double Pp[3 * 4 * 256];
-  __shared__ double ral[GC], rbe[GC];  // this round's rotation R: column x of R = ral[x] e_x + rbe[x] e_rpt[x]
-  __shared__ int rpt[GC];
-  __shared__ int need, rot_s;
+  __shared__ int need;
   if (state[0]) return;
   const int RP = (rows + 15) & ~15;
   const int ld = RP + 1;
@@ -579,10 +577,20 @@ __global__ __launch_bounds__(NT) void k_bjac_gstep(double* H, int ldh, int rows,
   const double negl = dstate[0];
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63, lr = l & 15, lk = l >> 4;
   auto gcol = [&](int lc) { return lc < GB ? bi * GB + lc : bj * GB + (lc - GB); };
-  for (int idx = threadIdx.x; idx < GC * RP; idx += NT) {
-    int lc = idx / RP, i = idx - lc * RP;
-    int gc = gcol(lc);
-    Hs[lc * ld + i] = (gc < cols && i < rows) ? H[int64_t(gc) * ldh + i] : 0.0;
+  {
+    // every load of the pair's columns is issued before the first LDS store
+    constexpr int KL = GC * GROWS / NT;
+    double hv[KL];
+#pragma unroll
+    for (int k = 0; k < KL; k++) {
+      const int idx = threadIdx.x + k * NT, lc = idx / RP, i = idx - lc * RP, gc = gcol(lc);
+      hv[k] = (idx < GC * RP && gc < cols && i < rows) ? H[int64_t(gc) * ldh + i] : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < KL; k++) {
+      const int idx = threadIdx.x + k * NT, lc = idx / RP, i = idx - lc * RP;
+      if (idx < GC * RP) Hs[lc * ld + i] = hv[k];
+    }
   }
   if (threadIdx.x == 0) need = 0;
   __syncthreads();
@@ -604,10 +612,15 @@ __global__ __launch_bounds__(NT) void k_bjac_gstep(double* H, int ldh, int rows,
     }
   }
   __syncthreads();
+  // The first step of a sweep rotates every pair of its 2 GB columns (GC - 1
+  // rounds); later steps only the GB^2 pairs across the two blocks (GB
+  // rounds): every block meets its own columns in the first step, so a sweep
+  // still rotates each pair of columns once, in n - 1 rounds in all.
+  const bool full = step == 0;
   for (int idx = threadIdx.x; idx < GC * GC; idx += NT) {
     int u = idx / GC, v = idx - u * GC;
     Q[u * GLD + v] = u == v ? 1.0 : 0.0;
-    if (u < v) {
+    if (u < v && (full || (u < GB && v >= GB))) {
       double a = G[u * GLD + u], bb = G[v * GLD + v], c = G[u * GLD + v];
       if (a > negl && bb > negl && fabs(c) > tol * sqrt(a) * sqrt(bb)) need = 1;
     }
@@ -615,55 +628,98 @@ __global__ __launch_bounds__(NT) void k_bjac_gstep(double* H, int ldh, int rows,
   __syncthreads();
   if (!need) return;
   if (threadIdx.x == 0) atomicOr(&state[1 + sweep], 1);
-  // Each round applies 16 disjoint rotations as G <- R^T G R and Q <- Q R in
-  // one pass: thread (i, j) forms its entry from the old G and Q into the
-  // other buffer ((G R)[i][j] and (G R)[i'][j] first, then the row
-  // combination), so a round needs two barriers.
-  static_assert(NT == GC * GC, "one thread per entry of the pair's Gram");
-  double* Gc = G;
-  double* Gn = G2;
-  double* Qc = Q;
-  double* Qn = Q2;
-  for (int inner = 0; inner < max_inner; inner++) {
-    if (threadIdx.x == 0) rot_s = 0;
-    __syncthreads();
-    for (int s = 0; s < GC - 1; s++) {
-      if (threadIdx.x < GB) {
-        int pr = threadIdx.x;
-        int u = rr_player(pr, s, GC), v = rr_player(GC - 1 - pr, s, GC);
-        if (u > v) { int t = u; u = v; v = t; }
-        double a = Gc[u * GLD + u], bb = Gc[v * GLD + v], c = Gc[u * GLD + v], gm = fabs(c);
-        double cs = 1.0, sn = 0.0;
-        if (a > negl && bb > negl && gm > tol * sqrt(a) * sqrt(bb)) {
-          double zeta = (bb - a) / (2.0 * gm);
-          double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-          cs = 1.0 / sqrt(1.0 + t * t);
-          sn = cs * t;
-          if (c < 0) sn = -sn;  // fold the sign of c (real phase) into the rotation
-          rot_s = 1;
-        }
-        // column u of R: cs e_u - sn e_v; column v: sn e_u + cs e_v (identity when inactive)
-        ral[u] = cs; rbe[u] = -sn; rpt[u] = v;
-        ral[v] = cs; rbe[v] = sn; rpt[v] = u;
-      }
-      __syncthreads();
-      {
-        const int i = threadIdx.x / GC, j = threadIdx.x - i * GC;
-        const double ai = ral[i], bi = rbe[i], aj = ral[j], bj = rbe[j];
-        const int ip = rpt[i], jp = rpt[j];
-        const double t1 = aj * Gc[i * GLD + j] + bj * Gc[i * GLD + jp];
-        const double t2 = aj * Gc[ip * GLD + j] + bj * Gc[ip * GLD + jp];
-        Gn[i * GLD + j] = ai * t1 + bi * t2;
-        Qn[i * GLD + j] = aj * Qc[i * GLD + j] + bj * Qc[i * GLD + jp];
-      }
-      __syncthreads();
-      double* t = Gc; Gc = Gn; Gn = t;
-      t = Qc; Qc = Qn; Qn = t;
+  // k-th pair (u < v) of round s
+  auto pair_of = [&](int k, int s, int& u, int& v) {
+    if (full) {
+      u = rr_player(k, s, GC);
+      v = rr_player(GC - 1 - k, s, GC);
+      if (u > v) { int t = u; u = v; v = t; }
+    } else {
+      u = k;
+      v = GB + (k + s) % GB;
     }
-    int any = rot_s;
-    __syncthreads();
-    if (!any) break;
+  };
+  // Each round applies GB disjoint rotations as G <- R^T G R and Q <- Q R:
+  // GB threads form the rotations; thread (bp, bq) of the first GB * GB then
+  // updates the 2 x 2 blocks of G and Q at rows {u_bp, v_bp} and columns
+  // {u_bq, v_bq} from the old buffers into the other ones.  The rotation of
+  // pair (u, v) zeroes G_uv: column u of R is cs e_u - sn e_v, column v is
+  // sn e_u + cs e_v, tan(2 theta) = 2 G_uv / (G_vv - G_uu), |theta| <= pi/4.
+  // Square roots and reciprocals come from the hardware estimates refined by
+  // Newton steps (one for the angle, two for cs, which fixes cs^2 + sn^2 = 1
+  // to rounding); the angle itself only steers convergence.
+  static_assert(GB * GB <= NT, "one thread per 2 x 2 block of the pair's Gram");
+  __shared__ double rc[GB], rs[GB];
+  const int bp = threadIdx.x / GB, bq = threadIdx.x % GB;
+  const int rounds = full ? GC - 1 : GB;
+  for (int inner = 0; inner < max_inner; inner++) {
+    int rotated = 0;
+    for (int s = 0; s < rounds; s++) {
+      const double* Gc = (s & 1) ? G2 : G;
+      double* Gn = (s & 1) ? G : G2;
+      const double* Qc = (s & 1) ? Q2 : Q;
+      double* Qn = (s & 1) ? Q : Q2;
+      if (threadIdx.x < GB) {
+        int u, v;
+        pair_of(threadIdx.x, s, u, v);
+        const double a = Gc[u * GLD + u], bb = Gc[v * GLD + v], c = Gc[u * GLD + v];
+        const double d = bb - a, e = 2.0 * fabs(c);
+        const double mde = fmax(fabs(d), e);
+        const int k = mde > 0.0 ? ilogb(mde) : 0;
+        const double ds = ldexp(d, -k), es = ldexp(e, -k);   // max(|ds|, es) in [1, 2)
+        const double x = ds * ds + es * es;
+        double y = __builtin_amdgcn_rsq(x);
+        y = y * (1.5 - 0.5 * x * y * y);
+        const double den = fabs(ds) + x * y;                  // |d| + hypot(d, e), scaled
+        double r = __builtin_amdgcn_rcp(den);
+        r = r * (2.0 - den * r);
+        const double t = es * r;                              // tan(theta) in [0, 1]
+        const double z = 1.0 + t * t;
+        double cs = __builtin_amdgcn_rsq(z);
+        cs = cs * (1.5 - 0.5 * z * cs * cs);
+        cs = cs * (1.5 - 0.5 * z * cs * cs);
+        double sn = cs * t;
+        if ((d < 0) != (c < 0)) sn = -sn;
+        const bool act = a > negl && bb > negl && fabs(c) > tol * sqrt(a) * sqrt(bb);
+        rc[threadIdx.x] = act ? cs : 1.0;
+        rs[threadIdx.x] = act ? sn : 0.0;
+        rotated |= act;
+      }
+      __syncthreads();
+      if (threadIdx.x < GB * GB) {
+        int up, vp, uq, vq;
+        pair_of(bp, s, up, vp);
+        pair_of(bq, s, uq, vq);
+        const double cp = rc[bp], sp = rs[bp], cq = rc[bq], sq = rs[bq];
+        const double g_uu = Gc[up * GLD + uq], g_uv = Gc[up * GLD + vq];
+        const double g_vu = Gc[vp * GLD + uq], g_vv = Gc[vp * GLD + vq];
+        const double q_uu = Qc[up * GLD + uq], q_uv = Qc[up * GLD + vq];
+        const double q_vu = Qc[vp * GLD + uq], q_vv = Qc[vp * GLD + vq];
+        // columns by pair bq, then rows by pair bp
+        const double x_uu = cq * g_uu - sq * g_uv, x_uv = sq * g_uu + cq * g_uv;
+        const double x_vu = cq * g_vu - sq * g_vv, x_vv = sq * g_vu + cq * g_vv;
+        Gn[up * GLD + uq] = cp * x_uu - sp * x_vu;
+        Gn[up * GLD + vq] = cp * x_uv - sp * x_vv;
+        Gn[vp * GLD + uq] = sp * x_uu + cp * x_vu;
+        Gn[vp * GLD + vq] = sp * x_uv + cp * x_vv;
+        Qn[up * GLD + uq] = cq * q_uu - sq * q_uv;
+        Qn[up * GLD + vq] = sq * q_uu + cq * q_uv;
+        Qn[vp * GLD + uq] = cq * q_vu - sq * q_vv;
+        Qn[vp * GLD + vq] = sq * q_vu + cq * q_vv;
+      }
+      __syncthreads();
+    }
+    // an odd number of rounds leaves the result in the second buffers
+    if (rounds & 1) {
+      for (int idx = threadIdx.x; idx < GC * GC; idx += NT) {
+        int u = idx / GC, v = idx - u * GC;
+        G[u * GLD + v] = G2[u * GLD + v];
+        Q[u * GLD + v] = Q2[u * GLD + v];
+      }
+    }
+    if (!__syncthreads_or(rotated)) break;
   }
+  const double* Qc = Q;
   // H[:, pair] = Hs Q  (row blocks of 16 rows, one wave each)
   for (int rb = w; rb < RP / 16; rb += NT / 64) {
     for (int cB = 0; cB < 2; cB++) {
