@@ -71,6 +71,24 @@ PC_DEV float group_max(float v, int g) {
   for (int o = g >> 1; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
   return v;
 }
+// maximum of v over the 64 lanes of the wave, returned as a wave-uniform
+// value: DPP steps within each row of 16 lanes (quad swaps, half-row and row
+// mirrors), the two row broadcasts that carry the running maximum into lane
+// 63, then a read of lane 63
+PC_DEV float wave_max64(float v) {
+  auto dpp = [](float x, auto ctrl, auto rows) {
+    const int xi = __builtin_bit_cast(int, x);
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(xi, xi, decltype(ctrl)::value,
+                                                                  decltype(rows)::value, 0xF, false));
+  };
+  v = fmaxf(v, dpp(v, std::integral_constant<int, 0xB1>{}, std::integral_constant<int, 0xF>{}));   // quad [1,0,3,2]
+  v = fmaxf(v, dpp(v, std::integral_constant<int, 0x4E>{}, std::integral_constant<int, 0xF>{}));   // quad [2,3,0,1]
+  v = fmaxf(v, dpp(v, std::integral_constant<int, 0x141>{}, std::integral_constant<int, 0xF>{}));  // row_half_mirror
+  v = fmaxf(v, dpp(v, std::integral_constant<int, 0x140>{}, std::integral_constant<int, 0xF>{}));  // row_mirror
+  v = fmaxf(v, dpp(v, std::integral_constant<int, 0x142>{}, std::integral_constant<int, 0xA>{}));  // row_bcast:15
+  v = fmaxf(v, dpp(v, std::integral_constant<int, 0x143>{}, std::integral_constant<int, 0xC>{}));  // row_bcast:31
+  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 63));
+}
 PC_DEV void split(float x, _Float16& h, _Float16& lo) {
   h = (_Float16)x;
   lo = (_Float16)(x - (float)h);
@@ -337,54 +355,112 @@ PC_DEV void rows_load(const float* __restrict__ X, int64_t ldx, int64_t m, int n
   }
 }
 
+template <int NB, int CENTER>
+PC_DEV void split_row(f4& vi, int i, int64_t m, int n, const double* mean, const double* var, int64_t tile, char* Fs,
+                      int* rexp) {
+  using RT = RowTile<NB>;
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int j = l % RT::LPR, col = 4 * j;
+  const int rt = w * RT::RPW + i * RT::RPI + l / RT::LPR;  // row in the tile
+  const int64_t row = tile * RT::TR + rt;
+  const bool rv = row < m;
+  // x_i0 (lane holding column 0 of this row; lane 0 when a row fills the wave)
+  float a;
+  if constexpr (RT::LPR == 64) a = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, vi[0]), 0));
+  else a = __shfl(vi[0], (l / RT::LPR) * RT::LPR);
+  double mu = 0.0, iv = 1.0;
+  if constexpr (CENTER == 2) {
+    mu = mean[rv ? row : 0];
+    iv = 1.0 / var[rv ? row : 0];
+  }
+  float mx = 0.0f;
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    float y = (rv && col + u < n) ? shifted<CENTER>(vi[u], a, mu, iv) : 0.0f;
+    vi[u] = y;
+    mx = fmaxf(mx, fabsf(y));
+  }
+  if constexpr (RT::LPR == 64) mx = wave_max64(mx);
+  else mx = group_max(mx, RT::LPR);
+  const int e = mx > 0.0f ? scale_exp(mx) : 0;
+  rexp[rt] = e;  // every lane of the row holds the same e (no branch)
+  const float sc = ldexpf(1.0f, e);
+  _Float16 hi[4], lo[4];
+#pragma unroll
+  for (int u = 0; u < 4; u++) split(vi[u] * sc, hi[u], lo[u]);
+  // columns 4j..4j+3 = k-slots t0..t0+3 of k-step ks, half hl
+  const int R = rt >> 5, cr = rt & 31, ks = j >> 2, hlw = (j >> 1) & 1, t0 = 4 * (j & 1);
+  const int off = (cr + 32 * hlw) * 16 + t0 * 2;
+  typedef _Float16 h4 __attribute__((ext_vector_type(4)));
+  h4 vh = {hi[0], hi[1], hi[2], hi[3]}, vl = {lo[0], lo[1], lo[2], lo[3]};
+  *reinterpret_cast<h4*>(Fs + ((R * RT::KSW + ks) * 2 + 0) * RT::FRAGB + off) = vh;
+  *reinterpret_cast<h4*>(Fs + ((R * RT::KSW + ks) * 2 + 1) * RT::FRAGB + off) = vl;
+}
+
 // shift, per-row scale (rexp), split, store A fragments: Fs[((R * KSW + ks) * 2 + part)]
 template <int NB, int CENTER>
 PC_DEV void rows_split(f4* v, int64_t m, int n, const double* mean, const double* var, int64_t tile, char* Fs,
                        int* rexp) {
-  using RT = RowTile<NB>;
-  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-  const int j = l % RT::LPR, col = 4 * j;
 #pragma unroll
-  for (int i = 0; i < RT::NLD; i++) {
-    const int rt = w * RT::RPW + i * RT::RPI + l / RT::LPR;  // row in the tile
-    const int64_t row = tile * RT::TR + rt;
-    const bool rv = row < m;
-    const float a = __shfl(v[i][0], (l / RT::LPR) * RT::LPR);  // x_i0 (lane holding column 0 of this row)
-    double mu = 0.0, iv = 1.0;
-    if constexpr (CENTER == 2) {
-      mu = mean[rv ? row : 0];
-      iv = 1.0 / var[rv ? row : 0];
-    }
-    float mx = 0.0f;
-#pragma unroll
-    for (int u = 0; u < 4; u++) {
-      float y = (rv && col + u < n) ? shifted<CENTER>(v[i][u], a, mu, iv) : 0.0f;
-      v[i][u] = y;
-      mx = fmaxf(mx, fabsf(y));
-    }
-    mx = group_max(mx, RT::LPR);
-    const int e = mx > 0.0f ? scale_exp(mx) : 0;
-    if (j == 0) rexp[rt] = e;
-    const float sc = ldexpf(1.0f, e);
-    _Float16 hi[4], lo[4];
-#pragma unroll
-    for (int u = 0; u < 4; u++) split(v[i][u] * sc, hi[u], lo[u]);
-    // columns 4j..4j+3 = k-slots t0..t0+3 of k-step ks, half hl
-    const int R = rt >> 5, cr = rt & 31, ks = j >> 2, hlw = (j >> 1) & 1, t0 = 4 * (j & 1);
-    const int off = (cr + 32 * hlw) * 16 + t0 * 2;
-    typedef _Float16 h4 __attribute__((ext_vector_type(4)));
-    h4 vh = {hi[0], hi[1], hi[2], hi[3]}, vl = {lo[0], lo[1], lo[2], lo[3]};
-    *reinterpret_cast<h4*>(Fs + ((R * RT::KSW + ks) * 2 + 0) * RT::FRAGB + off) = vh;
-    *reinterpret_cast<h4*>(Fs + ((R * RT::KSW + ks) * 2 + 1) * RT::FRAGB + off) = vl;
-  }
+  for (int i = 0; i < RowTile<NB>::NLD; i++) split_row<NB, CENTER>(v[i], i, m, n, mean, var, tile, Fs, rexp);
 }
 
 PC_DEV h8 frag(const char* Fs, int idx, int fragb, int l) { return *reinterpret_cast<const h8*>(Fs + idx * fragb + l * 16); }
 
+// LDS-staged row tiles (NB = 256, float4 rows).  buffer_load_dwordx4 ... lds
+// moves 16 bytes per lane from the buffer into LDS at M0 + 16 * lane, so one
+// instruction per row lays the tile out in row-major order.  The loads are
+// issued from inline asm: the compiler then neither counts them nor waits for
+// them (its own waits before LDS accesses would drain the pipeline), so the
+// kernel waits itself with s_waitcnt vmcnt(N), N counting the vector memory
+// operations issued after the ones waited for (they complete in issue
+// order).  No compiler-visible vector memory load may be issued between a
+// stage load and its wait.  M0 is saved and restored around the load.
+PC_DEV void lds_dma16(rsrc_t rs, uint32_t voff, uint32_t lds_addr) {
+  uint32_t save;
+  asm volatile("s_mov_b32 %0, m0\n\ts_mov_b32 m0, %1\n\tbuffer_load_dwordx4 %2, %3, 0 offen lds\n\ts_mov_b32 m0, %0"
+               : "=&s"(save)
+               : "s"(lds_addr), "v"(voff), "s"(rs)
+               : "memory");
+}
+template <int N>
+PC_DEV void wait_vm() {
+  static_assert(N >= 0 && N <= 63, "vmcnt range");
+  asm volatile("s_waitcnt vmcnt(%0)" ::"n"(N) : "memory");
+}
+
+// A wave loads and later reads only its own RPW rows, so a stage needs no
+// barrier, only the wave's own wait.  Rows >= m read 0 through the buffer
+// resource, like rows_load.
+template <int NB>
+PC_DEV void rows_stage(const float* __restrict__ X, int64_t ldx, int64_t m, int n, int64_t tile, char* stage) {
+  using RT = RowTile<NB>;
+  static_assert(RT::LPR == 64 && RT::RPI == 1, "one row per load instruction");
+  const int w = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6)), l = threadIdx.x & 63;
+  const int col = 4 * l;
+  const int64_t row0 = tile * RT::TR;
+  const int64_t nrow = m - row0 < RT::TR ? m - row0 : RT::TR;
+  const rsrc_t rs = make_rsrc(X + row0 * ldx, uint32_t(((nrow - 1) * ldx + n) * 4));
+  const uint32_t base = uint32_t(reinterpret_cast<uintptr_t>((__attribute__((address_space(3))) char*)stage));
+#pragma unroll
+  for (int i = 0; i < RT::NLD; i++) {
+    const int rt = w * RT::RPW + i;
+    lds_dma16(rs, uint32_t((rt * ldx + (col < n ? col : 0)) * 4), base + uint32_t(rt * NB * 4));
+  }
+}
+template <int NB>
+PC_DEV void rows_unstage(const char* stage, f4* v) {
+  using RT = RowTile<NB>;
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+#pragma unroll
+  for (int i = 0; i < RT::NLD; i++) v[i] = *reinterpret_cast<const f4*>(stage + (w * RT::RPW + i) * NB * 4 + l * 16);
+}
+
+
 // ---------------------------------------------------------------------------
 // P3: C = Ys W
 // ---------------------------------------------------------------------------
-template <int NB, int VEC, int CENTER>
+template <int NB, int VEC, int CENTER, bool STAGED = false>
 __global__ __launch_bounds__(NT2, 1) void k_gemm_mx2(const float* __restrict__ X, int64_t ldx, int64_t m, int n,
                                                      const double* __restrict__ mean,
                                                      const double* __restrict__ var, const h8* __restrict__ Wh,
@@ -423,6 +499,139 @@ __global__ __launch_bounds__(NT2, 1) void k_gemm_mx2(const float* __restrict__ X
   const int64_t ntiles = ceil_div(m, RT::TR);
   const int64_t t0 = int64_t(blockIdx.x) * tpc;
   const int64_t t1 = t0 + tpc < ntiles ? t0 + tpc : ntiles;
+  // stores of one 32-row block: rows >= m fall beyond num_records, columns >=
+  // nc get an offset beyond it; the hardware drops both (no branch, so the
+  // wait for the next tile's loads never has to cover these stores)
+  auto store_tile = [&](int64_t tile, const f16v& acc, const int* rx) {
+    const int col = J * 32 + c;
+    const int64_t row0 = tile * RT::TR;
+    const int64_t nrow = m - row0 < RT::TR ? m - row0 : RT::TR;
+    const rsrc_t cs = make_rsrc(C + row0 * ldc, uint32_t(((nrow - 1) * ldc + nc) * 4));
+    // lane offset of register 0; the other registers add a wave-uniform row
+    // step (the empty asm keeps the base from being hoisted as 16 offsets)
+    uint32_t off0 = uint32_t((4 * hl * ldc + col) * 4);
+    asm volatile("" : "+v"(off0));
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+      const int rt = 8 * (r >> 2) + 4 * hl + (r & 3);
+      const uint32_t off = col < nc ? off0 + uint32_t((8 * (r >> 2) + (r & 3)) * ldc * 4) : 0xFFFFFFFFu;
+      // (the builtin's data operand is a 32-bit integer: pass the float's bits)
+      __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, ldexpf(acc[r], -rx[rt]) * isc), cs, off, 0, 0);
+    }
+  };
+  // the tile's products and stores (A fragments in F, row exponents in rx)
+  auto products = [&](int64_t tile, const char* F, const int* rx) {
+    if (own) {
+      const int col = J * 32 + c;
+      // stores of this tile: rows >= m fall beyond num_records, columns >= nc get
+      // an offset beyond it; the hardware drops both (no branch, so the wait for
+      // the next tile's loads never has to cover these stores)
+      const int64_t row0 = tile * RT::TR;
+      const int64_t nrow = m - row0 < RT::TR ? m - row0 : RT::TR;
+      const rsrc_t cs = make_rsrc(C + row0 * ldc, uint32_t(((nrow - 1) * ldc + nc) * 4));
+#pragma unroll
+      for (int R = R0; R < RT::NR; R += RSTEP) {
+        f16v acc = {};
+#pragma unroll
+        for (int ks = 0; ks < KSW; ks++) {
+          h8 ah = frag(F, (R * KSW + ks) * 2 + 0, RT::FRAGB, l);
+          h8 al = frag(F, (R * KSW + ks) * 2 + 1, RT::FRAGB, l);
+          acc = mfma(ah, bh[ks], acc);
+          acc = mfma(ah, bl[ks], acc);
+          acc = mfma(al, bh[ks], acc);
+        }
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+          const int rt = 32 * R + 8 * (r >> 2) + 4 * hl + (r & 3);
+          const uint32_t off = col < nc ? uint32_t((rt * ldc + col) * 4) : 0xFFFFFFFFu;
+          // (the builtin's data operand is a 32-bit integer: pass the float's bits)
+          __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, ldexpf(acc[r], -rx[rt]) * isc), cs, off, 0,
+                                                0);
+        }
+      }
+    }
+  };
+  if constexpr (STAGED) {
+    static_assert(VEC == 1 && NB == 256 && CENTER != 2, "staged pipeline: 256 float4-aligned columns");
+    // Software pipeline over LDS: while the products of tile t run on
+    // fragments Fa / Fb (alternating), the same waves split tile t + 1 into
+    // the other fragment buffer, so the matrix cores and the vector ALU work
+    // side by side and a tile costs one barrier.  Tiles arrive through two
+    // LDS stages (St0 / St1: tile t0 + j lands in stage j mod 2); a stage is
+    // refilled with the tile two ahead as soon as its rows are in registers.
+    // Every wave owns one 32-column block here, so an iteration issues
+    // exactly NLD loads and ST = 16 stores per wave, and the wait for tile
+    // t + 1 leaves the later operations outstanding: NLD loads + ST stores
+    // before the second iteration, NLD + 2 ST afterwards.  (The mean / variance loads
+    // of CENTER 2 would be compiler-visible loads inside the loop: that case
+    // takes the register pipeline below.)
+    constexpr int NLD = RT::NLD, ST = 16, SB = RT::TR * NB * 4;
+    static_assert(NCBLK == NW2 && RT::NR == 1, "one 32-row block and one column block per wave");
+    __shared__ __align__(16) char St0[SB], St1[SB], Fb[RT::FBYTES];
+    __shared__ int rxb[RT::TR];
+    char* Fa = Fs;
+    int* rxa = rexp;
+    // the W fragments are loop invariant: settle their loads before the stage
+    // loads so that no wait for them falls inside the loop
+    __builtin_amdgcn_s_waitcnt(0);
+    if (t0 >= t1) return;
+    auto clamp = [&](int64_t t) { return t < t1 ? t : t1 - 1; };
+    rows_stage<NB>(X, ldx, m, n, t0, St0);
+    rows_stage<NB>(X, ldx, m, n, clamp(t0 + 1), St1);
+    wait_vm<NLD>();
+    {
+      f4 v[NLD];
+      rows_unstage<NB>(St0, v);
+      rows_split<NB, CENTER>(v, m, n, mean, var, t0, Fa, rxa);
+    }
+    lds_sync();
+    rows_stage<NB>(X, ldx, m, n, clamp(t0 + 2), St0);
+    // iteration k: products of tile t (fragments F, exponents rx), split of
+    // tile t + 1 (stage S) into (Fn, rxn), refill of S with tile t + 3
+    auto body = [&](int64_t tile, int k, const char* F, const int* rx, char* S, char* Fn, int* rxn) {
+      if (k == 0) wait_vm<NLD>();
+      else if (k == 1) wait_vm<NLD + ST>();
+      else wait_vm<NLD + 2 * ST>();
+      f4 v[NLD];
+      rows_unstage<NB>(S, v);
+      asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");  // the stage is read before it is refilled
+      rows_stage<NB>(X, ldx, m, n, clamp(tile + 3), S);
+      // products of tile t in four chunks of four k-steps (two accumulators,
+      // alternate k-steps), each followed by the split of one row of tile
+      // t + 1: the chunks are kept in this order, so the vector work of one
+      // wave meets the matrix work of the other wave on the SIMD
+      const int64_t nt = clamp(tile + 1);
+      f16v acc0 = {}, acc1 = {};
+#pragma unroll
+      for (int q = 0; q < NLD; q++) {
+#pragma unroll
+        for (int kk = 0; kk < KSW / NLD; kk++) {
+          const int ks = q * (KSW / NLD) + kk;
+          const h8 ah = frag(F, ks * 2 + 0, RT::FRAGB, l);
+          const h8 al = frag(F, ks * 2 + 1, RT::FRAGB, l);
+          f16v& acc = (ks & 1) ? acc1 : acc0;
+          acc = mfma(ah, bh[ks], acc);
+          acc = mfma(ah, bl[ks], acc);
+          acc = mfma(al, bh[ks], acc);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        split_row<NB, CENTER>(v[q], q, m, n, mean, var, nt, Fn, rxn);
+        __builtin_amdgcn_sched_barrier(0);
+      }
+      store_tile(tile, acc0 + acc1, rx);
+      lds_sync();
+    };
+    int64_t tile = t0;
+    int k = 0;
+    for (; tile + 1 < t1; tile += 2, k += 2) {
+      body(tile, k, Fa, rxa, St1, Fb, rxb);
+      body(tile + 1, k + 1, Fb, rxb, St0, Fa, rxa);
+    }
+    if (tile < t1) body(tile, k, Fa, rxa, St1, Fb, rxb);
+    // no LDS write may still be pending when the workgroup ends
+    wait_vm<0>();
+    return;
+  }
   // two tiles in flight per wave: buffer va / vb alternate, and the load issued
   // while a tile is processed is for the tile after next (always issued,
   // clamped to the chunk's last tile, so the code stays straight-line)
@@ -439,35 +648,7 @@ __global__ __launch_bounds__(NT2, 1) void k_gemm_mx2(const float* __restrict__ X
     rows_split<NB, CENTER>(v, m, n, mean, var, tile, Fs, rexp);
     lds_sync();
     rows_load<NB, VEC>(X, ldx, m, n, tile + 2 < t1 ? tile + 2 : t1 - 1, v);
-    if (own) {
-      const int col = J * 32 + c;
-      // stores of this tile: rows >= m fall beyond num_records, columns >= nc get
-      // an offset beyond it; the hardware drops both (no branch, so the wait for
-      // the next tile's loads never has to cover these stores)
-      const int64_t row0 = tile * RT::TR;
-      const int64_t nrow = m - row0 < RT::TR ? m - row0 : RT::TR;
-      const rsrc_t cs = make_rsrc(C + row0 * ldc, uint32_t(((nrow - 1) * ldc + nc) * 4));
-#pragma unroll
-      for (int R = R0; R < RT::NR; R += RSTEP) {
-        f16v acc = {};
-#pragma unroll
-        for (int ks = 0; ks < KSW; ks++) {
-          h8 ah = frag(Fs, (R * KSW + ks) * 2 + 0, RT::FRAGB, l);
-          h8 al = frag(Fs, (R * KSW + ks) * 2 + 1, RT::FRAGB, l);
-          acc = mfma(ah, bh[ks], acc);
-          acc = mfma(ah, bl[ks], acc);
-          acc = mfma(al, bh[ks], acc);
-        }
-#pragma unroll
-        for (int r = 0; r < 16; r++) {
-          const int rt = 32 * R + 8 * (r >> 2) + 4 * hl + (r & 3);
-          const uint32_t off = col < nc ? uint32_t((rt * ldc + col) * 4) : 0xFFFFFFFFu;
-          // (the builtin's data operand is a 32-bit integer: pass the float's bits)
-          __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, ldexpf(acc[r], -rexp[rt]) * isc), cs, off, 0,
-                                                0);
-        }
-      }
-    }
+    products(tile, Fs, rexp);
     lds_sync();
   };
   int64_t tile = t0;
@@ -676,6 +857,26 @@ int launch_gemm_mx2(hipStream_t st, const float* X, int64_t ldx, int64_t m, int 
   return dispatch_nb(n, [&](auto nbc) {
     constexpr int NB = decltype(nbc)::value;
     return dispatch_vc(load_mode(X, ldx, n, bz.zx), center, [&](auto vv, auto cc) {
+      constexpr int VV = decltype(vv)::value, CC = decltype(cc)::value;
+      if constexpr (VV == 1 && NB == 256 && CC != 2) {
+        // LDS-staged pipeline
+        {
+          // it counts its own vector memory operations: a register spill would
+          // add scratch traffic it does not count, so a build that spills is
+          // refused rather than run
+          static const bool spills = [] {
+            hipFuncAttributes fa{};
+            return hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&k_gemm_mx2<NB, VV, CC, true>)) !=
+                       hipSuccess ||
+                   fa.localSizeBytes != 0;
+          }();
+          if (spills) return -2;
+          hipLaunchKernelGGL((k_gemm_mx2<NB, VV, CC, true>), dim3(unsigned(chunks), 1, unsigned(bz.nz)), dim3(NT2), 0,
+                             st, X, ldx, m, n, mean, var, (const h8*)Wh, (const h8*)Wl, wscale, nc, tpc, C, ldc, bz.zx,
+                             bz.zb, bz.zc);
+          return 0;
+        }
+      }
       hipLaunchKernelGGL((k_gemm_mx2<NB, decltype(vv)::value, decltype(cc)::value>),
                          dim3(unsigned(chunks), 1, unsigned(bz.nz)), dim3(NT2), 0, st, X, ldx, m, n, mean, var,
                          (const h8*)Wh, (const h8*)Wl, wscale, nc, tpc, C, ldc, bz.zx, bz.zb, bz.zc);

@@ -963,7 +963,7 @@ static int launch_p3(pc_ctx* ctx, const Plan& p, const void* A, int64_t lda, int
   else
     rc = launch_gemm_f64(st, sdt, A, ldr, p.m, p.nr, center, mean, var, b.Wpad, p.nr, p.chunks3, p.tpc3, out.U, ldu,
                          bz);
-  if (rc != 0) return pc_fail(ctx, PC_ERR_SHAPE, "no gemm variant for n = %d", p.n);
+  if (rc != 0) return pc_fail(ctx, PC_ERR_SHAPE, "no gemm variant for n = %d (launcher code %d)", p.n, rc);
   pc_prof_end(ctx, PCP_GEMM, ev3);
   hipLaunchKernelGGL((k_null_fix<T, typename VtType<T, ST>::type>), dim3(nblk(int64_t(p.n) * p.n), 1, bz.nz),
                      dim3(256), 0, st, b.nnull, p.m, p.n, b.jsel, b.cnull, (typename VtType<T, ST>::type*)out.U,
