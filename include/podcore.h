@@ -271,6 +271,65 @@ int pc_dmd_modes(pc_ctx* ctx, const double* Y1, int64_t ldy, int64_t m, int64_t 
 int pc_dmd_reconstruct(pc_ctx* ctx, const void* Phi, int64_t ldphi, int64_t m, int64_t N, const double* mu_re,
                        const double* mu_im, const void* b, const double* t, int64_t nt, double* X, int64_t ldx);
 
+/* ---- serving -----------------------------------------------------------------
+ * No reference counterpart (pyLOM runs one POD per script): a native request
+ * loop for callers that serve many POD, projection and reconstruction
+ * requests on one GPU.  Each request is one call of pc_tsqr_svd (with the
+ * server's centring), pc_project or pc_reconstruct (both FP64) on a slot's own
+ * context and stream, so its result is bit for bit the one that call gives
+ * alone.  A request is admitted, first in first out with at most skip_ahead
+ * requests passing a head that does not fit, as soon as a slot is idle and its
+ * output bytes fit budget_bytes beside the resident bases and the unreleased
+ * outputs; idle bases are evicted least recently used first.  POD results stay
+ * resident as bases (U m x n, S n, VT n x n, row-major, the request's dtype)
+ * until evicted.  Each slot reserves at creation the workspace of the largest
+ * POD (m_max x n_max) either precision may run; a mixed server runs float64
+ * or n > 256 requests in FP64.  Input X is read on the device until the
+ * request is done: the caller keeps it alive.  Calls are thread-safe.
+ * pc_serve_wait returns PC_OK once the request is done (its outcome in
+ * result->status), or PC_PENDING when timeout_s (>= 0) expires first;
+ * timeout_s < 0 waits without limit.  A projection / reconstruction output
+ * (result->out, device, out_rows x out_cols, row stride out_ld) belongs to the
+ * server and is counted against the budget until pc_serve_release.
+ * pc_serve_basis gives the device pointers of a resident basis and pins
+ * (pin = 1) or unpins (pin = -1) it against eviction.  pc_serve_stats fills up
+ * to 8 counters: completed, rejected, evictions, in-flight output bytes,
+ * resident bytes (bases and unreleased outputs), queued, running, bases.     */
+#define PC_PENDING 1
+typedef struct pc_server pc_server;
+typedef enum { PC_REQ_POD = 0, PC_REQ_PROJECT = 1, PC_REQ_RECONSTRUCT = 2 } pc_req_kind;
+typedef struct {
+  int32_t kind;    /* pc_req_kind                                            */
+  int32_t dtype;   /* PC_F32 / PC_F64 of X (pod, project)                    */
+  const void* X;   /* pod: the snapshots m x n; project: m x n onto the basis */
+  int64_t ldx, m, n;
+  int64_t basis;   /* project / reconstruct: resident basis id               */
+  int64_t modes;   /* project / reconstruct: leading modes used (0 = all)    */
+} pc_request;
+typedef struct {
+  int32_t status;  /* PC_OK or the request's pc_status                       */
+  int32_t pad_;
+  int64_t basis;   /* pod: id of the basis holding (U, S, VT)                */
+  void* out;       /* project: modes x n; reconstruct: m x n                 */
+  int64_t out_rows, out_cols, out_ld;
+  double t_submit, t_admit, t_done;  /* seconds since the server was created */
+} pc_result;
+typedef struct {
+  void* U; void* S; void* VT;
+  int64_t m, n;
+  int32_t dtype; int32_t pad_;
+} pc_basis_view;
+int pc_serve_create(int device, int slots, int64_t budget_bytes, int skip_ahead, int precision, int center,
+                    int64_t m_max, int64_t n_max, pc_server** out);
+int pc_serve_destroy(pc_server* s);
+int pc_serve_submit(pc_server* s, const pc_request* req, int64_t* ticket);
+int pc_serve_wait(pc_server* s, int64_t ticket, double timeout_s, pc_result* result);
+int pc_serve_release(pc_server* s, int64_t ticket);
+int pc_serve_drain(pc_server* s, double timeout_s);
+int pc_serve_basis(pc_server* s, int64_t basis, int pin, pc_basis_view* view);
+int pc_serve_stats(pc_server* s, int64_t* counters, int n);
+const char* pc_serve_last_error(const pc_server* s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -19,6 +19,7 @@ STATUS = {
     0: "PC_OK", -1: "PC_ERR_INVALID", -2: "PC_ERR_SHAPE", -3: "PC_ERR_NOMEM",
     -4: "PC_ERR_HIP", -5: "PC_ERR_LINALG", -6: "PC_ERR_UNSUPPORTED",
 }
+STATUS_CODE = {v: k for k, v in STATUS.items()}
 
 i64 = C.c_int64
 vp = C.c_void_p
@@ -63,6 +64,15 @@ SIGNATURES = {
     "pc_dmd_operator": (ip, [vp, vp, i64, vp, i64, i64, i64, i64, vp, vp, i64, vp, i64]),
     "pc_dmd_modes": (ip, [vp, vp, i64, i64, i64, i64, vp, vp, i64, vp, vp, vp, vp, vp, vp, i64, vp]),
     "pc_dmd_reconstruct": (ip, [vp, vp, i64, i64, i64, vp, vp, vp, vp, i64, vp, i64]),
+    "pc_serve_create": (ip, [ip, ip, i64, ip, ip, ip, i64, i64, C.POINTER(vp)]),
+    "pc_serve_destroy": (ip, [vp]),
+    "pc_serve_submit": (ip, [vp, vp, C.POINTER(i64)]),
+    "pc_serve_wait": (ip, [vp, i64, C.c_double, vp]),
+    "pc_serve_release": (ip, [vp, i64]),
+    "pc_serve_drain": (ip, [vp, C.c_double]),
+    "pc_serve_basis": (ip, [vp, i64, ip, vp]),
+    "pc_serve_stats": (ip, [vp, C.POINTER(i64), ip]),
+    "pc_serve_last_error": (C.c_char_p, [vp]),
 }
 
 
@@ -70,6 +80,25 @@ class PCProblem(C.Structure):
     """Mirror of pc_problem (include/podcore.h)."""
     _fields_ = [("A", vp), ("lda", i64), ("m", i64), ("n", i64), ("center", C.c_int32), ("pad_", C.c_int32),
                 ("U", vp), ("ldu", i64), ("S", vp), ("VT", vp), ("ldvt", i64)]
+
+
+PC_PENDING = 1
+PC_REQ_POD, PC_REQ_PROJECT, PC_REQ_RECONSTRUCT = 0, 1, 2
+
+
+class PCRequest(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("dtype", C.c_int32), ("X", vp), ("ldx", i64), ("m", i64), ("n", i64),
+                ("basis", i64), ("modes", i64)]
+
+
+class PCResult(C.Structure):
+    _fields_ = [("status", C.c_int32), ("pad_", C.c_int32), ("basis", i64), ("out", vp), ("out_rows", i64),
+                ("out_cols", i64), ("out_ld", i64), ("t_submit", C.c_double), ("t_admit", C.c_double),
+                ("t_done", C.c_double)]
+
+
+class PCBasisView(C.Structure):
+    _fields_ = [("U", vp), ("S", vp), ("VT", vp), ("m", i64), ("n", i64), ("dtype", C.c_int32), ("pad_", C.c_int32)]
 
 
 class PodCoreError(RuntimeError):

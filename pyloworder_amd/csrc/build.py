@@ -15,7 +15,7 @@ from pathlib import Path
 HERE = Path(__file__).resolve().parent
 PKG = HERE.parent
 LIB = PKG / "libpodcore.so"
-SOURCES = ["ctx.cpp", "k_stream.hip", "k_mfma_f64.hip", "k_mfma_mx.hip", "k_mfma_mx2.hip", "k_spod.hip", "k_rng.hip", "k_wide.hip", "solve.hip"]
+SOURCES = ["ctx.cpp", "serve.cpp", "k_stream.hip", "k_mfma_f64.hip", "k_mfma_mx.hip", "k_mfma_mx2.hip", "k_spod.hip", "k_rng.hip", "k_wide.hip", "solve.hip"]
 ARCH = os.environ.get("PODCORE_ARCH", "gfx950")
 
 

@@ -364,3 +364,164 @@ def summary(server: Server, wall: float) -> dict:
     out["solves_per_s"] = round(sum(1 for r in ok if r.kind == POD) / wall, 2) if wall > 0 else None
     out["requests_per_s"] = round(len(ok) / wall, 2) if wall > 0 else None
     return out
+
+
+# ---------------------------------------------------------------------------
+# native request loop (pc_serve_*, include/podcore.h)
+# ---------------------------------------------------------------------------
+_HIP = None
+
+
+def _dev_copy(dst: int, src: int, nbytes: int) -> None:
+    """Device-to-device copy through the HIP runtime (synchronous)."""
+    import ctypes as C
+    global _HIP
+    if _HIP is None:
+        _HIP = C.CDLL("libamdhip64.so")
+        _HIP.hipMemcpy.restype = C.c_int
+        _HIP.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    rc = _HIP.hipMemcpy(C.c_void_p(dst), C.c_void_p(src), C.c_size_t(nbytes), 3)  # hipMemcpyDeviceToDevice
+    if rc != 0:
+        raise RuntimeError(f"hipMemcpy failed ({rc})")
+
+
+class NativeServer:
+    """Python handle of the native request loop: the admission, eviction,
+    slots and streams all run in the library (C++ threads), so the serving
+    loop holds no Python lock while requests run.
+
+    ``submit_*`` return a ticket; ``wait`` returns the request's record (status,
+    basis id, timings) once done, or None on timeout; ``output`` copies a
+    projection / reconstruction output into a new tensor and releases it;
+    ``basis`` copies a resident basis (U, S, VT).  Inputs stay referenced here
+    until their request is done.
+    """
+
+    def __init__(self, slots: int = 4, budget: float = 200e9, skip_ahead: int = 8, precision: str = "mixed",
+                 remove_mean: bool = True, m_max: int = 4_000_000, n_max: int = 512, device: Optional[int] = None):
+        import ctypes as C
+        import torch
+        from . import _lib
+        self._C, self._torch, self._lib_mod = C, torch, _lib
+        self.lib = _lib.load()
+        dev = torch.cuda.current_device() if device is None else device
+        prec = _lib.PC_PREC_MIXED if precision == "mixed" else _lib.PC_PREC_FP64
+        h = C.c_void_p()
+        rc = self.lib.pc_serve_create(dev, slots, int(budget), skip_ahead, prec, 1 if remove_mean else 0,
+                                      int(m_max), int(n_max), C.byref(h))
+        if rc != 0:
+            raise _lib.PodCoreError(rc, "pc_serve_create failed")
+        self.h = h
+        self.device = dev
+        self._inputs: Dict[int, Any] = {}
+
+    # -- requests -------------------------------------------------------------
+    def _submit(self, kind, X=None, basis=-1, modes=0) -> int:
+        from ._tensor import pc_dtype
+        C, _lib = self._C, self._lib_mod
+        q = _lib.PCRequest()
+        q.kind = kind
+        q.basis = int(basis)
+        q.modes = int(modes)
+        if X is not None:
+            if not (X.is_cuda and X.dim() == 2 and X.stride(1) == 1):
+                raise ValueError("X must be a 2-D CUDA tensor with unit column stride")
+            q.dtype = pc_dtype(X)
+            q.X = X.data_ptr()
+            q.ldx = X.stride(0)
+            q.m, q.n = X.shape
+        t = C.c_int64()
+        rc = self.lib.pc_serve_submit(self.h, C.byref(q), C.byref(t))
+        if rc != 0:
+            raise _lib.PodCoreError(rc, "pc_serve_submit: " + self._err())
+        if X is not None:
+            self._inputs[t.value] = X
+        return t.value
+
+    def submit_pod(self, X) -> int:
+        return self._submit(self._lib_mod.PC_REQ_POD, X)
+
+    def submit_project(self, basis: int, X, modes: int = 0) -> int:
+        return self._submit(self._lib_mod.PC_REQ_PROJECT, X, basis, modes)
+
+    def submit_reconstruct(self, basis: int, modes: int = 0) -> int:
+        return self._submit(self._lib_mod.PC_REQ_RECONSTRUCT, None, basis, modes)
+
+    def wait(self, ticket: int, timeout: Optional[float] = None) -> Optional[dict]:
+        C, _lib = self._C, self._lib_mod
+        r = _lib.PCResult()
+        rc = self.lib.pc_serve_wait(self.h, ticket, -1.0 if timeout is None else float(timeout), C.byref(r))
+        if rc == _lib.PC_PENDING:
+            return None
+        if rc != 0:
+            raise _lib.PodCoreError(rc, "pc_serve_wait: " + self._err())
+        self._inputs.pop(ticket, None)
+        out = {"status": r.status, "basis": r.basis, "out": r.out, "shape": (r.out_rows, r.out_cols),
+               "ld": r.out_ld, "t_submit": r.t_submit, "t_admit": r.t_admit, "t_done": r.t_done,
+               "latency": r.t_done - r.t_submit}
+        if r.status != 0:
+            out["error"] = self._err()
+        return out
+
+    def _err(self) -> str:
+        return self.lib.pc_serve_last_error(self.h).decode()
+
+    def output(self, ticket: int, dtype, release: bool = True):
+        """Copy of a finished projection / reconstruction output."""
+        rec = self.wait(ticket)
+        if rec["status"] != 0:
+            raise self._lib_mod.PodCoreError(rec["status"], rec.get("error", "request failed"))
+        rows, cols = rec["shape"]
+        t = self._torch.empty((rows, cols), dtype=dtype, device=f"cuda:{self.device}")
+        _dev_copy(t.data_ptr(), rec["out"], t.numel() * t.element_size())
+        if release:
+            self.release(ticket)
+        return t
+
+    def release(self, ticket: int) -> None:
+        rc = self.lib.pc_serve_release(self.h, ticket)
+        if rc != 0:
+            raise self._lib_mod.PodCoreError(rc, "pc_serve_release: " + self._err())
+
+    def basis(self, bid: int):
+        """Copies (U, S, VT) of a resident basis (pinned while copied)."""
+        C, _lib, torch = self._C, self._lib_mod, self._torch
+        v = _lib.PCBasisView()
+        rc = self.lib.pc_serve_basis(self.h, bid, 1, C.byref(v))
+        if rc != 0:
+            raise _lib.PodCoreError(rc, "pc_serve_basis: " + self._err())
+        try:
+            dt = torch.float32 if v.dtype == _lib.PC_F32 else torch.float64
+            dev = f"cuda:{self.device}"
+            U = torch.empty((v.m, v.n), dtype=dt, device=dev)
+            S = torch.empty((v.n,), dtype=dt, device=dev)
+            VT = torch.empty((v.n, v.n), dtype=dt, device=dev)
+            for t, p in ((U, v.U), (S, v.S), (VT, v.VT)):
+                _dev_copy(t.data_ptr(), p, t.numel() * t.element_size())
+        finally:
+            self.lib.pc_serve_basis(self.h, bid, -1, None)
+        return U, S, VT
+
+    def drain(self, timeout: Optional[float] = None) -> bool:
+        return self.lib.pc_serve_drain(self.h, -1.0 if timeout is None else float(timeout)) == 0
+
+    def stats(self) -> dict:
+        C = self._C
+        v = (C.c_int64 * 8)()
+        self.lib.pc_serve_stats(self.h, v, 8)
+        keys = ("completed", "rejected", "evictions", "inflight_bytes", "resident_bytes", "queued", "running",
+                "bases")
+        return dict(zip(keys, list(v)))
+
+    def close(self) -> None:
+        if self.h:
+            self.lib.pc_serve_destroy(self.h)
+            self.h = None
+            self._inputs.clear()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+        return False
