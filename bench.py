@@ -52,6 +52,8 @@ def parse():
     ap.add_argument("--mode", default="both", choices=["both", "mixed", "fp64"])
     ap.add_argument("--no-extras", action="store_true",
                     help="pod workload: skip the randomized, grouped and serving summaries")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default product-pass launch geometry")
+    ap.add_argument("--python-serve", action="store_true", help="serve through the Python loop, not pc_serve_*")
     ap.add_argument("--rand-r", type=int, default=74, help="randomized sketch width (64 modes + 10)")
     ap.add_argument("--rand-q", type=int, default=3, help="randomized power iterations")
     ap.add_argument("--rate", type=float, default=200.0, help="serve: Poisson arrival rate, requests/s")
@@ -90,12 +92,15 @@ def generate(m, n, seed=2024):
 class Solver:
     """Direct C-ABI calls (no Python-side allocation inside the timed loop)."""
 
-    def __init__(self, X, precision):
+    def __init__(self, X, precision, autotune=False):
         import torch
         from pyloworder_amd import _lib
         from pyloworder_amd._tensor import pc_dtype, real_dtype
         self.torch = torch
         self.ctx = _lib.context()
+        # product-pass launch geometry tuned on the first solve (warm-up); the
+        # tuner changes speed only, never the result
+        self.ctx.check(self.ctx.lib.pc_autotune(self.ctx.h, 1 if autotune else 0))
         self.X = X
         m, n = X.shape
         self.m, self.n = m, n
@@ -110,6 +115,11 @@ class Solver:
         c.check(c.lib.pc_tsqr_svd(c.h, self.dt, self.X.data_ptr(), self.X.stride(0), self.m, self.n, 1,
                                   self.U.data_ptr(), self.U.stride(0), self.S.data_ptr(), self.V.data_ptr(),
                                   self.V.stride(0), self.prec))
+
+    def tuned_chunks(self):
+        import ctypes as C
+        v = C.c_int64(0)
+        return v.value if self.ctx.lib.pc_autotune_query(self.dt, self.m, self.n, self.prec, C.byref(v)) == 1 else None
 
     def profile(self, on):
         self.ctx.check(self.ctx.lib.pc_profile(self.ctx.h, 1 if on else 0))
@@ -550,6 +560,9 @@ def run_serve(a, emit=True):
     wall = max(r.done for r in srv.completed) - t0
     srv.close()
     summ = serve.summary(srv, wall)
+    okr = [r for r in srv.completed if r.error is None]
+    summ["queue_p50_ms"] = serve.percentile([(r.admitted - r.arrival) * 1e3 for r in okr], 50)
+    summ["run_p50_ms"] = serve.percentile([(r.done - r.admitted) * 1e3 for r in okr], 50)
     gbs = algo["bytes"] / wall / 1e9
     out = {"metric": "serving: solves/s and p50/p99 latency (POD / project / reconstruct stream)",
            "value": summ["solves_per_s"], "unit": "solves/s", "n_gpus": 1, "steps": len(arrivals), "warmup": 2 * a.slots,
@@ -564,6 +577,142 @@ def run_serve(a, emit=True):
                       "workspace_reserve_per_slot_GB": round(reserve / 1e9, 3)},
            "p50_ms": summ["p50_ms"], "p99_ms": summ["p99_ms"], "serve": summ,
            "algorithmic_gbs": round(gbs, 1), "late_submissions": late}
+    if emit:
+        print(json.dumps(out), flush=True)
+    del pools
+    torch.cuda.empty_cache()
+    return out
+
+
+def run_serve_native(a, emit=True):
+    """The workload of run_serve through the native request loop (pc_serve_*):
+    admission, eviction, slots and streams run in the library's C++ threads."""
+    import torch
+    from pyloworder_amd.serve import NativeServer, percentile
+    M_MAX, N_MAX = 4_000_000, 512
+    log(f"[bench] serve (native): generating 3 input pools {M_MAX} x {N_MAX} float32")
+    pools = [pool_device(M_MAX, N_MAX, d, 100 + i) for i, d in enumerate((0.9, 0.935, 0.97))]
+    rng = np.random.default_rng(12)
+    srv = NativeServer(slots=a.slots, budget=200e9, precision="mixed", m_max=M_MAX, n_max=N_MAX)
+
+    def logu(lo, hi):
+        return float(np.exp(rng.uniform(np.log(lo), np.log(hi))))
+
+    reqs = []         # (ticket, kind, algorithmic bytes)
+    outs = []         # projection / reconstruction tickets whose output is not released yet
+    recs = {}         # ticket -> record, kept when an output is released
+
+    def release_done():
+        # outputs count against the budget until released: copy nothing, just
+        # record and release what has finished
+        still = []
+        for t in outs:
+            rec = srv.wait(t, 0.0)
+            if rec is None:
+                still.append(t)
+                continue
+            recs[t] = rec
+            if rec["status"] == 0:
+                srv.release(t)
+        outs[:] = still
+
+    def submit(kind):
+        ids = srv.resident() if kind != "pod" else []
+        shape = None
+        if ids:
+            bid = ids[int(rng.integers(len(ids)))]
+            shape = srv.basis_shape(bid)
+        if shape is not None:
+            m, N = shape
+            k = int(min(N, rng.integers(4, 65)))
+            if kind == "project":
+                n2 = int(round(logu(8, N_MAX)))
+                t = srv.submit_project(bid, pools[int(rng.integers(3))][:m, :n2], modes=k)
+                outs.append(t)
+                return t, kind, (m * k + m * n2 + k * n2) * 4
+            t = srv.submit_reconstruct(bid, modes=k)
+            outs.append(t)
+            return t, "reconstruct", (m * k + k * N + m * N) * 4
+        m = int(round(logu(1e4, M_MAX)))
+        n = min(int(round(logu(8, N_MAX))), m)
+        t = srv.submit_pod(pools[int(rng.integers(3))][:m, :n])
+        return t, "pod", 3 * m * n * 4
+
+    # warm-up: the stream-ordered allocator's pools
+    warm = [srv.submit_pod(pools[0][:M_MAX, :N_MAX]) for _ in range(2 * a.slots)]
+    srv.drain()
+    for t in warm:
+        srv.wait(t)
+    arrivals, t = [], 0.0
+    while True:
+        t += rng.exponential(1.0 / a.rate)
+        if t > a.duration:
+            break
+        arrivals.append(t)
+    kinds = rng.choice(["pod", "project", "reconstruct"], size=len(arrivals), p=[0.5, 0.25, 0.25])
+    log(f"[bench] serve (native): {len(arrivals)} requests over {a.duration} s at {a.rate}/s, {a.slots} slots")
+    t0 = time.perf_counter()
+    late = 0
+    for ta, kind in zip(arrivals, kinds):
+        now = time.perf_counter() - t0
+        if ta > now:
+            time.sleep(ta - now)
+        elif now - ta > 0.01:
+            late += 1
+        release_done()
+        reqs.append(submit(str(kind)))
+    t_wait = time.perf_counter()
+    while outs and time.perf_counter() - t_wait < 120:
+        release_done()
+        time.sleep(0.002)
+    if outs or not srv.drain(timeout=120):
+        raise RuntimeError(f"serve (native): requests still open after 120 s: {srv.stats()}")
+    t_end = time.perf_counter() - t0
+    lat = {"pod": [], "project": [], "reconstruct": []}
+    ok = rejected = errors = 0
+    queue, run = [], []
+    why = {}
+    algo = 0
+    first_submit = last_done = None
+    for t, kind, b in reqs:
+        rec = recs[t] if t in recs else srv.wait(t)
+        if rec["status"] != 0:
+            errors += 1
+            if rec["t_admit"] == 0.0:
+                rejected += 1
+            key = f"{kind}: {rec.get('error', rec['status'])}"
+            why[key] = why.get(key, 0) + 1
+            continue
+        ok += 1
+        algo += b
+        lat[kind].append(rec["latency"] * 1e3)
+        queue.append((rec["t_admit"] - rec["t_submit"]) * 1e3)
+        run.append((rec["t_done"] - rec["t_admit"]) * 1e3)
+        first_submit = rec["t_submit"] if first_submit is None else min(first_submit, rec["t_submit"])
+        last_done = rec["t_done"] if last_done is None else max(last_done, rec["t_done"])
+    wall = (last_done - first_submit) if ok else t_end
+    st = srv.stats()
+    srv.close()
+    all_lat = lat["pod"] + lat["project"] + lat["reconstruct"]
+    summ = {"requests": len(reqs), "ok": ok, "rejected": rejected, "errors": errors, "evictions": st["evictions"],
+            "wall_s": round(wall, 3), "p50_ms": percentile(all_lat, 50), "p99_ms": percentile(all_lat, 99),
+            "solves_per_s": round(len(lat["pod"]) / wall, 2), "requests_per_s": round(ok / wall, 2),
+            "queue_p50_ms": percentile(queue, 50), "run_p50_ms": percentile(run, 50), "failures": why}
+    for k in lat:
+        summ[k] = {"count": len(lat[k]), "p50_ms": percentile(lat[k], 50), "p99_ms": percentile(lat[k], 99)}
+    out = {"metric": "serving: solves/s and p50/p99 latency (POD / project / reconstruct stream)",
+           "value": summ["solves_per_s"], "unit": "solves/s", "n_gpus": 1, "steps": len(arrivals), "warmup": 2 * a.slots,
+           "ms_per_step": round(wall * 1e3 / max(1, ok), 3), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None,
+           "dtype": "f32 storage; POD n <= 256: fp16x2 MFMA / FP64 small, n > 256: FP64 MFMA; project / reconstruct: "
+                    "FP64 MFMA",
+           "data": "synthetic (device-generated)",
+           "config": {"workload": "Poisson arrivals, 50% POD.run / 25% project / 25% reconstruct on resident bases, "
+                                  "m log-U[1e4, 4e6], n log-U[8, 512], decay in {0.9, 0.935, 0.97}, float32",
+                      "rate_per_s": a.rate, "duration_s": a.duration, "slots": a.slots, "budget_GB": 200,
+                      "runtime": "native pc_serve_*"},
+           "p50_ms": summ["p50_ms"], "p99_ms": summ["p99_ms"], "serve": summ,
+           "algorithmic_gbs": round(algo / wall / 1e9, 1), "late_submissions": late}
     if emit:
         print(json.dumps(out), flush=True)
     del pools
@@ -620,7 +769,7 @@ def main():
     if a.workload == "grouped":
         return run_grouped(a)
     if a.workload == "serve":
-        return run_serve(a)
+        return run_serve(a) if a.python_serve else run_serve_native(a)
     if a.workload == "spod":
         return run_spod(a)
     if a.workload == "copy":
@@ -634,12 +783,13 @@ def main():
     res64 = None
     if a.mode in ("both", "fp64") and not a.no_fp64:
         log("[bench] fp64 path")
-        s64 = Solver(X64, "fp64")
+        s64 = Solver(X64, "fp64", autotune=not a.no_autotune)
         ms64, lat64 = time_solves(s64, a.steps, a.warmup)
         res64 = {"ms_per_solve": round(ms64, 3), "solves_per_s": round(1e3 / ms64, 3),
                  "hbm_gbs": round(3 * m * n * 8 / (ms64 * 1e-3) / 1e9, 1),
                  "p50_ms": round(lat64[len(lat64) // 2], 3), "p99_ms": round(lat64[min(len(lat64) - 1, int(0.99 * len(lat64)))], 3),
-                 "stages_ms": {k: round(v[0], 3) for k, v in stage_profile(s64).items()}}
+                 "stages_ms": {k: round(v[0], 3) for k, v in stage_profile(s64).items()},
+                 "product_pass_chunks": s64.tuned_chunks()}
         log(f"[bench] fp64: {res64}")
         del s64
     X32 = X64.float()
@@ -652,7 +802,7 @@ def main():
     del X64
     torch.cuda.empty_cache()
     log("[bench] mixed path")
-    smx = Solver(X32, "mixed")
+    smx = Solver(X32, "mixed", autotune=not a.no_autotune)
     msmx, latmx = time_solves(smx, a.steps, a.warmup)
     stages = stage_profile(smx)
     dom, roof = roofline_for(stages, m, n, 4)
@@ -718,7 +868,7 @@ def main():
         sa = argparse.Namespace(**vars(a))
         sa.duration = 10.0
         log("[bench] serving summary")
-        sv = run_serve(sa, emit=False)
+        sv = run_serve(sa, emit=False) if a.python_serve else run_serve_native(sa, emit=False)
         sm = sv["serve"]
         served = {"config": sv["config"], "solves_per_s": sv["value"], "p50_ms": sv["p50_ms"],
                   "p99_ms": sv["p99_ms"], "requests_per_s": sm.get("requests_per_s"),
@@ -749,6 +899,7 @@ def main():
         "p50_ms": round(latmx[len(latmx) // 2], 3),
         "p99_ms": round(latmx[min(len(latmx) - 1, int(0.99 * len(latmx)))], 3),
         "stages_ms": {k_: round(v[0], 3) for k_, v in stages.items()},
+        "autotune": None if a.no_autotune else {"product_pass_chunks": smx.tuned_chunks()},
         "dominant_kernel": dom,
         "roofline": roof,
         "accuracy_mixed_vs_fp64": acc,

@@ -271,6 +271,20 @@ int pc_dmd_modes(pc_ctx* ctx, const double* Y1, int64_t ldy, int64_t m, int64_t 
 int pc_dmd_reconstruct(pc_ctx* ctx, const void* Phi, int64_t ldphi, int64_t m, int64_t N, const double* mu_re,
                        const double* mu_im, const void* b, const double* t, int64_t nt, double* X, int64_t ldx);
 
+/* ---- launch tuning ------------------------------------------------------------
+ * No reference counterpart (tile shapes of the GPU passes).  The product pass
+ * U = Y W of pc_tsqr_svd / pc_tsqr / POD is row-local, so the number of
+ * workgroups its row tiles are spread over changes its speed, never its
+ * result.  pc_autotune(ctx, 1) makes the first such pass of a new (dtype, m,
+ * n, precision) on ctx time every candidate chunk count twice on the real
+ * operands and keep the fastest in a process-wide table; later solves of that
+ * shape on any context use it (batched or graph-captured launches only read
+ * it).  pc_autotune_query gives the tuned chunk count (returns 1) or 0 when
+ * the shape is not tuned; pc_autotune_clear empties the table.             */
+int pc_autotune(pc_ctx* ctx, int enable);
+int pc_autotune_query(pc_dtype dtype, int64_t m, int64_t n, int precision, int64_t* chunks);
+int pc_autotune_clear(void);
+
 /* ---- serving -----------------------------------------------------------------
  * No reference counterpart (pyLOM runs one POD per script): a native request
  * loop for callers that serve many POD, projection and reconstruction
@@ -278,13 +292,17 @@ int pc_dmd_reconstruct(pc_ctx* ctx, const void* Phi, int64_t ldphi, int64_t m, i
  * server's centring), pc_project or pc_reconstruct (both FP64) on a slot's own
  * context and stream, so its result is bit for bit the one that call gives
  * alone.  A request is admitted, first in first out with at most skip_ahead
- * requests passing a head that does not fit, as soon as a slot is idle and its
- * output bytes fit budget_bytes beside the resident bases and the unreleased
- * outputs; idle bases are evicted least recently used first.  POD results stay
+ * POD requests passing a head that does not fit (projections and
+ * reconstructions, which pin their basis from submission, may always pass),
+ * as soon as a slot is idle and its output bytes fit budget_bytes beside the
+ * resident bases and the unreleased outputs; idle bases are evicted least
+ * recently used first.  Unreleased outputs and pinned bases are never
+ * evicted: a request that only fits once the caller releases or unpins waits.  POD results stay
  * resident as bases (U m x n, S n, VT n x n, row-major, the request's dtype)
  * until evicted.  Each slot reserves at creation the workspace of the largest
- * POD (m_max x n_max) either precision may run; a mixed server runs float64
- * or n > 256 requests in FP64.  Input X is read on the device until the
+ * POD (m_max x n_max) either precision may run, taken from budget_bytes
+ * (PC_ERR_NOMEM when the workspaces alone exceed it); a mixed server runs
+ * float64 or n > 256 requests in FP64.  Input X is read on the device until the
  * request is done: the caller keeps it alive.  Calls are thread-safe.
  * pc_serve_wait returns PC_OK once the request is done (its outcome in
  * result->status), or PC_PENDING when timeout_s (>= 0) expires first;
@@ -293,8 +311,9 @@ int pc_dmd_reconstruct(pc_ctx* ctx, const void* Phi, int64_t ldphi, int64_t m, i
  * server and is counted against the budget until pc_serve_release.
  * pc_serve_basis gives the device pointers of a resident basis and pins
  * (pin = 1) or unpins (pin = -1) it against eviction.  pc_serve_stats fills up
- * to 8 counters: completed, rejected, evictions, in-flight output bytes,
- * resident bytes (bases and unreleased outputs), queued, running, bases.     */
+ * to 9 counters: completed, rejected, evictions, in-flight output bytes,
+ * resident bytes (bases and unreleased outputs), queued, running, bases, and
+ * the bytes left for outputs and bases once the workspaces are reserved.    */
 #define PC_PENDING 1
 typedef struct pc_server pc_server;
 typedef enum { PC_REQ_POD = 0, PC_REQ_PROJECT = 1, PC_REQ_RECONSTRUCT = 2 } pc_req_kind;
@@ -327,6 +346,8 @@ int pc_serve_wait(pc_server* s, int64_t ticket, double timeout_s, pc_result* res
 int pc_serve_release(pc_server* s, int64_t ticket);
 int pc_serve_drain(pc_server* s, double timeout_s);
 int pc_serve_basis(pc_server* s, int64_t basis, int pin, pc_basis_view* view);
+/* ids of the resident bases (up to cap of them, increasing id) and their count */
+int pc_serve_bases(pc_server* s, int64_t* ids, int64_t cap, int64_t* count);
 int pc_serve_stats(pc_server* s, int64_t* counters, int n);
 const char* pc_serve_last_error(const pc_server* s);
 

@@ -64,6 +64,9 @@ SIGNATURES = {
     "pc_dmd_operator": (ip, [vp, vp, i64, vp, i64, i64, i64, i64, vp, vp, i64, vp, i64]),
     "pc_dmd_modes": (ip, [vp, vp, i64, i64, i64, i64, vp, vp, i64, vp, vp, vp, vp, vp, vp, i64, vp]),
     "pc_dmd_reconstruct": (ip, [vp, vp, i64, i64, i64, vp, vp, vp, vp, i64, vp, i64]),
+    "pc_autotune": (ip, [vp, ip]),
+    "pc_autotune_query": (ip, [ip, i64, i64, ip, C.POINTER(i64)]),
+    "pc_autotune_clear": (ip, []),
     "pc_serve_create": (ip, [ip, ip, i64, ip, ip, ip, i64, i64, C.POINTER(vp)]),
     "pc_serve_destroy": (ip, [vp]),
     "pc_serve_submit": (ip, [vp, vp, C.POINTER(i64)]),
@@ -72,6 +75,7 @@ SIGNATURES = {
     "pc_serve_drain": (ip, [vp, C.c_double]),
     "pc_serve_basis": (ip, [vp, i64, ip, vp]),
     "pc_serve_stats": (ip, [vp, C.POINTER(i64), ip]),
+    "pc_serve_bases": (ip, [vp, C.POINTER(i64), i64, C.POINTER(i64)]),
     "pc_serve_last_error": (C.c_char_p, [vp]),
 }
 

@@ -99,6 +99,8 @@ struct pc_ctx {
   // driven by its own host thread during a grouped solve
   pc_ctx* sub[kGroupStreams] = {};
   std::string err;
+  // launch-geometry tuning of the product pass (pc_autotune)
+  int autotune = 0;
   // optional per-stage timing (pc_profile): events around each streaming pass
   int prof = 0;
   double prof_ms[8] = {0};

@@ -14,9 +14,9 @@
 //   * Slots are worker threads, each with a HIP stream of default priority and
 //     a pc_ctx whose workspace is reserved up front for the largest request
 //     the server accepts (no workspace reallocation while serving).
-//   * Outputs come from the stream-ordered allocator (hipMallocAsync /
-//     hipFreeAsync), which does not synchronise the device, so allocations of
-//     one slot never stall the work of another.
+//   * Output and basis blocks freed by releases and evictions are cached by
+//     size (up to a quarter of the output budget) and reused, so serving
+//     mostly allocates nothing.
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -76,7 +76,12 @@ struct pc_server {
   std::vector<std::thread> threads;
   std::vector<pc_ctx*> ctx;
   std::vector<hipStream_t> streams;
-  hipStream_t rstream = nullptr;  // stream-ordered frees from API threads
+  // device blocks of outputs and bases: freed blocks are cached by size and
+  // reused (best fit within a factor 2), up to cache_cap bytes
+  std::mutex cache_mu;
+  std::multimap<int64_t, void*> cache;  // size -> block
+  std::map<void*, int64_t> block_size;
+  int64_t cached = 0, cache_cap = 0;
   int64_t inflight_bytes = 0, resident_bytes = 0;
   int64_t next_ticket = 0, next_basis = 0;
   int64_t n_completed = 0, n_rejected = 0, n_evictions = 0, n_inflight = 0;
@@ -108,6 +113,57 @@ int64_t bytes_needed(const pc_request& q, const Basis* b) {
   return b->m * b->n * esize(b->dtype);
 }
 
+// Device block for an output or a basis.  A cached block of at least the
+// size (and at most twice it) is reused; otherwise hipMalloc, and when the
+// device is out of memory the cache is emptied (hipFree) and the allocation
+// tried once more.  Blocks are cached only once the work that used them has
+// completed (a request's stream is synchronised before it completes, and a
+// basis is freed only when no request pins it), so reuse needs no stream
+// ordering.
+void cache_trim(pc_server* s, int64_t keep) {
+  // caller holds cache_mu; frees the largest cached blocks first
+  while (s->cached > keep && !s->cache.empty()) {
+    auto it = std::prev(s->cache.end());
+    (void)hipFree(it->second);
+    s->block_size.erase(it->second);
+    s->cached -= it->first;
+    s->cache.erase(it);
+  }
+}
+
+bool pool_alloc(pc_server* s, void** p, int64_t bytes, hipStream_t) {
+  *p = nullptr;
+  const int64_t sz = (bytes + (int64_t(2) << 20) - 1) & ~((int64_t(2) << 20) - 1);  // 2 MiB granules
+  std::lock_guard<std::mutex> g(s->cache_mu);
+  auto it = s->cache.lower_bound(sz);
+  if (it != s->cache.end() && it->first <= 2 * sz) {
+    *p = it->second;
+    s->cached -= it->first;
+    s->cache.erase(it);
+    return true;
+  }
+  for (int attempt = 0; attempt < 2; attempt++) {
+    if (hipMalloc(p, size_t(sz)) == hipSuccess) {
+      s->block_size[*p] = sz;
+      return true;
+    }
+    (void)hipGetLastError();
+    *p = nullptr;
+    cache_trim(s, 0);
+  }
+  return false;
+}
+
+void pool_free(pc_server* s, void* p) {
+  if (!p) return;
+  std::lock_guard<std::mutex> g(s->cache_mu);
+  auto it = s->block_size.find(p);
+  if (it == s->block_size.end()) return;
+  s->cache.emplace(it->second, p);
+  s->cached += it->second;
+  cache_trim(s, s->cache_cap);
+}
+
 // evict idle bases, least recently used first, until need fits
 bool evict_for(pc_server* s, int64_t need) {
   while (s->inflight_bytes + s->resident_bytes + need > s->budget) {
@@ -120,9 +176,9 @@ bool evict_for(pc_server* s, int64_t need) {
       }
     if (victim < 0) return false;
     Basis& b = s->bases[victim];
-    (void)hipFreeAsync(b.U, s->rstream);
-    (void)hipFreeAsync(b.S, s->rstream);
-    (void)hipFreeAsync(b.VT, s->rstream);
+    pool_free(s, b.U);
+    pool_free(s, b.S);
+    pool_free(s, b.VT);
     s->resident_bytes -= b.nbytes;
     s->bases.erase(victim);
     s->n_evictions++;
@@ -137,10 +193,17 @@ void finish_rejected(pc_server* s, Req* r, int code, const std::string& msg) {
   s->n_rejected++;
 }
 
+// Requests that use a resident basis may pass the skip-ahead window: they pin
+// their basis from submission on, so a head request waiting for that memory
+// would otherwise wait on requests queued behind it.
 void admit(pc_server* s) {
   size_t i = 0;
-  while (!s->idle.empty() && i < s->pending.size() && int(i) <= s->skip_ahead) {
+  while (!s->idle.empty() && i < s->pending.size()) {
     Req* r = s->pending[i];
+    if (int(i) > s->skip_ahead && r->q.kind == PC_REQ_POD) {
+      i++;
+      continue;
+    }
     Basis* b = nullptr;
     if (r->q.kind != PC_REQ_POD) b = &s->bases[r->q.basis];  // pinned at submit: still resident
     const int64_t need = bytes_needed(r->q, b);
@@ -175,11 +238,10 @@ int run_request(pc_server* s, int slot, Req* r, const Basis* b, Basis* made) {
   if (q.kind == PC_REQ_POD) {
     const int64_t e = esize(q.dtype);
     void *U = nullptr, *S = nullptr, *VT = nullptr;
-    if (hipMallocAsync(&U, size_t(q.m * q.n * e), st) != hipSuccess ||
-        hipMallocAsync(&S, size_t(q.n * e), st) != hipSuccess ||
-        hipMallocAsync(&VT, size_t(q.n * q.n * e), st) != hipSuccess) {
-      if (U) (void)hipFreeAsync(U, st);
-      if (S) (void)hipFreeAsync(S, st);
+    if (!pool_alloc(s, &U, q.m * q.n * e, st) || !pool_alloc(s, &S, q.n * e, st) ||
+        !pool_alloc(s, &VT, q.n * q.n * e, st)) {
+      pool_free(s, U);
+      pool_free(s, S);
       r->err = "output allocation failed";
       return PC_ERR_NOMEM;
     }
@@ -188,10 +250,10 @@ int run_request(pc_server* s, int slot, Req* r, const Basis* b, Basis* made) {
     if (rc == PC_OK && hipStreamSynchronize(st) != hipSuccess) rc = PC_ERR_HIP;
     if (rc != PC_OK) {
       r->err = pc_last_error(ctx);
-      (void)hipFreeAsync(U, st);
-      (void)hipFreeAsync(S, st);
-      (void)hipFreeAsync(VT, st);
       (void)hipStreamSynchronize(st);
+      pool_free(s, U);
+      pool_free(s, S);
+      pool_free(s, VT);
       return rc;
     }
     made->U = U;
@@ -211,7 +273,7 @@ int run_request(pc_server* s, int slot, Req* r, const Basis* b, Basis* made) {
       r->err = "project: X must have the basis' rows and element type";
       return PC_ERR_SHAPE;
     }
-    if (hipMallocAsync(&r->out, size_t(k * q.n * e), st) != hipSuccess) {
+    if (!pool_alloc(s, &r->out, k * q.n * e, st)) {
       r->err = "output allocation failed";
       return PC_ERR_NOMEM;
     }
@@ -220,7 +282,7 @@ int run_request(pc_server* s, int slot, Req* r, const Basis* b, Basis* made) {
     // projections and reconstructions run FP64 MFMA on the stored data
     rc = pc_project(ctx, pc_dtype(b->dtype), b->U, b->n, q.X, q.ldx, b->m, k, q.n, r->out, q.n, PC_PREC_FP64);
   } else {
-    if (hipMallocAsync(&r->out, size_t(b->m * b->n * e), st) != hipSuccess) {
+    if (!pool_alloc(s, &r->out, b->m * b->n * e, st)) {
       r->err = "output allocation failed";
       return PC_ERR_NOMEM;
     }
@@ -232,8 +294,8 @@ int run_request(pc_server* s, int slot, Req* r, const Basis* b, Basis* made) {
   if (rc == PC_OK && hipStreamSynchronize(st) != hipSuccess) rc = PC_ERR_HIP;
   if (rc != PC_OK) {
     r->err = pc_last_error(ctx);
-    (void)hipFreeAsync(r->out, st);
     (void)hipStreamSynchronize(st);
+    pool_free(s, r->out);
     r->out = nullptr;
   }
   return rc;
@@ -299,12 +361,11 @@ extern "C" int pc_serve_create(int device, int slots, int64_t budget_bytes, int 
   auto fail = [&](int code) {
     for (pc_ctx* c : s->ctx) (void)pc_ctx_destroy(c);
     for (hipStream_t st : s->streams) (void)hipStreamDestroy(st);
-    if (s->rstream) (void)hipStreamDestroy(s->rstream);
     delete s;
     return code;
   };
   if (hipSetDevice(device) != hipSuccess) return fail(PC_ERR_HIP);
-  if (hipStreamCreateWithFlags(&s->rstream, hipStreamNonBlocking) != hipSuccess) return fail(PC_ERR_HIP);
+
   // one context per slot, its workspace reserved for the largest request of
   // either precision a POD may take (a reservation now, not a reallocation
   // in the middle of serving)
@@ -316,6 +377,12 @@ extern "C" int pc_serve_create(int device, int slots, int64_t budget_bytes, int 
   const int64_t wd = pc_workspace_query(0, PC_F64, m_max, n_max, 0, PC_PREC_FP64);
   if (wd > ws) ws = wd;
   if (ws < 0) return fail(PC_ERR_SHAPE);
+  // the budget covers the slots' workspaces too: what is left is for outputs
+  // and resident bases
+  s->budget = budget_bytes - int64_t(slots) * ws;
+  if (s->budget <= 0) return fail(PC_ERR_NOMEM);
+  // freed blocks kept for reuse: up to a quarter of the output budget
+  s->cache_cap = s->budget / 4;
   for (int i = 0; i < slots; i++) {
     pc_ctx* c = nullptr;
     hipStream_t st = nullptr;
@@ -348,19 +415,16 @@ extern "C" int pc_serve_destroy(pc_server* s) {
   s->cv.notify_all();
   for (auto& t : s->threads) t.join();
   (void)hipSetDevice(s->device);
-  for (auto& kv : s->reqs) {
-    if (kv.second->out) (void)hipFreeAsync(kv.second->out, s->rstream);
-    delete kv.second;
+  (void)hipDeviceSynchronize();
+  for (auto& kv : s->reqs) delete kv.second;
+  {
+    std::lock_guard<std::mutex> g(s->cache_mu);
+    for (auto& kv : s->block_size) (void)hipFree(kv.first);  // outputs, bases and cached blocks
+    s->block_size.clear();
+    s->cache.clear();
   }
-  for (auto& kv : s->bases) {
-    (void)hipFreeAsync(kv.second.U, s->rstream);
-    (void)hipFreeAsync(kv.second.S, s->rstream);
-    (void)hipFreeAsync(kv.second.VT, s->rstream);
-  }
-  (void)hipStreamSynchronize(s->rstream);
   for (pc_ctx* c : s->ctx) (void)pc_ctx_destroy(c);
   for (hipStream_t st : s->streams) (void)hipStreamDestroy(st);
-  (void)hipStreamDestroy(s->rstream);
   delete s;
   return PC_OK;
 }
@@ -428,7 +492,7 @@ extern "C" int pc_serve_release(pc_server* s, int64_t ticket) {
   Req* r = it->second;
   if (r->status == PC_PENDING) return serve_fail(s, PC_ERR_INVALID, "request still pending");
   if (r->out) {
-    (void)hipFreeAsync(r->out, s->rstream);
+    pool_free(s, r->out);
     s->resident_bytes -= r->nbytes;
   }
   delete r;
@@ -466,12 +530,23 @@ extern "C" int pc_serve_basis(pc_server* s, int64_t basis, int pin, pc_basis_vie
   return PC_OK;
 }
 
+extern "C" int pc_serve_bases(pc_server* s, int64_t* ids, int64_t cap, int64_t* count) {
+  if (!s || !count || cap < 0 || (cap > 0 && !ids)) return PC_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(s->mu);
+  int64_t k = 0;
+  for (auto& kv : s->bases)
+    if (k < cap) ids[k++] = kv.first;
+  *count = int64_t(s->bases.size());
+  return PC_OK;
+}
+
 extern "C" int pc_serve_stats(pc_server* s, int64_t* v, int n) {
   if (!s || !v || n < 0) return PC_ERR_INVALID;
   std::lock_guard<std::mutex> lk(s->mu);
-  const int64_t all[8] = {s->n_completed, s->n_rejected, s->n_evictions, s->inflight_bytes,
-                          s->resident_bytes, int64_t(s->pending.size()), s->n_inflight, int64_t(s->bases.size())};
-  for (int i = 0; i < n && i < 8; i++) v[i] = all[i];
+  const int64_t all[9] = {s->n_completed, s->n_rejected, s->n_evictions, s->inflight_bytes,
+                          s->resident_bytes, int64_t(s->pending.size()), s->n_inflight, int64_t(s->bases.size()),
+                          s->budget};
+  for (int i = 0; i < n && i < 9; i++) v[i] = all[i];
   return PC_OK;
 }
 

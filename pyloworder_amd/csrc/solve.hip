@@ -17,6 +17,9 @@
 #include <algorithm>
 #include <atomic>
 #include <thread>
+#include <array>
+#include <map>
+
 #include "common.hpp"
 #include "kernels.hpp"
 #include "k_small.hip"
@@ -944,25 +947,90 @@ template <> void rounded_back<cdouble>(hipStream_t st, const Plan& p, SmallBufs<
 }
 
 // P3: U = Y W (the operand prepared by to_operand), then the null-space fix of U
+// ---------------------------------------------------------------------------
+// Launch-geometry tuner of the product pass.  U = Y W is row-local: how its
+// row tiles are spread over workgroups (the chunk count) changes its speed and
+// never its result, so it is the parameter tuned.  With tuning enabled on a
+// context (pc_autotune), the first product pass of a new (dtype, m, n,
+// precision) times every candidate chunk count twice on the real operands
+// (rewriting the same U) and keeps the fastest, second timing, in a
+// process-wide table that later solves of that shape use on any context.
+// Batched and captured launches only read the table.
+// ---------------------------------------------------------------------------
+namespace tune {
+std::mutex mu;
+std::map<std::array<int64_t, 4>, int> table;
+std::array<int64_t, 4> key(const Plan& p) { return {int64_t(p.dt), p.m, int64_t(p.n), p.mixed ? 1 : 0}; }
+}  // namespace tune
+
 template <class T, class ST>
-static int launch_p3(pc_ctx* ctx, const Plan& p, const void* A, int64_t lda, int center, double* mean, double* var,
+static int launch_p3(pc_ctx* ctx, const Plan& p0, const void* A, int64_t lda, int center, double* mean, double* var,
                      SmallBufs<T>& b, CoreOut out, const PassBatch& bz = PassBatch()) {
   hipStream_t st = ctx->stream;
+  pc_dtype sdt = p0.cplx ? (p0.dt == PC_C64 ? PC_F32 : PC_F64) : p0.dt;
+  int64_t ldr = p0.cplx ? 2 * lda : lda;
+  int64_t ldu = p0.cplx ? 2 * out.ldu : out.ldu;
+  Plan p = p0;
+  auto launch = [&](int chunks, int64_t tpc) {
+    if (p.wide)
+      return launch_wgemm_f64(st, sdt, A, ldr, p.m, p.nr, center, mean, var, b.Wpad, p.NB, p.nr, chunks, tpc, out.U,
+                              ldu, 0);
+    if (p.mixed)
+      return launch_gemm_mx2(st, (const float*)A, ldr, p.m, p.nr, center, mean, var, b.Wh, b.Wl, b.wscale, p.nr,
+                             chunks, tpc, (float*)out.U, ldu, bz);
+    return launch_gemm_f64(st, sdt, A, ldr, p.m, p.nr, center, mean, var, b.Wpad, p.nr, chunks, tpc, out.U, ldu,
+                           bz);
+  };
+  const int64_t tiles = ceil_div(p.m, p.tr3);
+  auto geometry = [&](int chunks) {
+    p.tpc3 = ceil_div(tiles, int64_t(chunks));
+    p.chunks3 = int(ceil_div(tiles, p.tpc3));
+  };
+  {
+    std::unique_lock<std::mutex> lk(tune::mu);
+    auto it = tune::table.find(tune::key(p0));
+    if (it != tune::table.end()) {
+      geometry(it->second);
+    } else if (ctx->autotune && bz.nz == 1) {
+      lk.unlock();
+      hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+      (void)hipStreamIsCapturing(st, &cap);
+      if (cap == hipStreamCaptureStatusNone) {
+        hipEvent_t e0, e1;
+        PC_HIP(ctx, hipEventCreate(&e0));
+        PC_HIP(ctx, hipEventCreate(&e1));
+        int best = p0.chunks3;
+        float best_ms = 0.0f;
+        const int cand[] = {p0.chunks3, 256, 512, 768, 1024, 1536, 2048, 3072};
+        for (int c : cand) {
+          if (c > tiles) continue;
+          geometry(c);
+          float ms = 0.0f;
+          for (int rep = 0; rep < 2; rep++) {
+            PC_HIP(ctx, hipEventRecord(e0, st));
+            const int rc = launch(p.chunks3, p.tpc3);
+            if (rc != 0) return pc_fail(ctx, PC_ERR_SHAPE, "no gemm variant for n = %d (launcher code %d)", p.n, rc);
+            PC_HIP(ctx, hipEventRecord(e1, st));
+            PC_HIP(ctx, hipEventSynchronize(e1));
+            PC_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
+          }
+          if (best_ms == 0.0f || ms < best_ms) {
+            best_ms = ms;
+            best = p.chunks3;
+          }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        geometry(best);
+        std::lock_guard<std::mutex> g(tune::mu);
+        tune::table[tune::key(p0)] = best;
+      }
+    }
+  }
   int rc;
   hipEvent_t ev3;
   pc_prof_begin(ctx, &ev3);
-  pc_dtype sdt = p.cplx ? (p.dt == PC_C64 ? PC_F32 : PC_F64) : p.dt;
-  int64_t ldr = p.cplx ? 2 * lda : lda;
-  int64_t ldu = p.cplx ? 2 * out.ldu : out.ldu;
-  if (p.wide)
-    rc = launch_wgemm_f64(st, sdt, A, ldr, p.m, p.nr, center, mean, var, b.Wpad, p.NB, p.nr, p.chunks3, p.tpc3,
-                          out.U, ldu, 0);
-  else if (p.mixed)
-    rc = launch_gemm_mx2(st, (const float*)A, ldr, p.m, p.nr, center, mean, var, b.Wh, b.Wl, b.wscale, p.nr,
-                         p.chunks3, p.tpc3, (float*)out.U, ldu, bz);
-  else
-    rc = launch_gemm_f64(st, sdt, A, ldr, p.m, p.nr, center, mean, var, b.Wpad, p.nr, p.chunks3, p.tpc3, out.U, ldu,
-                         bz);
+  rc = launch(p.chunks3, p.tpc3);
   if (rc != 0) return pc_fail(ctx, PC_ERR_SHAPE, "no gemm variant for n = %d (launcher code %d)", p.n, rc);
   pc_prof_end(ctx, PCP_GEMM, ev3);
   hipLaunchKernelGGL((k_null_fix<T, typename VtType<T, ST>::type>), dim3(nblk(int64_t(p.n) * p.n), 1, bz.nz),
@@ -1351,4 +1419,25 @@ extern "C" int pc_tsqr_svd_noalloc(pc_ctx* ctx, pc_dtype dt, const void* A, int6
   ctx->ws_skip = skip0;
   ctx->ws_fixed = fixed0;
   return rc;
+}
+
+extern "C" int pc_autotune(pc_ctx* ctx, int enable) {
+  if (!ctx || enable < 0 || enable > 1) return pc_fail(ctx, PC_ERR_INVALID, "pc_autotune: bad arguments");
+  ctx->autotune = enable;
+  return PC_OK;
+}
+
+extern "C" int pc_autotune_query(pc_dtype dtype, int64_t m, int64_t n, int precision, int64_t* chunks) {
+  if (!chunks) return PC_ERR_INVALID;
+  std::lock_guard<std::mutex> g(tune::mu);
+  auto it = tune::table.find({int64_t(dtype), m, n, precision == PC_PREC_MIXED ? 1 : 0});
+  if (it == tune::table.end()) return 0;
+  *chunks = it->second;
+  return 1;
+}
+
+extern "C" int pc_autotune_clear(void) {
+  std::lock_guard<std::mutex> g(tune::mu);
+  tune::table.clear();
+  return PC_OK;
 }

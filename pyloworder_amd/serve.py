@@ -502,15 +502,31 @@ class NativeServer:
             self.lib.pc_serve_basis(self.h, bid, -1, None)
         return U, S, VT
 
+    def resident(self) -> List[int]:
+        """Ids of the resident bases."""
+        C = self._C
+        cnt = C.c_int64(0)
+        self.lib.pc_serve_bases(self.h, None, 0, C.byref(cnt))
+        ids = (C.c_int64 * max(1, cnt.value))()
+        self.lib.pc_serve_bases(self.h, ids, cnt.value, C.byref(cnt))
+        return list(ids[: cnt.value])
+
+    def basis_shape(self, bid: int):
+        """(m, n) of a resident basis, or None once it is evicted."""
+        v = self._lib_mod.PCBasisView()
+        if self.lib.pc_serve_basis(self.h, bid, 0, self._C.byref(v)) != 0:
+            return None
+        return v.m, v.n
+
     def drain(self, timeout: Optional[float] = None) -> bool:
         return self.lib.pc_serve_drain(self.h, -1.0 if timeout is None else float(timeout)) == 0
 
     def stats(self) -> dict:
         C = self._C
-        v = (C.c_int64 * 8)()
-        self.lib.pc_serve_stats(self.h, v, 8)
+        v = (C.c_int64 * 9)()
+        self.lib.pc_serve_stats(self.h, v, 9)
         keys = ("completed", "rejected", "evictions", "inflight_bytes", "resident_bytes", "queued", "running",
-                "bases")
+                "bases", "output_budget")
         return dict(zip(keys, list(v)))
 
     def close(self) -> None:

@@ -115,15 +115,25 @@ def test_native_fp64_server_and_float64_inputs(pools):
             assert torch.equal(U, Ud) and torch.equal(S, Sd) and torch.equal(VT, Vd)
 
 
+def _workspaces(slots, m_max, n_max):
+    """Bytes a server of this shape reserves for its slots' workspaces."""
+    with NativeServer(slots=slots, budget=100e9, m_max=m_max, n_max=n_max) as probe:
+        return int(100e9) - probe.stats()["output_budget"]
+
+
 def test_native_admission_eviction_and_errors(pools):
     m, n = 100_000, 64
     one = m * n * 4 + n * n * 4 + n * 4          # bytes of one basis
-    with NativeServer(slots=2, budget=int(2.5 * one), skip_ahead=2, m_max=300_000, n_max=300) as srv:
+    ws = _workspaces(2, 300_000, 300)
+    with pytest.raises(_lib.PodCoreError):       # the workspaces alone exceed the budget
+        NativeServer(slots=2, budget=ws // 2, m_max=300_000, n_max=300)
+    with NativeServer(slots=2, budget=ws + int(2.5 * one), skip_ahead=2, m_max=300_000, n_max=300) as srv:
         ts = [srv.submit_pod(pools[i % 2][:m, :n]) for i in range(4)]
         assert srv.drain(timeout=120)
         recs = [srv.wait(t) for t in ts]
         assert all(r["status"] == 0 for r in recs)
         st = srv.stats()
+        assert st["output_budget"] == int(2.5 * one)
         assert st["evictions"] >= 2 and st["bases"] <= 2 and st["resident_bytes"] <= 2.5 * one
         # the oldest basis was evicted: a projection onto it fails, onto the newest runs
         bad = srv.submit_project(recs[0]["basis"], pools[0][:m, :8])
@@ -145,7 +155,8 @@ def test_native_admission_eviction_and_errors(pools):
 def test_native_pinned_basis_survives_eviction(pools):
     m, n = 100_000, 64
     one = m * n * 4 + n * n * 4 + n * 4
-    with NativeServer(slots=1, budget=int(1.5 * one), m_max=300_000, n_max=300) as srv:
+    with NativeServer(slots=1, budget=_workspaces(1, 300_000, 300) + int(1.5 * one), m_max=300_000,
+                      n_max=300) as srv:
         first = srv.wait(srv.submit_pod(pools[0][:m, :n]))
         lib, h = srv.lib, srv.h
         assert lib.pc_serve_basis(h, first["basis"], 1, None) == 0       # pin
