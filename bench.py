@@ -804,6 +804,8 @@ def main():
     log("[bench] mixed path")
     smx = Solver(X32, "mixed", autotune=not a.no_autotune)
     msmx, latmx = time_solves(smx, a.steps, a.warmup)
+    tuned = smx.tuned_chunks()
+    log(f"[bench] mixed: {msmx:.3f} ms per solve, product pass chunks {tuned}")
     stages = stage_profile(smx)
     dom, roof = roofline_for(stages, m, n, 4)
     # accuracy of the mixed result against FP64 math on the same stored data
@@ -899,7 +901,7 @@ def main():
         "p50_ms": round(latmx[len(latmx) // 2], 3),
         "p99_ms": round(latmx[min(len(latmx) - 1, int(0.99 * len(latmx)))], 3),
         "stages_ms": {k_: round(v[0], 3) for k_, v in stages.items()},
-        "autotune": None if a.no_autotune else {"product_pass_chunks": smx.tuned_chunks()},
+        "autotune": None if a.no_autotune else {"product_pass_chunks": tuned},
         "dominant_kernel": dom,
         "roofline": roof,
         "accuracy_mixed_vs_fp64": acc,

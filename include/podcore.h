@@ -294,14 +294,17 @@ int pc_autotune_clear(void);
  * alone.  A request is admitted, first in first out with at most skip_ahead
  * POD requests passing a head that does not fit (projections and
  * reconstructions, which pin their basis from submission, may always pass),
- * as soon as a slot is idle and its output bytes fit budget_bytes beside the
- * resident bases and the unreleased outputs; idle bases are evicted least
- * recently used first.  Unreleased outputs and pinned bases are never
- * evicted: a request that only fits once the caller releases or unpins waits.  POD results stay
- * resident as bases (U m x n, S n, VT n x n, row-major, the request's dtype)
- * until evicted.  Each slot reserves at creation the workspace of the largest
- * POD (m_max x n_max) either precision may run, taken from budget_bytes
- * (PC_ERR_NOMEM when the workspaces alone exceed it); a mixed server runs
+ * as soon as a slot is idle and its output blocks fit in the output arena
+ * beside the resident bases and the unreleased outputs; idle bases are
+ * evicted least recently used first.  Unreleased outputs and pinned bases are
+ * never evicted: a request that only fits once the caller releases or unpins
+ * waits.  POD results stay resident as bases (U m x n, S n, VT n x n,
+ * row-major, the request's dtype) until evicted.  budget_bytes is reserved at
+ * creation: each slot's workspace for the largest POD (m_max x n_max) either
+ * precision may run, and the rest as one device allocation (the output arena)
+ * from which every output and basis block is carved (4 KiB granules), so
+ * serving calls neither hipMalloc nor hipFree (PC_ERR_NOMEM when the
+ * workspaces alone exceed budget_bytes or the device cannot hold it); a mixed server runs
  * float64 or n > 256 requests in FP64.  Input X is read on the device until the
  * request is done: the caller keeps it alive.  Calls are thread-safe.
  * pc_serve_wait returns PC_OK once the request is done (its outcome in

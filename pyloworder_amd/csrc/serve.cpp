@@ -14,9 +14,14 @@
 //   * Slots are worker threads, each with a HIP stream of default priority and
 //     a pc_ctx whose workspace is reserved up front for the largest request
 //     the server accepts (no workspace reallocation while serving).
-//   * Output and basis blocks freed by releases and evictions are cached by
-//     size (up to a quarter of the output budget) and reused, so serving
-//     mostly allocates nothing.
+//   * The output budget is one device allocation made at creation (an
+//     arena).  A request's output blocks are carved from it, best fit over
+//     address-ordered free ranges, when the request is admitted, and go back
+//     (coalesced with their neighbours) on release or eviction.  Serving
+//     itself calls neither hipMalloc nor hipFree, and both synchronise or
+//     stall the device.  Admission is therefore allocation: a request whose
+//     blocks do not fit evicts idle bases, least recently used first, until
+//     they do or none is left.
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -45,7 +50,8 @@ struct Req {
   int64_t ticket = -1;
   int status = PC_PENDING;
   int64_t basis = -1;
-  void* out = nullptr;
+  void* blk[3] = {nullptr, nullptr, nullptr};  // a POD's U, S, VT, reserved at admission
+  void* out = nullptr;                         // a projection's / reconstruction's output
   int64_t out_rows = 0, out_cols = 0;
   int64_t nbytes = 0;
   double t_submit = 0, t_admit = 0, t_done = 0;
@@ -54,6 +60,77 @@ struct Req {
 };
 
 int64_t esize(int dtype) { return dtype == PC_F64 ? 8 : 4; }
+
+// Sub-allocator over one device allocation.  Free ranges are kept twice: by
+// offset, to coalesce a freed block with its neighbours, and by length, for
+// the best fit (the shortest free range that holds the block).  Every block
+// is a whole number of granules, so all of them are 4 KiB aligned.
+struct Arena {
+  static constexpr int64_t kGran = 4096;
+  char* base = nullptr;
+  int64_t size = 0;
+  std::map<int64_t, int64_t> free_at;         // offset -> length
+  std::multimap<int64_t, int64_t> free_len;   // length -> offset
+  std::map<int64_t, int64_t> used;            // offset -> length
+
+  static int64_t round(int64_t bytes) { return bytes <= 0 ? kGran : (bytes + kGran - 1) / kGran * kGran; }
+
+  void init(void* p, int64_t bytes) {
+    base = static_cast<char*>(p);
+    size = bytes / kGran * kGran;
+    free_at.clear();
+    free_len.clear();
+    used.clear();
+    if (size > 0) add_free(0, size);
+  }
+  void add_free(int64_t off, int64_t len) {
+    free_at[off] = len;
+    free_len.emplace(len, off);
+  }
+  void drop_free(int64_t off, int64_t len) {
+    free_at.erase(off);
+    auto r = free_len.equal_range(len);
+    for (auto it = r.first; it != r.second; ++it)
+      if (it->second == off) {
+        free_len.erase(it);
+        return;
+      }
+  }
+  void* alloc(int64_t bytes) {
+    const int64_t len = round(bytes);
+    auto it = free_len.lower_bound(len);
+    if (it == free_len.end()) return nullptr;
+    const int64_t off = it->second, have = it->first;
+    drop_free(off, have);
+    if (have > len) add_free(off + len, have - len);
+    used[off] = len;
+    return base + off;
+  }
+  void release(void* p) {
+    if (!p) return;
+    auto u = used.find(static_cast<char*>(p) - base);
+    if (u == used.end()) return;
+    int64_t off = u->first, len = u->second;
+    used.erase(u);
+    auto nx = free_at.find(off + len);  // the free range right after
+    if (nx != free_at.end()) {
+      const int64_t nlen = nx->second;
+      drop_free(nx->first, nlen);
+      len += nlen;
+    }
+    auto pv = free_at.lower_bound(off);  // the free range right before
+    if (pv != free_at.begin()) {
+      --pv;
+      if (pv->first + pv->second == off) {
+        const int64_t poff = pv->first, plen = pv->second;
+        drop_free(poff, plen);
+        off = poff;
+        len += plen;
+      }
+    }
+    add_free(off, len);
+  }
+};
 
 }  // namespace
 
@@ -76,12 +153,10 @@ struct pc_server {
   std::vector<std::thread> threads;
   std::vector<pc_ctx*> ctx;
   std::vector<hipStream_t> streams;
-  // device blocks of outputs and bases: freed blocks are cached by size and
-  // reused (best fit within a factor 2), up to cache_cap bytes
-  std::mutex cache_mu;
-  std::multimap<int64_t, void*> cache;  // size -> block
-  std::map<void*, int64_t> block_size;
-  int64_t cached = 0, cache_cap = 0;
+  // outputs and bases, carved from one allocation of the output budget
+  // (guarded by mu, like the rest of the server's state)
+  void* arena_mem = nullptr;
+  Arena arena;
   int64_t inflight_bytes = 0, resident_bytes = 0;
   int64_t next_ticket = 0, next_basis = 0;
   int64_t n_completed = 0, n_rejected = 0, n_evictions = 0, n_inflight = 0;
@@ -105,85 +180,70 @@ int pod_precision(const pc_server* s, const pc_request& q) {
   return s->precision;
 }
 
-// device bytes of a request's outputs (a POD's stay resident as its basis)
-int64_t bytes_needed(const pc_request& q, const Basis* b) {
-  if (q.kind == PC_REQ_POD) return q.m * q.n * esize(q.dtype) + q.n * q.n * esize(q.dtype) + q.n * esize(q.dtype);
+// byte sizes of a request's output blocks (a POD's U, S, VT stay resident
+// as its basis; a projection or reconstruction has one block)
+int block_sizes(const pc_request& q, const Basis* b, int64_t sz[3]) {
+  if (q.kind == PC_REQ_POD) {
+    const int64_t e = esize(q.dtype);
+    sz[0] = q.m * q.n * e;
+    sz[1] = q.n * e;
+    sz[2] = q.n * q.n * e;
+    return 3;
+  }
   const int64_t k = q.modes > 0 && q.modes < b->n ? q.modes : b->n;
-  if (q.kind == PC_REQ_PROJECT) return k * q.n * esize(b->dtype);
-  return b->m * b->n * esize(b->dtype);
+  sz[0] = q.kind == PC_REQ_PROJECT ? k * q.n * esize(b->dtype) : b->m * b->n * esize(b->dtype);
+  return 1;
 }
 
-// Device block for an output or a basis.  A cached block of at least the
-// size (and at most twice it) is reused; otherwise hipMalloc, and when the
-// device is out of memory the cache is emptied (hipFree) and the allocation
-// tried once more.  Blocks are cached only once the work that used them has
-// completed (a request's stream is synchronised before it completes, and a
-// basis is freed only when no request pins it), so reuse needs no stream
-// ordering.
-void cache_trim(pc_server* s, int64_t keep) {
-  // caller holds cache_mu; frees the largest cached blocks first
-  while (s->cached > keep && !s->cache.empty()) {
-    auto it = std::prev(s->cache.end());
-    (void)hipFree(it->second);
-    s->block_size.erase(it->second);
-    s->cached -= it->first;
-    s->cache.erase(it);
-  }
+// arena bytes a request's outputs take (whole granules)
+int64_t bytes_needed(const pc_request& q, const Basis* b) {
+  int64_t sz[3];
+  const int nb = block_sizes(q, b, sz);
+  int64_t t = 0;
+  for (int i = 0; i < nb; i++) t += Arena::round(sz[i]);
+  return t;
 }
 
-bool pool_alloc(pc_server* s, void** p, int64_t bytes, hipStream_t) {
-  *p = nullptr;
-  const int64_t sz = (bytes + (int64_t(2) << 20) - 1) & ~((int64_t(2) << 20) - 1);  // 2 MiB granules
-  std::lock_guard<std::mutex> g(s->cache_mu);
-  auto it = s->cache.lower_bound(sz);
-  if (it != s->cache.end() && it->first <= 2 * sz) {
-    *p = it->second;
-    s->cached -= it->first;
-    s->cache.erase(it);
-    return true;
-  }
-  for (int attempt = 0; attempt < 2; attempt++) {
-    if (hipMalloc(p, size_t(sz)) == hipSuccess) {
-      s->block_size[*p] = sz;
+// evict the least recently used idle basis (no queued or running users);
+// false when every resident basis is pinned
+bool evict_one(pc_server* s) {
+  int64_t victim = -1;
+  double oldest = 0;
+  for (auto& kv : s->bases)
+    if (kv.second.pins == 0 && (victim < 0 || kv.second.last_used < oldest)) {
+      victim = kv.first;
+      oldest = kv.second.last_used;
+    }
+  if (victim < 0) return false;
+  Basis& b = s->bases[victim];
+  s->arena.release(b.U);
+  s->arena.release(b.S);
+  s->arena.release(b.VT);
+  s->resident_bytes -= b.nbytes;
+  s->bases.erase(victim);
+  s->n_evictions++;
+  return true;
+}
+
+// carve a request's output blocks from the arena, evicting idle bases until
+// they fit; false (nothing held) when they cannot fit yet
+bool reserve(pc_server* s, Req* r, const Basis* b) {
+  int64_t sz[3];
+  const int nb = block_sizes(r->q, b, sz);
+  for (;;) {
+    void* p[3] = {nullptr, nullptr, nullptr};
+    int got = 0;
+    while (got < nb && (p[got] = s->arena.alloc(sz[got])) != nullptr) got++;
+    if (got == nb) {
+      if (r->q.kind == PC_REQ_POD)
+        for (int i = 0; i < 3; i++) r->blk[i] = p[i];
+      else
+        r->out = p[0];
       return true;
     }
-    (void)hipGetLastError();
-    *p = nullptr;
-    cache_trim(s, 0);
+    for (int i = 0; i < got; i++) s->arena.release(p[i]);
+    if (!evict_one(s)) return false;
   }
-  return false;
-}
-
-void pool_free(pc_server* s, void* p) {
-  if (!p) return;
-  std::lock_guard<std::mutex> g(s->cache_mu);
-  auto it = s->block_size.find(p);
-  if (it == s->block_size.end()) return;
-  s->cache.emplace(it->second, p);
-  s->cached += it->second;
-  cache_trim(s, s->cache_cap);
-}
-
-// evict idle bases, least recently used first, until need fits
-bool evict_for(pc_server* s, int64_t need) {
-  while (s->inflight_bytes + s->resident_bytes + need > s->budget) {
-    int64_t victim = -1;
-    double oldest = 0;
-    for (auto& kv : s->bases)
-      if (kv.second.pins == 0 && (victim < 0 || kv.second.last_used < oldest)) {
-        victim = kv.first;
-        oldest = kv.second.last_used;
-      }
-    if (victim < 0) return false;
-    Basis& b = s->bases[victim];
-    pool_free(s, b.U);
-    pool_free(s, b.S);
-    pool_free(s, b.VT);
-    s->resident_bytes -= b.nbytes;
-    s->bases.erase(victim);
-    s->n_evictions++;
-  }
-  return true;
 }
 
 void finish_rejected(pc_server* s, Req* r, int code, const std::string& msg) {
@@ -207,13 +267,13 @@ void admit(pc_server* s) {
     Basis* b = nullptr;
     if (r->q.kind != PC_REQ_POD) b = &s->bases[r->q.basis];  // pinned at submit: still resident
     const int64_t need = bytes_needed(r->q, b);
-    if (need > s->budget) {
+    if (need > s->arena.size) {
       s->pending.erase(s->pending.begin() + i);
       if (b) b->pins--;
       finish_rejected(s, r, PC_ERR_NOMEM, "request outputs exceed the server budget");
       continue;
     }
-    if (!evict_for(s, need)) {
+    if (!reserve(s, r, b)) {
       i++;
       continue;
     }
@@ -237,23 +297,13 @@ int run_request(pc_server* s, int slot, Req* r, const Basis* b, Basis* made) {
   const pc_request& q = r->q;
   if (q.kind == PC_REQ_POD) {
     const int64_t e = esize(q.dtype);
-    void *U = nullptr, *S = nullptr, *VT = nullptr;
-    if (!pool_alloc(s, &U, q.m * q.n * e, st) || !pool_alloc(s, &S, q.n * e, st) ||
-        !pool_alloc(s, &VT, q.n * q.n * e, st)) {
-      pool_free(s, U);
-      pool_free(s, S);
-      r->err = "output allocation failed";
-      return PC_ERR_NOMEM;
-    }
+    void *U = r->blk[0], *S = r->blk[1], *VT = r->blk[2];
     int rc = pc_tsqr_svd(ctx, pc_dtype(q.dtype), q.X, q.ldx, q.m, q.n, s->center, U, q.n, S, VT, q.n,
                          pod_precision(s, q));
     if (rc == PC_OK && hipStreamSynchronize(st) != hipSuccess) rc = PC_ERR_HIP;
     if (rc != PC_OK) {
       r->err = pc_last_error(ctx);
       (void)hipStreamSynchronize(st);
-      pool_free(s, U);
-      pool_free(s, S);
-      pool_free(s, VT);
       return rc;
     }
     made->U = U;
@@ -262,7 +312,7 @@ int run_request(pc_server* s, int slot, Req* r, const Basis* b, Basis* made) {
     made->m = q.m;
     made->n = q.n;
     made->dtype = q.dtype;
-    made->nbytes = q.m * q.n * e + q.n * e + q.n * q.n * e;
+    made->nbytes = r->nbytes;
     return PC_OK;
   }
   const int64_t k = q.modes > 0 && q.modes < b->n ? q.modes : b->n;
@@ -273,19 +323,11 @@ int run_request(pc_server* s, int slot, Req* r, const Basis* b, Basis* made) {
       r->err = "project: X must have the basis' rows and element type";
       return PC_ERR_SHAPE;
     }
-    if (!pool_alloc(s, &r->out, k * q.n * e, st)) {
-      r->err = "output allocation failed";
-      return PC_ERR_NOMEM;
-    }
     r->out_rows = k;
     r->out_cols = q.n;
     // projections and reconstructions run FP64 MFMA on the stored data
     rc = pc_project(ctx, pc_dtype(b->dtype), b->U, b->n, q.X, q.ldx, b->m, k, q.n, r->out, q.n, PC_PREC_FP64);
   } else {
-    if (!pool_alloc(s, &r->out, b->m * b->n * e, st)) {
-      r->err = "output allocation failed";
-      return PC_ERR_NOMEM;
-    }
     r->out_rows = b->m;
     r->out_cols = b->n;
     rc = pc_reconstruct(ctx, pc_dtype(b->dtype), b->U, b->n, b->S, b->VT, b->n, b->m, k, b->n, r->out, b->n,
@@ -295,8 +337,6 @@ int run_request(pc_server* s, int slot, Req* r, const Basis* b, Basis* made) {
   if (rc != PC_OK) {
     r->err = pc_last_error(ctx);
     (void)hipStreamSynchronize(st);
-    pool_free(s, r->out);
-    r->out = nullptr;
   }
   return rc;
 }
@@ -320,7 +360,13 @@ void worker(pc_server* s, int slot) {
       bb.pins--;
       bb.last_used = r->t_done;
     }
-    if (rc == PC_OK && r->q.kind == PC_REQ_POD) {
+    if (rc != PC_OK) {
+      // the stream was synchronised: the blocks are free to reuse
+      for (void*& p : r->blk) s->arena.release(p), p = nullptr;
+      s->arena.release(r->out);
+      r->out = nullptr;
+    } else if (r->q.kind == PC_REQ_POD) {
+      for (void*& p : r->blk) p = nullptr;  // now the basis' blocks
       made.last_used = r->t_done;
       r->basis = s->next_basis++;
       s->bases[r->basis] = made;
@@ -361,6 +407,7 @@ extern "C" int pc_serve_create(int device, int slots, int64_t budget_bytes, int 
   auto fail = [&](int code) {
     for (pc_ctx* c : s->ctx) (void)pc_ctx_destroy(c);
     for (hipStream_t st : s->streams) (void)hipStreamDestroy(st);
+    if (s->arena_mem) (void)hipFree(s->arena_mem);
     delete s;
     return code;
   };
@@ -381,8 +428,13 @@ extern "C" int pc_serve_create(int device, int slots, int64_t budget_bytes, int 
   // and resident bases
   s->budget = budget_bytes - int64_t(slots) * ws;
   if (s->budget <= 0) return fail(PC_ERR_NOMEM);
-  // freed blocks kept for reuse: up to a quarter of the output budget
-  s->cache_cap = s->budget / 4;
+  // the output budget is reserved now, as one allocation
+  if (hipMalloc(&s->arena_mem, size_t(s->budget)) != hipSuccess) {
+    (void)hipGetLastError();
+    s->arena_mem = nullptr;
+    return fail(PC_ERR_NOMEM);
+  }
+  s->arena.init(s->arena_mem, s->budget);
   for (int i = 0; i < slots; i++) {
     pc_ctx* c = nullptr;
     hipStream_t st = nullptr;
@@ -417,12 +469,7 @@ extern "C" int pc_serve_destroy(pc_server* s) {
   (void)hipSetDevice(s->device);
   (void)hipDeviceSynchronize();
   for (auto& kv : s->reqs) delete kv.second;
-  {
-    std::lock_guard<std::mutex> g(s->cache_mu);
-    for (auto& kv : s->block_size) (void)hipFree(kv.first);  // outputs, bases and cached blocks
-    s->block_size.clear();
-    s->cache.clear();
-  }
+  (void)hipFree(s->arena_mem);  // outputs and bases
   for (pc_ctx* c : s->ctx) (void)pc_ctx_destroy(c);
   for (hipStream_t st : s->streams) (void)hipStreamDestroy(st);
   delete s;
@@ -492,7 +539,7 @@ extern "C" int pc_serve_release(pc_server* s, int64_t ticket) {
   Req* r = it->second;
   if (r->status == PC_PENDING) return serve_fail(s, PC_ERR_INVALID, "request still pending");
   if (r->out) {
-    pool_free(s, r->out);
+    s->arena.release(r->out);
     s->resident_bytes -= r->nbytes;
   }
   delete r;

@@ -117,8 +117,8 @@ def test_native_fp64_server_and_float64_inputs(pools):
 
 def _workspaces(slots, m_max, n_max):
     """Bytes a server of this shape reserves for its slots' workspaces."""
-    with NativeServer(slots=slots, budget=100e9, m_max=m_max, n_max=n_max) as probe:
-        return int(100e9) - probe.stats()["output_budget"]
+    with NativeServer(slots=slots, budget=20e9, m_max=m_max, n_max=n_max) as probe:
+        return int(20e9) - probe.stats()["output_budget"]
 
 
 def test_native_admission_eviction_and_errors(pools):
