@@ -306,7 +306,10 @@ int pc_autotune_clear(void);
  * serving calls neither hipMalloc nor hipFree (PC_ERR_NOMEM when the
  * workspaces alone exceed budget_bytes or the device cannot hold it); a mixed server runs
  * float64 or n > 256 requests in FP64.  Input X is read on the device until the
- * request is done: the caller keeps it alive.  Calls are thread-safe.
+ * request is done: the caller keeps it alive.  request->stream names the
+ * stream X is produced on (NULL: the null stream): pc_serve_submit records an
+ * event there and the request's slot waits for it before reading X, so X may
+ * still be in flight at submission.  Calls are thread-safe.
  * pc_serve_wait returns PC_OK once the request is done (its outcome in
  * result->status), or PC_PENDING when timeout_s (>= 0) expires first;
  * timeout_s < 0 waits without limit.  A projection / reconstruction output
@@ -327,6 +330,7 @@ typedef struct {
   int64_t ldx, m, n;
   int64_t basis;   /* project / reconstruct: resident basis id               */
   int64_t modes;   /* project / reconstruct: leading modes used (0 = all)    */
+  void* stream;    /* hipStream_t X is produced on (NULL: the null stream)   */
 } pc_request;
 typedef struct {
   int32_t status;  /* PC_OK or the request's pc_status                       */

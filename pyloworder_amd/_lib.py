@@ -92,7 +92,7 @@ PC_REQ_POD, PC_REQ_PROJECT, PC_REQ_RECONSTRUCT = 0, 1, 2
 
 class PCRequest(C.Structure):
     _fields_ = [("kind", C.c_int32), ("dtype", C.c_int32), ("X", vp), ("ldx", i64), ("m", i64), ("n", i64),
-                ("basis", i64), ("modes", i64)]
+                ("basis", i64), ("modes", i64), ("stream", vp)]
 
 
 class PCResult(C.Structure):

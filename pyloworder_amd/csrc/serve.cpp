@@ -52,6 +52,7 @@ struct Req {
   int64_t basis = -1;
   void* blk[3] = {nullptr, nullptr, nullptr};  // a POD's U, S, VT, reserved at admission
   void* out = nullptr;                         // a projection's / reconstruction's output
+  hipEvent_t ready = nullptr;                  // X's producer at submission
   int64_t out_rows = 0, out_cols = 0;
   int64_t nbytes = 0;
   double t_submit = 0, t_admit = 0, t_done = 0;
@@ -160,6 +161,7 @@ struct pc_server {
   int64_t inflight_bytes = 0, resident_bytes = 0;
   int64_t next_ticket = 0, next_basis = 0;
   int64_t n_completed = 0, n_rejected = 0, n_evictions = 0, n_inflight = 0;
+  std::vector<hipEvent_t> events;  // free input-ready events
   bool stop = false;
   std::string err;
 
@@ -247,6 +249,10 @@ bool reserve(pc_server* s, Req* r, const Basis* b) {
 }
 
 void finish_rejected(pc_server* s, Req* r, int code, const std::string& msg) {
+  if (r->ready) {
+    s->events.push_back(r->ready);
+    r->ready = nullptr;
+  }
   r->status = code;
   r->err = msg;
   r->t_done = s->now();
@@ -295,6 +301,11 @@ int run_request(pc_server* s, int slot, Req* r, const Basis* b, Basis* made) {
   pc_ctx* ctx = s->ctx[slot];
   hipStream_t st = s->streams[slot];
   const pc_request& q = r->q;
+  // X may still be in flight on its producer's stream
+  if (r->ready && hipStreamWaitEvent(st, r->ready, 0) != hipSuccess) {
+    r->err = "hipStreamWaitEvent on the input's stream failed";
+    return PC_ERR_HIP;
+  }
   if (q.kind == PC_REQ_POD) {
     const int64_t e = esize(q.dtype);
     void *U = r->blk[0], *S = r->blk[1], *VT = r->blk[2];
@@ -374,6 +385,10 @@ void worker(pc_server* s, int slot) {
     } else if (rc == PC_OK) {
       // a projection / reconstruction output is counted until released
       s->resident_bytes += r->nbytes;
+    }
+    if (r->ready) {
+      s->events.push_back(r->ready);  // the slot stream was synchronised: the wait is over
+      r->ready = nullptr;
     }
     r->status = rc;
     s->work[slot] = nullptr;
@@ -458,6 +473,8 @@ extern "C" int pc_serve_destroy(pc_server* s) {
     // queued requests are cancelled; those running finish
     for (Req* r : s->pending) {
       if (r->q.kind != PC_REQ_POD) s->bases[r->q.basis].pins--;
+      if (r->ready) s->events.push_back(r->ready);
+      r->ready = nullptr;
       r->status = PC_ERR_INVALID;
       r->err = "server destroyed";
     }
@@ -470,6 +487,7 @@ extern "C" int pc_serve_destroy(pc_server* s) {
   (void)hipDeviceSynchronize();
   for (auto& kv : s->reqs) delete kv.second;
   (void)hipFree(s->arena_mem);  // outputs and bases
+  for (hipEvent_t ev : s->events) (void)hipEventDestroy(ev);
   for (pc_ctx* c : s->ctx) (void)pc_ctx_destroy(c);
   for (hipStream_t st : s->streams) (void)hipStreamDestroy(st);
   delete s;
@@ -501,6 +519,35 @@ extern "C" int pc_serve_submit(pc_server* s, const pc_request* q, int64_t* ticke
       return PC_OK;  // the request exists and reports its error
     }
     it->second.pins++;  // a basis with queued or running users is never evicted
+  }
+  if (q->kind != PC_REQ_RECONSTRUCT) {
+    // mark what X's producer has queued so far; the slot waits for it (the
+    // event lives on the server's device, made current for the call)
+    int prev = s->device;
+    (void)hipGetDevice(&prev);
+    if (prev != s->device) (void)hipSetDevice(s->device);
+    struct Restore {
+      int dev, cur;
+      ~Restore() {
+        if (dev != cur) (void)hipSetDevice(dev);
+      }
+    } restore{prev, s->device};
+    if (s->events.empty()) {
+      hipEvent_t ev = nullptr;
+      if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+        if (q->kind != PC_REQ_POD) s->bases[q->basis].pins--;
+        finish_rejected(s, r, PC_ERR_HIP, "hipEventCreate failed");
+        return PC_OK;
+      }
+      s->events.push_back(ev);
+    }
+    r->ready = s->events.back();
+    s->events.pop_back();
+    if (hipEventRecord(r->ready, static_cast<hipStream_t>(q->stream)) != hipSuccess) {
+      if (q->kind != PC_REQ_POD) s->bases[q->basis].pins--;
+      finish_rejected(s, r, PC_ERR_HIP, "hipEventRecord on the request's stream failed");
+      return PC_OK;
+    }
   }
   s->pending.push_back(r);
   admit(s);

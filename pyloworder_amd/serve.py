@@ -44,6 +44,7 @@ class Request:
     arrival: float = 0.0
     rid: int = -1
     # filled by the server
+    ready: Any = None  # marker of the input's producer at submission
     admitted: float = 0.0
     done: float = 0.0
     result: Any = None
@@ -137,6 +138,20 @@ class DeviceExecutor:
         import torch
         return torch.cuda.Stream()
 
+    def input_ready(self, req: Request):
+        """Event marking the work queued so far on the submitter's current
+        stream (the producer of req.X); None when there is no input."""
+        import torch
+        if req.X is None:
+            return None
+        ev = torch.cuda.Event()
+        ev.record(torch.cuda.current_stream(req.X.device))
+        return ev
+
+    def wait_input(self, stream, marker):
+        if marker is not None and stream is not None:
+            stream.wait_event(marker)
+
     def stream_context(self, stream):
         import torch
         return torch.cuda.stream(stream)
@@ -205,6 +220,10 @@ class Server:
 
     # -- public ---------------------------------------------------------------
     def submit(self, req: Request) -> Request:
+        # the request's slot stream waits for what the submitter's stream has
+        # queued so far (req.X may still be in flight there)
+        mark = getattr(self.ex, "input_ready", None)
+        req.ready = mark(req) if mark is not None else None
         with self._cv:
             if req.arrival == 0.0:
                 req.arrival = self.clock()
@@ -316,6 +335,8 @@ class Server:
                     return
                 req, basis = self._work[slot]
             try:
+                if req.ready is not None:
+                    self.ex.wait_input(stream, req.ready)
                 with self.ex.stream_context(stream) if stream is not None else _NullCtx():
                     res = self.ex.run(req, basis)
                 self.ex.synchronize(stream)
@@ -430,6 +451,8 @@ class NativeServer:
             q.X = X.data_ptr()
             q.ldx = X.stride(0)
             q.m, q.n = X.shape
+            # X may still be in flight on torch's stream: the request waits for it
+            q.stream = self._torch.cuda.current_stream(X.device).cuda_stream
         t = C.c_int64()
         rc = self.lib.pc_serve_submit(self.h, C.byref(q), C.byref(t))
         if rc != 0:

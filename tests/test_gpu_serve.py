@@ -120,3 +120,26 @@ def test_missing_basis_is_rejected_not_fatal():
         r = s.submit(Request(RECONSTRUCT, basis=99))
         assert s.drain(timeout=10)
     assert isinstance(r.error, KeyError)
+
+
+def test_request_waits_for_its_input_stream():
+    """A request submitted while its input is still being produced on another
+    stream runs on the finished input (its slot stream waits on the event
+    recorded at submission)."""
+    side = torch.cuda.Stream()
+    g = torch.Generator(device="cuda").manual_seed(12)
+    with torch.cuda.stream(side):
+        A = torch.randn(400_000, 64, generator=g, device="cuda", dtype=torch.float64)
+        B = torch.linalg.qr(torch.randn(64, 64, generator=g, device="cuda", dtype=torch.float64))[0]
+        for _ in range(40):
+            A = A @ B
+        X = A.float()
+    with Server(serve.DeviceExecutor("mixed"), slots=2, budget=20e9) as s:
+        with torch.cuda.stream(side):
+            r = s.submit(Request(POD, X=X))
+        assert s.drain(timeout=60)
+    assert r.error is None, r.error
+    side.synchronize()
+    U, S, V = pl.POD.run(X, remove_mean=True, precision="mixed")
+    torch.cuda.synchronize()
+    assert _same(r.result, (U, S, V))

@@ -166,3 +166,32 @@ def test_native_pinned_basis_survives_eviction(pools):
         rec = srv.wait(second, timeout=60)
         assert rec is not None and rec["status"] == 0
         assert srv.stats()["evictions"] == 1
+
+
+def _slow_input(m, n, seed):
+    """X produced on a side stream by a chain of products that takes some
+    milliseconds, returned while that stream may still be running."""
+    side = torch.cuda.Stream()
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    with torch.cuda.stream(side):
+        A = torch.randn(m, n, generator=g, device="cuda", dtype=torch.float64)
+        B = torch.linalg.qr(torch.randn(n, n, generator=g, device="cuda", dtype=torch.float64))[0]
+        for _ in range(40):
+            A = A @ B
+        X = A.float()
+    return side, X
+
+
+def test_native_request_waits_for_its_input_stream():
+    """A request submitted while its input is still being produced on another
+    stream reads the finished input (the slot waits on the submitter's stream)."""
+    with NativeServer(slots=2, budget=20e9, m_max=400_000, n_max=64) as srv:
+        side, X = _slow_input(400_000, 64, 11)
+        with torch.cuda.stream(side):
+            t = srv.submit_pod(X)
+        rec = srv.wait(t, timeout=60)
+        assert rec is not None and rec["status"] == 0, rec
+        U, S, VT = srv.basis(rec["basis"])
+        side.synchronize()
+        Ud, Sd, Vd = _direct_pod(X, "mixed")
+        assert torch.equal(U, Ud) and torch.equal(S, Sd) and torch.equal(VT, Vd)
