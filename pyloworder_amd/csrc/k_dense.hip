@@ -24,6 +24,45 @@ PC_DEV void upper_tile(int t, int nt, int& I, int& J) {
   J = i + t;
 }
 
+// DPP move of a double (both 32-bit halves), lanes that the row mask leaves
+// out return `old`
+template <int CTRL, int ROWS>
+PC_DEV double dpp_f64(double x, double old) {
+  const long long xi = __builtin_bit_cast(long long, x), oi = __builtin_bit_cast(long long, old);
+  const int lo = __builtin_amdgcn_update_dpp(int(oi), int(xi), CTRL, ROWS, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(int(oi >> 32), int(xi >> 32), CTRL, ROWS, 0xF, false);
+  return __builtin_bit_cast(double, (long long)(unsigned)lo | ((long long)hi << 32));
+}
+PC_DEV double readlane63_f64(double x) {
+  const long long xi = __builtin_bit_cast(long long, x);
+  const int lo = __builtin_amdgcn_readlane(int(xi), 63), hi = __builtin_amdgcn_readlane(int(xi >> 32), 63);
+  return __builtin_bit_cast(double, (long long)(unsigned)lo | ((long long)hi << 32));
+}
+// Sum over the 64 lanes of a wave, returned to every lane: within each row
+// of 16 lanes two quad permutations and two rotations (by 4, by 8) leave the
+// row's sum in all its lanes; row 0 is added into row 1 and row 2 into row 3
+// (row_bcast:15), then row 1 into rows 2 and 3 (row_bcast:31), so lane 63
+// holds every lane once.  Call with the whole wave active.
+PC_DEV double wave_sum_dpp(double v) {
+  v += dpp_f64<0xB1, 0xF>(v, 0.0);   // quad_perm [1,0,3,2]
+  v += dpp_f64<0x4E, 0xF>(v, 0.0);   // quad_perm [2,3,0,1]
+  v += dpp_f64<0x124, 0xF>(v, 0.0);  // row_ror:4
+  v += dpp_f64<0x128, 0xF>(v, 0.0);  // row_ror:8
+  v += dpp_f64<0x142, 0xA>(v, 0.0);  // row_bcast:15 into rows 1, 3
+  v += dpp_f64<0x143, 0xC>(v, 0.0);  // row_bcast:31 into rows 2, 3
+  return readlane63_f64(v);
+}
+// Maximum over the 64 lanes (NaN-free input), returned to every lane
+PC_DEV double wave_max_dpp(double v) {
+  v = fmax(v, dpp_f64<0xB1, 0xF>(v, v));
+  v = fmax(v, dpp_f64<0x4E, 0xF>(v, v));
+  v = fmax(v, dpp_f64<0x124, 0xF>(v, v));
+  v = fmax(v, dpp_f64<0x128, 0xF>(v, v));
+  v = fmax(v, dpp_f64<0x142, 0xA>(v, v));
+  v = fmax(v, dpp_f64<0x143, 0xC>(v, v));
+  return readlane63_f64(v);
+}
+
 }  // namespace dense
 
 // ---------------------------------------------------------------------------
@@ -157,10 +196,8 @@ __global__ __launch_bounds__(NT) void k_pchol_fast(const double* __restrict__ G,
   using namespace dense;
   __shared__ double P[PW * PLD];
   __shared__ double dg[NMAX];
-  __shared__ double wv_best[NT / 64];
-  __shared__ int wv_idx[NT / 64];
-  __shared__ int pv_idx, stop_s;
-  __shared__ double pv_val;
+  __shared__ double wv_best[2 * (NT / 64)];
+  __shared__ int wv_idx[2 * (NT / 64)];
   __shared__ double sh[32];
   const int tid = threadIdx.x, w = tid >> 6, l = tid & 63, lr = l & 15, lk = l >> 4;
   for (int i = w; i < n; i += NT / 64)
@@ -177,58 +214,57 @@ __global__ __launch_bounds__(NT) void k_pchol_fast(const double* __restrict__ G,
     return;
   }
   const double thr = tol * dmax;
-  if (tid == 0) stop_s = 0;
   int rank = n;
+  // Each of the first n threads owns one index q = tid and keeps its Schur
+  // diagonal dq in a register (-2 once pivoted, -3 for threads beyond n).
+  // A pivot step is one barrier: the owners form their entry of the new row
+  // and update dq, and every wave leaves its (max dq, lowest index at that
+  // max) in one of two LDS slots, read by all threads after the barrier to
+  // pick the next pivot (largest dq, ties to the lower index).
+  const int nw = (n + 63) >> 6;
+  const bool own = tid < n;
+  double dq = own ? dg[tid] : -3.0;
+  auto wave_argmax = [&](int slot) {
+    const double m = wave_max_dpp(dq);
+    const uint64_t at = __ballot(dq == m);
+    if (l == 0) {
+      wv_best[slot * (NT / 64) + w] = m;
+      wv_idx[slot * (NT / 64) + w] = 64 * w + (at ? __builtin_ctzll(at) : 63);
+    }
+  };
+  wave_argmax(0);
   __syncthreads();
+  int slot = 0;
   for (int k0 = 0; k0 < n; k0 += PW) {
     const int bs = min(PW, n - k0);
     int jj = 0;
     for (; jj < bs; jj++) {
       const int j = k0 + jj;
-      // argmax of the Schur diagonal over unpivoted indices (ties: lower index)
-      double best = -1.0;
-      int bi = n;
-      for (int q = tid; q < n; q += NT) {
-        double d = dg[q];
-        if (d > best) { best = d; bi = q; }
+      double b = wv_best[slot * (NT / 64)];
+      int p = wv_idx[slot * (NT / 64)];
+      for (int q = 1; q < nw; q++) {
+        const double v = wv_best[slot * (NT / 64) + q];
+        if (v > b) { b = v; p = wv_idx[slot * (NT / 64) + q]; }
       }
-      for (int o = 32; o > 0; o >>= 1) {
-        double ob = __shfl_xor(best, o);
-        int oi = __shfl_xor(bi, o);
-        if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
-      }
-      if (l == 0) { wv_best[w] = best; wv_idx[w] = bi; }
-      __syncthreads();
-      if (tid == 0) {
-        double b = -1.0;
-        int ii = n;
-        for (int q = 0; q < NT / 64; q++)
-          if (wv_best[q] > b || (wv_best[q] == b && wv_idx[q] < ii)) { b = wv_best[q]; ii = wv_idx[q]; }
-        pv_idx = ii;
-        pv_val = b;
-        if (!(b > thr) || ii >= n) stop_s = 1;
-        else perm[j] = ii;
-      }
-      __syncthreads();
-      if (stop_s) break;
-      const int p = pv_idx;
-      const double r = sqrt(pv_val);
-      for (int q = tid; q < n; q += NT) {
+      if (!(b > thr) || p >= n) break;  // the same decision in every thread
+      if (tid == 0) perm[j] = p;
+      if (own) {
+        const int q = tid;
         double v;
+        const double r = sqrt(b);
         if (q == p) v = r;
-        else if (dg[q] < 0.0) v = 0.0;  // already pivoted
+        else if (dq < 0.0) v = 0.0;  // already pivoted
         else {
-          double s = p <= q ? A[int64_t(p) * n + q] : A[int64_t(q) * n + p];
-          for (int t = 0; t < jj; t++) s -= P[t * PLD + p] * P[t * PLD + q];
-          v = s / r;
+          double sacc = p <= q ? A[int64_t(p) * n + q] : A[int64_t(q) * n + p];
+          for (int t = 0; t < jj; t++) sacc -= P[t * PLD + p] * P[t * PLD + q];
+          v = sacc / r;
         }
         P[jj * PLD + q] = v;
+        if (q == p) dq = -2.0;
+        else if (dq >= 0.0) dq = fmax(dq - v * v, 0.0);
       }
-      __syncthreads();
-      for (int q = tid; q < n; q += NT) {
-        if (q == p) dg[q] = -2.0;
-        else if (dg[q] >= 0.0) dg[q] = fmax(dg[q] - P[jj * PLD + q] * P[jj * PLD + q], 0.0);
-      }
+      slot ^= 1;
+      wave_argmax(slot);
       __syncthreads();
     }
     if (jj < bs) rank = k0 + jj;
@@ -265,6 +301,8 @@ __global__ __launch_bounds__(NT) void k_pchol_fast(const double* __restrict__ G,
   // rows at and beyond the rank are zero; perm completes with the unpivoted indices
   for (int t = rank + w; t < n; t += NT / 64)
     for (int q = l; q < n; q += 64) R[int64_t(t) * ldr + q] = 0.0;
+  if (own) dg[tid] = dq;
+  __syncthreads();
   if (tid == 0) {
     int k = rank;
     for (int q = 0; q < n; q++)
@@ -294,6 +332,10 @@ __global__ __launch_bounds__(NT) void k_pchol_fast(const double* __restrict__ G,
 // I - V T V^T (compact WY, T upper) updates the trailing columns with MFMA
 // (X = V^T A_t, X <- T^T X, A_t -= V X); Q is accumulated backwards the
 // same way.  work: n*n (the factored matrix) + n (tau) + 32*32*ceil(n/32) (T).
+// Latency is what costs here (one workgroup): wave sums use DPP, a column
+// of the panel costs one barrier (its scaling waits for the panel's end),
+// V^T V and the products with T run on MFMA, and the K-loops over global
+// memory load eight k-steps ahead of their products.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(NT) void k_house_qr_fast(const double* __restrict__ A, int lda, int n,
                                                       double* __restrict__ Q, int ldq, double* __restrict__ R,
@@ -302,27 +344,123 @@ __global__ __launch_bounds__(NT) void k_house_qr_fast(const double* __restrict__
   __shared__ double Pn[PW * PLD];  // panel columns (column c at Pn[c * PLD], rows relative to the panel start)
   __shared__ double Xs[PW * PLD];  // X = V^T A_t (row i at Xs[i * PLD]); also V^T V
   __shared__ double Ts[PW * (PW + 1)];
-  __shared__ double tau_s[PW];
+  __shared__ double tau_s[PW], sc_s[PW], beta_s[PW];
+  constexpr int NW = NT / 64;
   const int tid = threadIdx.x, w = tid >> 6, l = tid & 63, lr = l & 15, lk = l >> 4;
   double* Wm = work;
   double* tau = work + int64_t(n) * n;
   double* Tg = tau + n;
-  for (int i = w; i < n; i += NT / 64)
+  for (int i = w; i < n; i += NW)
     for (int j = l; j < n; j += 64) Wm[int64_t(i) * n + j] = A[int64_t(i) * lda + j];
   __syncthreads();
   const int npan = (n + PW - 1) / PW;
   // V element (row i of the panel, column c), unit diagonal, zero above
   auto vget = [&](int i, int c) -> double { return i < c ? 0.0 : (i == c ? 1.0 : Pn[c * PLD + i]); };
+  // X = V^T B for a K = mr tall B (row k, column col at B[k * ldb]), two
+  // tile rows of 16 x ntl tiles, KD k-steps of loads ahead of the products
+  constexpr int KD = 4;
+  auto vt_times = [&](const double* B, int64_t ldb, int mr, int bs, int ncols, int ntl) {
+    for (int t = w; t < 2 * ntl; t += NW) {
+      const int I = t / ntl, J = t % ntl, col = 16 * J + lr, ca = 16 * I + lr;
+      d4s acc = {0.0, 0.0, 0.0, 0.0};
+      const double* Bc = B + col;
+      for (int k0 = 0; k0 < mr; k0 += 4 * KD) {
+        double bv[KD];
+#pragma unroll
+        for (int u = 0; u < KD; u++) {
+          const int k = k0 + 4 * u + lk;
+          bv[u] = (k < mr && col < ncols) ? Bc[int64_t(k) * ldb] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < KD; u++) {
+          const int k = k0 + 4 * u + lk;
+          acc = mfma64s((k < mr && ca < bs) ? vget(k, ca) : 0.0, bv[u], acc);
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < 4; r++) Xs[(16 * I + 4 * r + lk) * PLD + col] = acc[r];
+    }
+  };
+  // X <- T^T X (trans) or T X, on MFMA; every wave holds its (at most two)
+  // result tiles in registers until all have read X
+  auto t_times = [&](bool trans, int ntl) {
+    d4s y[2];
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+      const int t = w + u * NW;
+      y[u] = d4s{0.0, 0.0, 0.0, 0.0};
+      if (t < 2 * ntl) {
+        const int I = t / ntl, J = t % ntl, col = 16 * J + lr, i = 16 * I + lr;
+#pragma unroll
+        for (int k0 = 0; k0 < PW; k0 += 4) {
+          const int k = k0 + lk;
+          const double tv = trans ? Ts[k * (PW + 1) + i] : Ts[i * (PW + 1) + k];
+          y[u] = mfma64s(tv, Xs[k * PLD + col], y[u]);
+        }
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+      const int t = w + u * NW;
+      if (t < 2 * ntl) {
+        const int I = t / ntl, J = t % ntl, col = 16 * J + lr;
+#pragma unroll
+        for (int r = 0; r < 4; r++) Xs[(16 * I + 4 * r + lk) * PLD + col] = y[u][r];
+      }
+    }
+    __syncthreads();
+  };
+  // C -= V X for an mr x ncols block C (row-major, ldc), two tiles per wave
+  // at a time: both tiles' loads go out before their products
+  auto minus_vx = [&](double* C, int64_t ldc, int mr, int bs, int ncols, int nml, int ntl) {
+    const int ntiles = nml * ntl;
+    for (int t = w; t < ntiles; t += 2 * NW) {
+      d4s acc[2];
+#pragma unroll
+      for (int u = 0; u < 2; u++) {
+        const int tt = t + u * NW, I = tt / ntl, J = tt % ntl, col = 16 * J + lr;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int row = 16 * I + 4 * r + lk;
+          acc[u][r] = (tt < ntiles && row < mr && col < ncols) ? -C[int64_t(row) * ldc + col] : 0.0;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 2; u++) {
+        const int tt = t + u * NW, I = tt / ntl, J = tt % ntl, col = 16 * J + lr, arow = 16 * I + lr;
+        if (tt < ntiles) {
+#pragma unroll
+          for (int k0 = 0; k0 < PW; k0 += 4) {
+            const int kc = k0 + lk;
+            const double av = (arow < mr && kc < bs) ? vget(arow, kc) : 0.0;
+            acc[u] = mfma64s(av, Xs[kc * PLD + col], acc[u]);
+          }
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 2; u++) {
+        const int tt = t + u * NW, I = tt / ntl, J = tt % ntl, col = 16 * J + lr;
+        if (tt < ntiles) {
+#pragma unroll
+          for (int r = 0; r < 4; r++) {
+            const int row = 16 * I + 4 * r + lk;
+            if (row < mr && col < ncols) C[int64_t(row) * ldc + col] = -acc[u][r];
+          }
+        }
+      }
+    }
+  };
   for (int pp = 0; pp < npan; pp++) {
     const int c0 = pp * PW, bs = min(PW, n - c0), mr = n - c0;
-    for (int c = w; c < PW; c += NT / 64)
+    for (int c = w; c < PW; c += NW)
       for (int i = l; i < PLD; i += 64) Pn[c * PLD + i] = (c < bs && i < mr) ? Wm[int64_t(c0 + i) * n + c0 + c] : 0.0;
     __syncthreads();
     for (int j = 0; j < bs; j++) {
       // every wave forms the reflector of column j itself (same arithmetic, same result)
       double s = 0.0;
       for (int i = j + 1 + l; i < mr; i += 64) s += Pn[j * PLD + i] * Pn[j * PLD + i];
-      for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+      s = wave_sum_dpp(s);
       const double al = Pn[j * PLD + j];
       double tj, sc, beta;
       if (s == 0.0) {
@@ -333,14 +471,15 @@ __global__ __launch_bounds__(NT) void k_house_qr_fast(const double* __restrict__
         tj = (beta - al) / beta;
         sc = 1.0 / (al - beta);
       }
-      // apply H_j to the panel columns c > j
-      for (int c = j + 1 + w; c < bs; c += NT / 64) {
+      if (tid == 0) { tau_s[j] = tj; sc_s[j] = sc; beta_s[j] = beta; }
+      // apply H_j to the panel columns c > j (column j itself is read, not written)
+      for (int c = j + 1 + w; c < bs; c += NW) {
         double d = 0.0;
         for (int i = j + l; i < mr; i += 64) {
           double v = i == j ? 1.0 : Pn[j * PLD + i] * sc;
           d += v * Pn[c * PLD + i];
         }
-        for (int o = 32; o > 0; o >>= 1) d += __shfl_xor(d, o);
+        d = wave_sum_dpp(d);
         double f = tj * d;
         for (int i = j + l; i < mr; i += 64) {
           double v = i == j ? 1.0 : Pn[j * PLD + i] * sc;
@@ -348,30 +487,38 @@ __global__ __launch_bounds__(NT) void k_house_qr_fast(const double* __restrict__
         }
       }
       __syncthreads();
-      if (w == 0) {
-        for (int i = j + 1 + l; i < mr; i += 64) Pn[j * PLD + i] *= sc;
-        if (l == 0) { Pn[j * PLD + j] = beta; tau_s[j] = tj; }
-      }
-      __syncthreads();
     }
-    // V^T V (bs x bs) into Xs, then T (forward, columnwise)
-    for (int idx = tid; idx < PW * PW; idx += NT) {
-      int a = idx >> 5, b = idx & 31;
-      double s = 0.0;
-      if (a < b && b < bs)
-        for (int i = b; i < mr; i++) s += vget(i, a) * vget(i, b);
-      Xs[a * PLD + b] = s;
+    // the panel's reflectors in place: v below the diagonal, beta on it
+    for (int c = w; c < bs; c += NW) {
+      const double sc = sc_s[c];
+      for (int i = c + 1 + l; i < mr; i += 64) Pn[c * PLD + i] *= sc;
+      if (l == 0) Pn[c * PLD + c] = beta_s[c];
+    }
+    __syncthreads();
+    // V^T V (bs x bs) into Xs: four 16 x 16 tiles, one wave each, two
+    // accumulators over alternating k-steps
+    if (w < 4) {
+      const int I = w >> 1, J = w & 1, ca = 16 * I + lr, cb = 16 * J + lr;
+      d4s a0 = {0.0, 0.0, 0.0, 0.0}, a1 = {0.0, 0.0, 0.0, 0.0};
+      for (int k0 = 0; k0 < mr; k0 += 8) {
+        const int k = k0 + lk, k2 = k0 + 4 + lk;
+        const double x0 = (k < mr && ca < bs) ? vget(k, ca) : 0.0;
+        const double y0 = (k < mr && cb < bs) ? vget(k, cb) : 0.0;
+        const double x1 = (k2 < mr && ca < bs) ? vget(k2, ca) : 0.0;
+        const double y1 = (k2 < mr && cb < bs) ? vget(k2, cb) : 0.0;
+        a0 = mfma64s(x0, y0, a0);
+        a1 = mfma64s(x1, y1, a1);
+      }
+#pragma unroll
+      for (int r = 0; r < 4; r++) Xs[(16 * I + 4 * r + lk) * PLD + cb] = a0[r] + a1[r];
     }
     __syncthreads();
     if (w == 0) {
       for (int j = 0; j < bs; j++) {
         // T[0:j, j] = -tau_j T[0:j, 0:j] (V^T v_j)[0:j]
-        double y = l < j ? Xs[l * PLD + j] : 0.0;
         double acc = 0.0;
-        for (int t = 0; t < j; t++) {
-          double yt = __shfl(y, t);
-          if (l <= t && l < j) acc += Ts[l * (PW + 1) + t] * yt;
-        }
+        if (l < j)
+          for (int t = l; t < j; t++) acc += Ts[l * (PW + 1) + t] * Xs[t * PLD + j];
         if (l < j) Ts[l * (PW + 1) + j] = -tau_s[j] * acc;
         if (l == j) Ts[j * (PW + 1) + j] = tau_s[j];
         if (l > j && l < PW) Ts[l * (PW + 1) + j] = 0.0;
@@ -384,7 +531,7 @@ __global__ __launch_bounds__(NT) void k_house_qr_fast(const double* __restrict__
     }
     __syncthreads();
     // store the panel (R part and V) and T
-    for (int c = w; c < bs; c += NT / 64)
+    for (int c = w; c < bs; c += NW)
       for (int i = l; i < mr; i += 64) Wm[int64_t(c0 + i) * n + c0 + c] = Pn[c * PLD + i];
     for (int idx = tid; idx < PW * PW; idx += NT) Tg[int64_t(pp) * PW * PW + idx] = Ts[(idx >> 5) * (PW + 1) + (idx & 31)];
     if (tid < bs) tau[c0 + tid] = tau_s[tid];
@@ -392,115 +539,33 @@ __global__ __launch_bounds__(NT) void k_house_qr_fast(const double* __restrict__
     const int c1 = c0 + bs, nt = n - c1;
     if (nt > 0) {
       const int ntl = (nt + 15) >> 4, nml = (mr + 15) >> 4;
-      // X = V^T A_t  (2 x ntl tiles, K = mr)
-      for (int t = w; t < 2 * ntl; t += NT / 64) {
-        const int I = t / ntl, J = t % ntl, col = 16 * J + lr;
-        d4s acc = {0.0, 0.0, 0.0, 0.0};
-        for (int k0 = 0; k0 < mr; k0 += 4) {
-          int k = k0 + lk;
-          double a = k < mr ? vget(k, 16 * I + lr) : 0.0;
-          double bb = (k < mr && col < nt) ? Wm[int64_t(c0 + k) * n + c1 + col] : 0.0;
-          acc = mfma64s(a, bb, acc);
-        }
-#pragma unroll
-        for (int r = 0; r < 4; r++) Xs[(16 * I + 4 * r + lk) * PLD + col] = acc[r];
-      }
+      vt_times(Wm + int64_t(c0) * n + c1, n, mr, PW, nt, ntl);
       __syncthreads();
-      // X <- T^T X (column by column in registers)
-      // (in place per column, downwards: row i only needs rows <= i)
-      for (int c = tid; c < nt; c += NT) {
-        for (int i = PW - 1; i >= 0; i--) {
-          double s = 0.0;
-          for (int t = 0; t <= i; t++) s += Ts[t * (PW + 1) + i] * Xs[t * PLD + c];
-          Xs[i * PLD + c] = s;
-        }
-      }
-      __syncthreads();
-      // A_t -= V X  (nml x ntl tiles, K = 32)
-      for (int t = w; t < nml * ntl; t += NT / 64) {
-        const int I = t / ntl, J = t % ntl, col = 16 * J + lr;
-        d4s acc;
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-          int row = 16 * I + 4 * r + lk;
-          acc[r] = (row < mr && col < nt) ? -Wm[int64_t(c0 + row) * n + c1 + col] : 0.0;
-        }
-        const int arow = 16 * I + lr;
-#pragma unroll
-        for (int k0 = 0; k0 < PW; k0 += 4) {
-          double a = arow < mr ? vget(arow, k0 + lk) : 0.0;
-          acc = mfma64s(a, Xs[(k0 + lk) * PLD + col], acc);
-        }
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-          int row = 16 * I + 4 * r + lk;
-          if (row < mr && col < nt) Wm[int64_t(c0 + row) * n + c1 + col] = -acc[r];
-        }
-      }
+      t_times(true, ntl);
+      minus_vx(Wm + int64_t(c0) * n + c1, n, mr, PW, nt, nml, ntl);
     }
     __syncthreads();
   }
   // R
-  for (int i = w; i < n; i += NT / 64)
+  for (int i = w; i < n; i += NW)
     for (int j = l; j < n; j += 64) R[int64_t(i) * ldr + j] = i <= j ? Wm[int64_t(i) * n + j] : 0.0;
   // Q = H_0 ... H_{n-1} I, accumulated backwards over the panels
-  for (int i = w; i < n; i += NT / 64)
+  for (int i = w; i < n; i += NW)
     for (int j = l; j < n; j += 64) Q[int64_t(i) * ldq + j] = i == j ? 1.0 : 0.0;
   __syncthreads();
   for (int pp = npan - 1; pp >= 0; pp--) {
     const int c0 = pp * PW, bs = min(PW, n - c0), mr = n - c0, nq = n - c0;
-    for (int c = w; c < PW; c += NT / 64)
+    for (int c = w; c < PW; c += NW)
       for (int i = l; i < PLD; i += 64)
         Pn[c * PLD + i] = (c < bs && i < mr && i > c) ? Wm[int64_t(c0 + i) * n + c0 + c] : 0.0;
     for (int idx = tid; idx < PW * PW; idx += NT) Ts[(idx >> 5) * (PW + 1) + (idx & 31)] = Tg[int64_t(pp) * PW * PW + idx];
     __syncthreads();
     const int ntl = (nq + 15) >> 4, nml = (mr + 15) >> 4;
-    // X = V^T Q_sub (K = mr)
-    for (int t = w; t < 2 * ntl; t += NT / 64) {
-      const int I = t / ntl, J = t % ntl, col = 16 * J + lr;
-      d4s acc = {0.0, 0.0, 0.0, 0.0};
-      for (int k0 = 0; k0 < mr; k0 += 4) {
-        int k = k0 + lk;
-        double a = (k < mr && 16 * I + lr < bs) ? vget(k, 16 * I + lr) : 0.0;
-        double bb = (k < mr && col < nq) ? Q[int64_t(c0 + k) * ldq + c0 + col] : 0.0;
-        acc = mfma64s(a, bb, acc);
-      }
-#pragma unroll
-      for (int r = 0; r < 4; r++) Xs[(16 * I + 4 * r + lk) * PLD + col] = acc[r];
-    }
+    double* Qs = Q + int64_t(c0) * ldq + c0;
+    vt_times(Qs, ldq, mr, bs, nq, ntl);
     __syncthreads();
-    // X <- T X
-    // (in place per column, upwards: row i only needs rows >= i)
-    for (int c = tid; c < nq; c += NT) {
-      for (int i = 0; i < PW; i++) {
-        double s = 0.0;
-        for (int t = i; t < PW; t++) s += Ts[i * (PW + 1) + t] * Xs[t * PLD + c];
-        Xs[i * PLD + c] = s;
-      }
-    }
-    __syncthreads();
-    // Q_sub -= V X
-    for (int t = w; t < nml * ntl; t += NT / 64) {
-      const int I = t / ntl, J = t % ntl, col = 16 * J + lr;
-      d4s acc;
-#pragma unroll
-      for (int r = 0; r < 4; r++) {
-        int row = 16 * I + 4 * r + lk;
-        acc[r] = (row < mr && col < nq) ? -Q[int64_t(c0 + row) * ldq + c0 + col] : 0.0;
-      }
-      const int arow = 16 * I + lr;
-#pragma unroll
-      for (int k0 = 0; k0 < PW; k0 += 4) {
-        int kc = k0 + lk;
-        double a = (arow < mr && kc < bs) ? vget(arow, kc) : 0.0;
-        acc = mfma64s(a, Xs[(k0 + lk) * PLD + col], acc);
-      }
-#pragma unroll
-      for (int r = 0; r < 4; r++) {
-        int row = 16 * I + 4 * r + lk;
-        if (row < mr && col < nq) Q[int64_t(c0 + row) * ldq + c0 + col] = -acc[r];
-      }
-    }
+    t_times(false, ntl);
+    minus_vx(Qs, ldq, mr, bs, nq, nml, ntl);
     __syncthreads();
   }
 }
