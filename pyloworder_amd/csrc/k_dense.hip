@@ -240,12 +240,19 @@ __global__ __launch_bounds__(NT) void k_pchol_fast(const double* __restrict__ G,
     int jj = 0;
     for (; jj < bs; jj++) {
       const int j = k0 + jj;
-      double b = wv_best[slot * (NT / 64)];
-      int p = wv_idx[slot * (NT / 64)];
-      for (int q = 1; q < nw; q++) {
-        const double v = wv_best[slot * (NT / 64) + q];
-        if (v > b) { b = v; p = wv_idx[slot * (NT / 64) + q]; }
+      // all candidates are read at once (n <= 256: at most four waves own indices)
+      double cb[NMAX / 64];
+      int ci[NMAX / 64];
+#pragma unroll
+      for (int q = 0; q < NMAX / 64; q++) {
+        cb[q] = wv_best[slot * (NT / 64) + q];
+        ci[q] = wv_idx[slot * (NT / 64) + q];
       }
+      double b = cb[0];
+      int p = ci[0];
+#pragma unroll
+      for (int q = 1; q < NMAX / 64; q++)
+        if (q < nw && cb[q] > b) { b = cb[q]; p = ci[q]; }
       if (!(b > thr) || p >= n) break;  // the same decision in every thread
       if (tid == 0) perm[j] = p;
       if (own) {
@@ -256,6 +263,7 @@ __global__ __launch_bounds__(NT) void k_pchol_fast(const double* __restrict__ G,
         else if (dq < 0.0) v = 0.0;  // already pivoted
         else {
           double sacc = p <= q ? A[int64_t(p) * n + q] : A[int64_t(q) * n + p];
+#pragma unroll 8
           for (int t = 0; t < jj; t++) sacc -= P[t * PLD + p] * P[t * PLD + q];
           v = sacc / r;
         }
@@ -513,21 +521,51 @@ __global__ __launch_bounds__(NT) void k_house_qr_fast(const double* __restrict__
       for (int r = 0; r < 4; r++) Xs[(16 * I + 4 * r + lk) * PLD + cb] = a0[r] + a1[r];
     }
     __syncthreads();
+    // T of the compact WY form.  T^-1 = diag(1 / tau) + strictly upper
+    // (V^T V) (Puglisi 1992), so in 16 x 16 blocks T11 and T22 come from back
+    // substitution, one lane per column in registers: T_jj = tau_j,
+    // T_ij = -tau_i sum_{k = i+1..j} (V^T V)_ik T_kj; multiplying by tau_i
+    // (not dividing by 1 / tau_i) keeps a trivial reflector (tau_i = 0) exact,
+    // its row and column of T zero.  Then T12 = -T11 (V^T V)_12 T22 on MFMA.
     if (w == 0) {
-      for (int j = 0; j < bs; j++) {
-        // T[0:j, j] = -tau_j T[0:j, 0:j] (V^T v_j)[0:j]
-        double acc = 0.0;
-        if (l < j)
-          for (int t = l; t < j; t++) acc += Ts[l * (PW + 1) + t] * Xs[t * PLD + j];
-        if (l < j) Ts[l * (PW + 1) + j] = -tau_s[j] * acc;
-        if (l == j) Ts[j * (PW + 1) + j] = tau_s[j];
-        if (l > j && l < PW) Ts[l * (PW + 1) + j] = 0.0;
+      auto wave_sync = [] {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      };
+      {
+        // lane jl of half-wave `base / 16` owns column jl of block (base, base);
+        // its column lives in Ts (row stride PW + 1), rows from the bottom up
+        const int base = 16 * ((l >> 4) & 1), jl = l & 15;
+        double* tc = Ts + base * (PW + 1) + base + jl;
+#pragma unroll 1
+        for (int i = 15; i >= 0; i--) {
+          double acc = 0.0;
+#pragma unroll
+          for (int k = 0; k < 16; k++)
+            if (k > i && k <= jl) acc += Xs[(base + i) * PLD + base + k] * tc[k * (PW + 1)];
+          const double ti = base + i < bs ? tau_s[base + i] : 0.0;
+          const double v = i == jl ? ti : (i < jl ? -ti * acc : 0.0);
+          if (l < 32) tc[i * (PW + 1)] = v;
+        }
+        if (l < 16)
+          for (int i = 0; i < 16; i++) Ts[(16 + i) * (PW + 1) + jl] = 0.0;
       }
-      for (int j = bs; j < PW; j++)
-        if (l < PW) Ts[l * (PW + 1) + j] = 0.0;
+      wave_sync();
+      // W = (V^T V)_12 T22 into the lower-left block of Xs (not read again)
+      d4s acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int k0 = 0; k0 < 16; k0 += 4)
+        acc = mfma64s(Xs[lr * PLD + 16 + k0 + lk], Ts[(16 + k0 + lk) * (PW + 1) + 16 + lr], acc);
+#pragma unroll
+      for (int r = 0; r < 4; r++) Xs[(16 + 4 * r + lk) * PLD + lr] = acc[r];
+      wave_sync();
+      acc = d4s{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int k0 = 0; k0 < 16; k0 += 4)
+        acc = mfma64s(Ts[lr * (PW + 1) + k0 + lk], Xs[(16 + k0 + lk) * PLD + lr], acc);
+#pragma unroll
+      for (int r = 0; r < 4; r++) Ts[(4 * r + lk) * (PW + 1) + 16 + lr] = -acc[r];
     }
     __syncthreads();
     // store the panel (R part and V) and T
