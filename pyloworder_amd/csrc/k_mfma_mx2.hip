@@ -667,7 +667,8 @@ __global__ __launch_bounds__(NT2, 1) void k_zgram_mx2(const float* __restrict__ 
                                                       const double* __restrict__ mean,
                                                       const double* __restrict__ var, const h8* __restrict__ Wf,
                                                       const float* __restrict__ wscale, int64_t tpc,
-                                                      double* __restrict__ part, int64_t zx = 0, int64_t zb = 0) {
+                                                      double* __restrict__ part, int upper, int64_t zx = 0,
+                                                      int64_t zb = 0) {
   X = zoff(X, zx);
   mean = zoff(mean, zb);
   var = zoff(var, zb);
@@ -684,7 +685,13 @@ __global__ __launch_bounds__(NT2, 1) void k_zgram_mx2(const float* __restrict__ 
   __shared__ int rexp[RT::TR];
   __shared__ float red[NW2];
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63, hl = l >> 5, c = l & 31;
-  const int J = w;
+  // Column block of Z this wave forms.  With an upper triangular W1 (upper
+  // != 0: W1 = R1^-1, zero-padded) block J only needs the k-steps of rows
+  // < 32 (J + 1), a share J + 1 of NBLK; when every wave owns a block, waves
+  // 2k and 2k + 1 take blocks k and NBLK - 1 - k, so each pair does the same
+  // share.  The skipped products are exact zeros: results do not change.
+  const int J = __builtin_amdgcn_readfirstlane(NBLK == NW2 && upper ? ((w & 1) ? NBLK - 1 - (w >> 1) : (w >> 1)) : w);
+  const int kend = upper ? (2 * J + 2 < KSW ? 2 * J + 2 : KSW) : KSW;
   const bool own = NBLK >= NW2 || J < NBLK;  // see k_gemm_mx2
   h8 bw[KSW];
   float isc = 1.0f;
@@ -724,8 +731,10 @@ __global__ __launch_bounds__(NT2, 1) void k_zgram_mx2(const float* __restrict__ 
       if (own) {
 #pragma unroll
         for (int ks = 0; ks < KSW; ks++) {
-          z[R] = mfma(frag(Fs, (R * KSW + ks) * 2 + 0, RT::FRAGB, l), bw[ks], z[R]);
-          z[R] = mfma(frag(Fs, (R * KSW + ks) * 2 + 1, RT::FRAGB, l), bw[ks], z[R]);
+          if (ks < kend) {  // wave-uniform
+            z[R] = mfma(frag(Fs, (R * KSW + ks) * 2 + 0, RT::FRAGB, l), bw[ks], z[R]);
+            z[R] = mfma(frag(Fs, (R * KSW + ks) * 2 + 1, RT::FRAGB, l), bw[ks], z[R]);
+          }
         }
 #pragma unroll
         for (int r = 0; r < 16; r++) {
@@ -837,13 +846,13 @@ int launch_gram_mx2(hipStream_t st, const float* X, int64_t ldx, int64_t m, int 
 
 int launch_zgram_mx2(hipStream_t st, const float* X, int64_t ldx, int64_t m, int n, int center, const double* mean,
                      const double* var, const _Float16* Wf, const float* wscale, int chunks, int64_t tpc,
-                     double* part, const PassBatch& bz) {
+                     double* part, const PassBatch& bz, bool upper) {
   return dispatch_nb(n, [&](auto nbc) {
     constexpr int NB = decltype(nbc)::value;
     return dispatch_vc(load_mode(X, ldx, n, bz.zx), center, [&](auto vv, auto cc) {
       hipLaunchKernelGGL((k_zgram_mx2<NB, decltype(vv)::value, decltype(cc)::value>),
                          dim3(unsigned(chunks), 1, unsigned(bz.nz)), dim3(NT2), 0, st, X, ldx, m, n, mean, var,
-                         (const h8*)Wf, wscale, tpc, part, bz.zx, bz.zb);
+                         (const h8*)Wf, wscale, tpc, part, upper ? 1 : 0, bz.zx, bz.zb);
       return 0;
     });
   });

@@ -26,9 +26,11 @@ int mx2_gram_rows(int n);
 int mx2_tile_rows(int n);
 int launch_gram_mx2(hipStream_t st, const float* X, int64_t ldx, int64_t m, int n, int center, double* mean_io,
                     const double* var, int chunks, int64_t tpc, double* part, const PassBatch& bz = PassBatch());
+// upper: Wf is upper triangular (zero rows below each 32-column block's
+// diagonal), so the Z = Ys W products skip the zero k-steps
 int launch_zgram_mx2(hipStream_t st, const float* X, int64_t ldx, int64_t m, int n, int center, const double* mean,
                      const double* var, const _Float16* Wf, const float* wscale, int chunks, int64_t tpc,
-                     double* part, const PassBatch& bz = PassBatch());
+                     double* part, const PassBatch& bz = PassBatch(), bool upper = false);
 int launch_gemm_mx2(hipStream_t st, const float* X, int64_t ldx, int64_t m, int n, int center, const double* mean,
                     const double* var, const _Float16* Wh, const _Float16* Wl, const float* wscale, int nc,
                     int chunks, int64_t tpc, float* C, int64_t ldc, const PassBatch& bz = PassBatch());

@@ -866,7 +866,7 @@ static int launch_streams(pc_ctx* ctx, const Plan& p, const void* A, int64_t lda
       rc = launch_gram_mx2(st, (const float*)A, ldr, p.m, p.nr, center, mean, var, p.chunks, p.tpc, part, bz);
     else
       rc = launch_zgram_mx2(st, (const float*)A, ldr, p.m, p.nr, center, mean, var, b.Wh, b.wscale, p.chunks3,
-                            p.tpc3, part, bz);
+                            p.tpc3, part, bz, /*upper: W1 = R1^-1, zero-padded*/ true);
   } else {
     if (pass == 1)
       rc = launch_gram_f64(st, sdt, A, ldr, p.m, p.nr, center, 0, nullptr, nullptr, center ? mean : nullptr,
