@@ -117,9 +117,15 @@ PC_DEV void gram_tile(double (*S)[NB + PADD], const int* bI, const int* bJ, int 
   }
 }
 
-// Z = S W for the tile (TR x NB times NB x NB), result into Zs.
+// Z = S W for the tile (TR x NB times NB x NB), result into Zs.  Slot s of
+// wave w holds block b = w + 8 s (row block b & 1, 16-column block
+// J = b >> 1 = w / 2 + 4 s).  With an upper triangular W (upper != 0) block J
+// only needs the k-steps kk < 4 (J + 1) (rows < 16 (J + 1)): the k range is
+// cut into segments, segment f running the slots s >= f up to slot f's end,
+// so the loops carry no per-step conditions.  The products skipped are exact
+// zeros: Z does not change.
 template <int NB>
-PC_DEV void zmul_tile(double (*S)[NB + PADD], const double* __restrict__ W, double (*Zs)[NB + PADD]) {
+PC_DEV void zmul_tile(double (*S)[NB + PADD], const double* __restrict__ W, double (*Zs)[NB + PADD], int upper) {
   constexpr int nb = NB / 16;
   constexpr int NBLK = 2 * nb;                    // TR/16 = 2 row blocks
   constexpr int SLZ = (NBLK + 7) / 8;
@@ -128,19 +134,32 @@ PC_DEV void zmul_tile(double (*S)[NB + PADD], const double* __restrict__ W, doub
   d4 acc[SLZ];
 #pragma unroll
   for (int s = 0; s < SLZ; s++) acc[s] = d4{0, 0, 0, 0};
-#pragma unroll 4
-  for (int kk = 0; kk < NB / 4; kk++) {
+  auto step = [&](int kk, int first) {
     const double* wrow = W + int64_t(kk * 4 + kr) * NB;
 #pragma unroll
     for (int s = 0; s < SLZ; s++) {
       int b = w + 8 * s;
-      if (b < NBLK) {
+      if (s >= first && b < NBLK) {
         int R = b & 1, J = b >> 1;
         double a = S[R * 16 + cc][kk * 4 + kr];
         double bb = wrow[J * 16 + cc];
         acc[s] = mfma64(a, bb, acc[s]);
       }
     }
+  };
+  if (NBLK % 8 == 0 && upper) {
+    const int j0 = __builtin_amdgcn_readfirstlane(w >> 1);
+    int kbeg = 0;
+#pragma unroll
+    for (int f = 0; f < SLZ; f++) {
+      const int kend = 4 * (j0 + 4 * f + 1);
+#pragma unroll 4
+      for (int kk = kbeg; kk < kend; kk++) step(kk, f);
+      kbeg = kend;
+    }
+  } else {
+#pragma unroll 4
+    for (int kk = 0; kk < NB / 4; kk++) step(kk, 0);
   }
 #pragma unroll
   for (int s = 0; s < SLZ; s++) {
@@ -160,7 +179,8 @@ template <class TS, int NB, bool ZMODE>
 __global__ __launch_bounds__(NTH) void k_gram_f64(const TS* __restrict__ X, int64_t ldx, int64_t m, int n, int center,
                                                   int stats_in, const double* mean_in, const double* var_in,
                                                   double* mean_out, double* var_out, const double* __restrict__ W,
-                                                  int64_t tiles_per_chunk, double* __restrict__ part, int64_t zx = 0, int64_t zb = 0) {
+                                                  int64_t tiles_per_chunk, double* __restrict__ part, int upper,
+                                                  int64_t zx = 0, int64_t zb = 0) {
   X = zoff(X, zx);
   mean_in = zoff(mean_in, zb);
   var_in = zoff(var_in, zb);
@@ -193,7 +213,7 @@ __global__ __launch_bounds__(NTH) void k_gram_f64(const TS* __restrict__ X, int6
     load_tile<TS, NB>(X, ldx, m, n, t * TR, center, stats_in, mean_in, var_in, mean_out, var_out, S);
     __syncthreads();
     if constexpr (ZMODE) {
-      zmul_tile<NB>(S, W, Zs);
+      zmul_tile<NB>(S, W, Zs, upper);
       __syncthreads();
       gram_tile<NB, SL>(Zs, bI, bJ, nslot, acc);
     } else {
@@ -344,28 +364,28 @@ int f64_tile_rows() { return TR; }
 template <class TS>
 static int launch_gram_t(hipStream_t st, const TS* X, int64_t ldx, int64_t m, int n, int center, int stats_in,
                          const double* mean_in, const double* var_in, double* mean_out, double* var_out,
-                         const double* W, int chunks, int64_t tpc, double* part, const PassBatch& bz) {
+                         const double* W, int chunks, int64_t tpc, double* part, const PassBatch& bz, bool upper) {
   return dispatch_nb(n, [&](auto nbc) {
     constexpr int NB = decltype(nbc)::value;
     const dim3 g(unsigned(chunks), 1, unsigned(bz.nz));
     if (W)
       hipLaunchKernelGGL((k_gram_f64<TS, NB, true>), g, dim3(NTH), 0, st, X, ldx, m, n, center, stats_in, mean_in,
-                         var_in, mean_out, var_out, W, tpc, part, bz.zx, bz.zb);
+                         var_in, mean_out, var_out, W, tpc, part, upper ? 1 : 0, bz.zx, bz.zb);
     else
       hipLaunchKernelGGL((k_gram_f64<TS, NB, false>), g, dim3(NTH), 0, st, X, ldx, m, n, center, stats_in, mean_in,
-                         var_in, mean_out, var_out, W, tpc, part, bz.zx, bz.zb);
+                         var_in, mean_out, var_out, W, tpc, part, 0, bz.zx, bz.zb);
     return 0;
   });
 }
 
 int launch_gram_f64(hipStream_t st, pc_dtype dt, const void* X, int64_t ldx, int64_t m, int n, int center,
                     int stats_in, const double* mean_in, const double* var_in, double* mean_out, double* var_out,
-                    const double* W, int chunks, int64_t tpc, double* part, const PassBatch& bz) {
+                    const double* W, int chunks, int64_t tpc, double* part, const PassBatch& bz, bool upper) {
   if (dt == PC_F64)
     return launch_gram_t<double>(st, (const double*)X, ldx, m, n, center, stats_in, mean_in, var_in, mean_out, var_out,
-                                 W, chunks, tpc, part, bz);
+                                 W, chunks, tpc, part, bz, upper);
   return launch_gram_t<float>(st, (const float*)X, ldx, m, n, center, stats_in, mean_in, var_in, mean_out, var_out, W,
-                              chunks, tpc, part, bz);
+                              chunks, tpc, part, bz, upper);
 }
 
 template <class TS, class TO>

@@ -11,9 +11,12 @@ struct PassBatch {
   int nz = 1;
   int64_t zx = 0, zb = 0, zc = 0;
 };
+// W non-null: the second pass, Gram of Z = Y W; upper: W is upper triangular
+// (zero-padded), so Z skips the k-steps where W is zero
 int launch_gram_f64(hipStream_t st, pc_dtype dt, const void* X, int64_t ldx, int64_t m, int n, int center,
                     int stats_in, const double* mean_in, const double* var_in, double* mean_out, double* var_out,
-                    const double* W, int chunks, int64_t tpc, double* part, const PassBatch& bz = PassBatch());
+                    const double* W, int chunks, int64_t tpc, double* part, const PassBatch& bz = PassBatch(),
+                    bool upper = false);
 int launch_gemm_f64(hipStream_t st, pc_dtype dt, const void* X, int64_t ldx, int64_t m, int n, int center,
                     const double* mean_in, const double* var_in, const double* W, int nc, int chunks, int64_t tpc,
                     void* C, int64_t ldc, const PassBatch& bz = PassBatch());

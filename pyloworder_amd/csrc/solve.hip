@@ -873,7 +873,7 @@ static int launch_streams(pc_ctx* ctx, const Plan& p, const void* A, int64_t lda
                            center == 2 ? var : nullptr, nullptr, p.chunks, p.tpc, part, bz);
     else
       rc = launch_gram_f64(st, sdt, A, ldr, p.m, p.nr, center, 1, mean, var, nullptr, nullptr, b.Wpad, p.chunks,
-                           p.tpc, part, bz);
+                           p.tpc, part, bz, /*upper: W1 = R1^-1, zero-padded*/ true);
   }
   if (rc != 0) return pc_fail(ctx, PC_ERR_SHAPE, "streaming kernel has no variant for n = %d", p.n);
   PC_HIP(ctx, hipGetLastError());
