@@ -34,11 +34,21 @@ __global__ __launch_bounds__(256) void k_pack_w(const double* __restrict__ W, in
   int J = blockIdx.x;
   int ncb = NCB / 32;
   __shared__ float fsc[32];
-  // column maxima
+  __shared__ double pmx[8][32];
+  // column maxima: eight row groups per column, their loads all in flight
+  {
+    const int c = threadIdx.x & 31, g = threadIdx.x >> 5, col = J * 32 + c;
+    double mx = 0.0;
+#pragma unroll 8
+    for (int i = g; i < NB; i += 8) mx = fmax(mx, fabs(W[int64_t(i) * ldw + col]));
+    pmx[g][c] = mx;
+  }
+  __syncthreads();
   if (threadIdx.x < 32) {
     int col = J * 32 + threadIdx.x;
     double mx = 0.0;
-    for (int i = 0; i < NB; i++) mx = fmax(mx, fabs(W[int64_t(i) * ldw + col]));
+#pragma unroll
+    for (int g = 0; g < 8; g++) mx = fmax(mx, pmx[g][threadIdx.x]);
     int e = mx > 0.0 ? scale_exp(float(mx)) : 0;
     if (mx > 0.0 && float(mx) * ldexpf(1.0f, e) >= 16384.0f) e -= 1;  // rounding of mx to float
     e = e > 120 ? 120 : (e < -120 ? -120 : e);

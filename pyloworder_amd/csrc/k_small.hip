@@ -415,7 +415,7 @@ __global__ __launch_bounds__(NT) void k_bjac_init(const T* H, int ldh, int rows,
   __shared__ double sh[32];
   for (int i = threadIdx.x; i < nflags; i += NT) state[i] = 0;
   for (int64_t idx = threadIdx.x; idx < int64_t(cols) * cols; idx += NT) {
-    int i = int(idx / cols), j = int(idx % cols);
+    int j = int(idx / cols), i = int(idx % cols);  // consecutive threads, consecutive addresses
     J[int64_t(j) * ldj + i] = T(i == j ? 1.0 : 0.0);
   }
   double f = 0.0;

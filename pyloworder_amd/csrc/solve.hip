@@ -582,6 +582,7 @@ void launch_q1_rows<cdouble>(hipStream_t st, const Plan& p, const void* A, int64
 // select the d lowest-leverage candidates, build the completion:
 //   Md = I - U_J U_J^H = R^H R,  C = R^-1,  W[:, g:] = -W[:, :g] U_J^H C,
 //   jsel[a] = row index j_a, cnull[a][b] = C[a][b]  (U[j_a][g+b] += C[a][b])
+inline size_t null_build_lds(int n, int ncand) { return size_t(ncand) * sizeof(double) + size_t(n) * sizeof(int); }
 template <class T>
 __global__ __launch_bounds__(1024) void k_null_build(int64_t m, int n, const int* nnull, int ncand, const double* lev,
                                                      const T* urows, T* W, int ldw, int* jsel, T* cnull, T* work,
@@ -597,13 +598,18 @@ __global__ __launch_bounds__(1024) void k_null_build(int64_t m, int n, const int
   int d = nnull[0];
   if (d == 0) return;
   int g = n - d;
-  HIP_DYNAMIC_SHARED(int, pick)  // d <= n candidate indices
+  // dynamic LDS: the candidates' sort keys (ncand doubles), then the picks
+  // (d <= n indices); null_build_lds() gives its size
+  HIP_DYNAMIC_SHARED(double, keys)
+  int* pick = reinterpret_cast<int*>(keys + ncand);
+  for (int c = threadIdx.x; c < ncand; c += 1024) keys[c] = key_asc(lev[c]);
+  __syncthreads();
   // rank candidates by leverage (ties by index), keep the first d
   for (int c = threadIdx.x; c < ncand; c += 1024) {
-    double v = key_asc(lev[c]);
+    double v = keys[c];
     int r = 0;
     for (int e = 0; e < ncand; e++) {
-      double u = key_asc(lev[e]);
+      double u = keys[e];
       r += (u < v) || (u == v && e < c);
     }
     if (r < d) pick[r] = c;
@@ -736,8 +742,10 @@ __global__ __launch_bounds__(256) void k_center_right(T* Hc, int ld, int n, int6
   int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
   T s = T(0.0);
-  for (int j = 0; j < n; j++) s += Hc[int64_t(j) * ld + i];
+#pragma unroll 16
+  for (int j = 0; j < n; j++) s += Hc[int64_t(j) * ld + i];  // loads ahead, sum in order
   s = s * (1.0 / n);
+#pragma unroll 16
   for (int j = 0; j < n; j++) Hc[int64_t(j) * ld + i] -= s;
 }
 
@@ -1168,7 +1176,7 @@ static int core_t(pc_ctx* ctx, const Plan& p, const void* A, int64_t lda, int ce
   // null completion (kernels return at once when nnull == 0)
   int ncand = int(std::min<int64_t>(p.m, 4 * int64_t(n) + 16));
   launch_null_lev<T>(st, p, A, lda, (p.mixed && center == 1) ? 3 : center, mean, var, b, ncand);
-  hipLaunchKernelGGL((k_null_build<T>), dim3(1), dim3(1024), size_t(n) * sizeof(int), st, p.m, n, b.nnull, ncand, b.lev, b.urows, b.Wf, n,
+  hipLaunchKernelGGL((k_null_build<T>), dim3(1), dim3(1024), null_build_lds(n, ncand), st, p.m, n, b.nnull, ncand, b.lev, b.urows, b.Wf, n,
                      b.jsel, b.cnull, nwork, b.status);
   if (mode == CORE_QR) {
     // LAPACK's signs (numpy.linalg.qr): D from the first n rows of Q, then
@@ -1288,7 +1296,7 @@ static int core_batch_t(pc_ctx* ctx, const Plan& p, const pc_problem* pr, int64_
                      b.tmp, n, (const int*)nullptr, T(0.0), b.Wf, n, 0, zb);
   const int ncand = int(std::min<int64_t>(p.m, 4 * int64_t(n) + 16));
   launch_null_lev<T>(st, p, pr[0].A, lda, (p.mixed && center == 1) ? 3 : center, mean, var, b, ncand, nz, zb, d.zA);
-  hipLaunchKernelGGL((k_null_build<T>), dim3(1, 1, z), dim3(1024), size_t(n) * sizeof(int), st, p.m, n, b.nnull, ncand, b.lev, b.urows,
+  hipLaunchKernelGGL((k_null_build<T>), dim3(1, 1, z), dim3(1024), null_build_lds(n, ncand), st, p.m, n, b.nnull, ncand, b.lev, b.urows,
                      b.Wf, n, b.jsel, b.cnull, nwork, b.status, zb);
   to_operand<T>(st, p, b.Wf, n, b, true, nz, zb);
   // ---- P3 (and the null-space fix of each U)
