@@ -635,15 +635,22 @@ __global__ __launch_bounds__(64) void k_trinv_cols(const double* __restrict__ R,
     const int cl = j - c0 < 63 ? j - c0 : 63;  // last column of the stretch
     const int rows = c0 + cl + 1;              // rows 0 .. c0 + cl hold the upper triangle
     wave_sync();  // the previous stretch has been read
-    if (l <= cl)
+    if (l <= cl) {
+#pragma unroll 8
       for (int k = 0; k < rows; k++) Rs[k * 65 + l] = R[int64_t(k) * ldr + c0 + l];
+    }
     wave_sync();
     for (int t = cl; t >= 0; --t) {
       const int i = c0 + t;
       double rv = r[0];
 #pragma unroll
       for (int q = 1; q < 4; q++) rv = g == q ? r[q] : rv;
-      const double wi = __shfl(rv, t) / Rs[i * 65 + t];
+      // lane t's right-hand side (t is wave-uniform: a lane read, not a shuffle)
+      const long long rb = __builtin_bit_cast(long long, rv);
+      const double rt = __builtin_bit_cast(
+          double, (long long)(unsigned)__builtin_amdgcn_readlane(int(rb), t) |
+                      ((long long)__builtin_amdgcn_readlane(int(rb >> 32), t) << 32));
+      const double wi = rt / Rs[i * 65 + t];
 #pragma unroll
       for (int q = 0; q < 4; q++) {
         if (g == q && l == t) w[q] = wi;

@@ -354,24 +354,38 @@ __global__ __launch_bounds__(256) void k_gemm_small(int opA, int opB, int M, int
   int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // ty 0..7
   int i0 = blockIdx.y * 32, j0 = blockIdx.x * 32;
   T acc[4] = {T(0.0), T(0.0), T(0.0), T(0.0)};
-  for (int k0 = 0; k0 < K; k0 += 32) {
-    for (int r = ty; r < 32; r += 8) {
+  // this thread's elements of the next 32-wide k slab of A and B, loaded
+  // while the current slab's products run
+  T va[4], ub[4];
+  auto fetch = [&](int k0) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int r = ty + 8 * q;
       int i = i0 + r, k = k0 + tx;
       T v = T(0.0);
       if (i < M && k < K) {
         if (opA == 0) v = A[int64_t(i) * lda + (acol ? acol[k] : k)];
         else { v = A[int64_t(k) * lda + i]; if (opA == 2) v = conj_(v); }
       }
-      As[r][tx] = v;
+      va[q] = v;
       int kk = k0 + r, j = j0 + tx;
       T u = T(0.0);
       if (kk < K && j < N) {
         if (opB == 0) u = B[int64_t(brow ? brow[kk] : kk) * ldb + j];
         else { u = B[int64_t(j) * ldb + kk]; if (opB == 2) u = conj_(u); }
       }
-      Bs[r][tx] = u;
+      ub[q] = u;
+    }
+  };
+  fetch(0);
+  for (int k0 = 0; k0 < K; k0 += 32) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      As[ty + 8 * q][tx] = va[q];
+      Bs[ty + 8 * q][tx] = ub[q];
     }
     __syncthreads();
+    if (k0 + 32 < K) fetch(k0 + 32);
 #pragma unroll 8
     for (int k = 0; k < 32; k++) {
       T b = Bs[k][tx];
