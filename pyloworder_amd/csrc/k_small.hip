@@ -568,19 +568,21 @@ constexpr int GB = 16;
 constexpr int GC = 2 * GB;
 constexpr int GLD = GC + 1;
 constexpr int GROWS = 256;
+// dstate layout of the Gram-driven steps: [0] negligible squared norm, then
+// from kBjacQOff the nb2 / 2 pair rotations (GC x GC each) and their flags
+constexpr int kBjacQOff = 4;
+constexpr int kBjacDstate = kBjacQOff + (GROWS / GB / 2) * GC * GC + GROWS / GB / 2;
 
 typedef double d4s __attribute__((ext_vector_type(4)));
 PC_DEV d4s mfma64s(double a, double b, d4s c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
 
-__global__ __launch_bounds__(NT) void k_bjac_gstep(double* H, int ldh, int rows, int cols, double* J, int ldj, int nb2,
-                                                   int step, int sweep, int* state, const double* dstate, double tol,
-                                                   int max_inner) {
+__global__ __launch_bounds__(NT) void k_bjac_gstep(const double* H, int ldh, int rows, int cols, int nb2, int step,
+                                                   int sweep, int* state, double* dstate, double tol, int max_inner) {
   __shared__ double Hs[GC * (GROWS + 1)];
   __shared__ double G[GC * GLD], G2[GC * GLD];
   __shared__ double Q[GC * GLD], Q2[GC * GLD];
   __shared__ double Pp[3 * 4 * 256];
   __shared__ int need;
-  if (state[0]) return;
   const int RP = (rows + 15) & ~15;
   const int ld = RP + 1;
   // the launch holds 8 workgroups per pair and only every eighth works:
@@ -588,11 +590,11 @@ __global__ __launch_bounds__(NT) void k_bjac_gstep(double* H, int ldh, int rows,
   if (blockIdx.x & 7) return;
   const int p = blockIdx.x >> 3;
   const int bi = rr_player(p, step, nb2), bj = rr_player(nb2 - 1 - p, step, nb2);
-  const double negl = dstate[0];
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63, lr = l & 15, lk = l >> 4;
   auto gcol = [&](int lc) { return lc < GB ? bi * GB + lc : bj * GB + (lc - GB); };
   {
-    // every load of the pair's columns is issued before the first LDS store
+    // every load of the pair's columns is issued before the first LDS store,
+    // and together with the done flag (a converged iteration discards them)
     constexpr int KL = GC * GROWS / NT;
     double hv[KL];
 #pragma unroll
@@ -600,29 +602,50 @@ __global__ __launch_bounds__(NT) void k_bjac_gstep(double* H, int ldh, int rows,
       const int idx = threadIdx.x + k * NT, lc = idx / RP, i = idx - lc * RP, gc = gcol(lc);
       hv[k] = (idx < GC * RP && gc < cols && i < rows) ? H[int64_t(gc) * ldh + i] : 0.0;
     }
+    if (state[0]) return;
 #pragma unroll
     for (int k = 0; k < KL; k++) {
       const int idx = threadIdx.x + k * NT, lc = idx / RP, i = idx - lc * RP;
       if (idx < GC * RP) Hs[lc * ld + i] = hv[k];
     }
   }
+  const double negl = dstate[0];
   if (threadIdx.x == 0) need = 0;
   __syncthreads();
-  // G = Hs^T Hs: wave w computes output block (bI, bJ) = ((w&3)>>1, w&1) over row quarter q = w>>2
+  // G = Hs^T Hs, symmetric: waves w < 12 form the upper 16 x 16 tiles
+  // t = w >> 2 ((0,0), (0,1), (1,1)) over row quarter q = w & 3, and tile
+  // (0,1) is mirrored into (1,0)
   {
-    const int blk = w & 3, q = w >> 2, bI = blk >> 1, bJ = blk & 1;
+    const int t = w >> 2, q = w & 3, bI = t == 2 ? 1 : 0, bJ = t == 0 ? 0 : 1;
     const int kq = RP / 4;
     const double* ca = Hs + (16 * bI + lr) * ld;
     const double* cb = Hs + (16 * bJ + lr) * ld;
     d4s acc = {0.0, 0.0, 0.0, 0.0};
-    for (int k0 = q * kq; k0 < (q + 1) * kq; k0 += 4) acc = mfma64s(ca[k0 + lk], cb[k0 + lk], acc);
-    if (q > 0)
-      for (int r = 0; r < 4; r++) Pp[((q - 1) * 4 + blk) * 256 + r * 64 + l] = acc[r];
+    if (t < 3) {
+      // the quarter's operands (kq / 4 <= GROWS / 16 k-steps) are read from
+      // LDS before the products, so the products do not wait on each read
+      constexpr int KS = GROWS / 16;
+      double xa[KS], xb[KS];
+#pragma unroll
+      for (int k = 0; k < KS; k++) {
+        const int k0 = q * kq + 4 * k;
+        xa[k] = 4 * k < kq ? ca[k0 + lk] : 0.0;
+        xb[k] = 4 * k < kq ? cb[k0 + lk] : 0.0;
+      }
+#pragma unroll
+      for (int k = 0; k < KS; k++)
+        if (4 * k < kq) acc = mfma64s(xa[k], xb[k], acc);
+      if (q > 0)
+        for (int r = 0; r < 4; r++) Pp[((q - 1) * 3 + t) * 256 + r * 64 + l] = acc[r];
+    }
     __syncthreads();
-    if (q == 0) {
+    if (t < 3 && q == 0) {
       for (int q2 = 0; q2 < 3; q2++)
-        for (int r = 0; r < 4; r++) acc[r] += Pp[(q2 * 4 + blk) * 256 + r * 64 + l];
-      for (int r = 0; r < 4; r++) G[(16 * bI + 4 * r + lk) * GLD + 16 * bJ + lr] = acc[r];
+        for (int r = 0; r < 4; r++) acc[r] += Pp[(q2 * 3 + t) * 256 + r * 64 + l];
+      for (int r = 0; r < 4; r++) {
+        G[(16 * bI + 4 * r + lk) * GLD + 16 * bJ + lr] = acc[r];
+        if (t == 1) G[(16 + lr) * GLD + 4 * r + lk] = acc[r];
+      }
     }
   }
   __syncthreads();
@@ -640,7 +663,12 @@ __global__ __launch_bounds__(NT) void k_bjac_gstep(double* H, int ldh, int rows,
     }
   }
   __syncthreads();
-  if (!need) return;
+  double* qout = dstate + kBjacQOff + int64_t(p) * GC * GC;
+  double* qflag = dstate + kBjacQOff + int64_t(nb2 / 2) * GC * GC + p;
+  if (!need) {
+    if (threadIdx.x == 0) *qflag = 0.0;  // k_bjac_apply leaves this pair as it is
+    return;
+  }
   if (threadIdx.x == 0) atomicOr(&state[1 + sweep], 1);
   // k-th pair (u < v) of round s
   auto pair_of = [&](int k, int s, int& u, int& v) {
@@ -733,40 +761,71 @@ __global__ __launch_bounds__(NT) void k_bjac_gstep(double* H, int ldh, int rows,
     }
     if (!__syncthreads_or(rotated)) break;
   }
-  const double* Qc = Q;
-  // H[:, pair] = Hs Q  (row blocks of 16 rows, one wave each)
-  for (int rb = w; rb < RP / 16; rb += NT / 64) {
-    for (int cB = 0; cB < 2; cB++) {
-      d4s acc = {0.0, 0.0, 0.0, 0.0};
-      for (int k0 = 0; k0 < GC; k0 += 4)
-        acc = mfma64s(Hs[(k0 + lk) * ld + 16 * rb + lr], Qc[(k0 + lk) * GLD + 16 * cB + lr], acc);
-      int gc = gcol(16 * cB + lr);
-      for (int r = 0; r < 4; r++) {
-        int row = 16 * rb + 4 * r + lk;
-        if (row < rows && gc < cols) H[int64_t(gc) * ldh + row] = acc[r];
-      }
-    }
+  // the pair's rotation Q (row-major) for k_bjac_apply, which forms
+  // H[:, pair] Q and J[:, pair] Q over many workgroups
+  for (int idx = threadIdx.x; idx < GC * GC; idx += NT) qout[idx] = Q[(idx / GC) * GLD + idx % GC];
+  if (threadIdx.x == 0) *qflag = 1.0;
+}
+
+// ---------------------------------------------------------------------------
+// Second half of a block step of k_bjac_gstep: H[:, pair] <- H[:, pair] Q and
+// J[:, pair] <- J[:, pair] Q with the pair's rotation Q from dstate.  One
+// workgroup per (pair, 32-row slice), so the FP64 MFMA work of the step is
+// spread over 8 (nb2 / 2) workgroups per pair instead of one; every slice
+// reads and writes only its own rows, and the products are the ones the
+// single-workgroup version formed (same k order).
+// ---------------------------------------------------------------------------
+constexpr int kBjacSlices = GROWS / 32;
+__global__ __launch_bounds__(256) void k_bjac_apply(double* H, int ldh, int rows, int cols, double* J, int ldj,
+                                                    int nb2, int step, const int* state, const double* dstate) {
+  __shared__ double Qs[GC * GLD];
+  const int p = blockIdx.x / kBjacSlices, sl = blockIdx.x % kBjacSlices;
+  const int bi = rr_player(p, step, nb2), bj = rr_player(nb2 - 1 - p, step, nb2);
+  auto gcol = [&](int lc) { return lc < GB ? bi * GB + lc : bj * GB + (lc - GB); };
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63, lr = l & 15, lk = l >> 4;
+  // wave w: matrix w >> 1 (0 H, 1 J), 16-row block w & 1 of the slice, both
+  // 16-column blocks of the pair.  Every global read (flags, Q, the slice)
+  // is issued before the first test, so the launch waits on memory once.
+  double* M = (w >> 1) ? J : H;
+  const int ld = (w >> 1) ? ldj : ldh, nr = (w >> 1) ? cols : rows;
+  const int row = 32 * sl + 16 * (w & 1);
+  const double* qin = dstate + kBjacQOff + int64_t(p) * GC * GC;
+  double av[GC / 4], qv[GC * GC / 256];
+#pragma unroll
+  for (int kk = 0; kk < GC / 4; kk++) {
+    const int gc = gcol(4 * kk + lk);
+    av[kk] = (gc < cols && row + lr < nr) ? M[int64_t(gc) * ld + row + lr] : 0.0;
   }
-  // J[:, pair] = J[:, pair] Q  (J is cols x cols, column-major)
-  const int CP = (cols + 15) & ~15;
-  for (int rb = w; rb < CP / 16; rb += NT / 64) {
-    double av[GC / 4];
-    int row = 16 * rb + lr;
-    for (int kk = 0; kk < GC / 4; kk++) {
-      int gc = gcol(4 * kk + lk);
-      av[kk] = (gc < cols && row < cols) ? J[int64_t(gc) * ldj + row] : 0.0;
-    }
-    d4s acc[2];
-    for (int cB = 0; cB < 2; cB++) {
-      acc[cB] = d4s{0.0, 0.0, 0.0, 0.0};
-      for (int kk = 0; kk < GC / 4; kk++) acc[cB] = mfma64s(av[kk], Qc[(4 * kk + lk) * GLD + 16 * cB + lr], acc[cB]);
-    }
-    for (int cB = 0; cB < 2; cB++) {
-      int gc = gcol(16 * cB + lr);
-      for (int r = 0; r < 4; r++) {
-        int orow = 16 * rb + 4 * r + lk;
-        if (orow < cols && gc < cols) J[int64_t(gc) * ldj + orow] = acc[cB][r];
-      }
+#pragma unroll
+  for (int t = 0; t < GC * GC / 256; t++) qv[t] = qin[threadIdx.x + 256 * t];
+  const int done = state[0];
+  const double rot = dstate[kBjacQOff + int64_t(nb2 / 2) * GC * GC + p];
+  if (done || rot == 0.0) return;
+#pragma unroll
+  for (int t = 0; t < GC * GC / 256; t++) {
+    const int idx = threadIdx.x + 256 * t;
+    Qs[(idx / GC) * GLD + idx % GC] = qv[t];
+  }
+  __syncthreads();
+  double q0[GC / 4], q1[GC / 4];
+#pragma unroll
+  for (int kk = 0; kk < GC / 4; kk++) {
+    q0[kk] = Qs[(4 * kk + lk) * GLD + lr];
+    q1[kk] = Qs[(4 * kk + lk) * GLD + 16 + lr];
+  }
+  d4s acc[2] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
+#pragma unroll
+  for (int kk = 0; kk < GC / 4; kk++) {
+    acc[0] = mfma64s(av[kk], q0[kk], acc[0]);
+    acc[1] = mfma64s(av[kk], q1[kk], acc[1]);
+  }
+#pragma unroll
+  for (int cB = 0; cB < 2; cB++) {
+    const int gc = gcol(16 * cB + lr);
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int orow = row + 4 * r + lk;
+      if (orow < nr && gc < cols) M[int64_t(gc) * ld + orow] = acc[cB][r];
     }
   }
 }

@@ -103,8 +103,10 @@ void launch_block_jacobi(pc_ctx* ctx, T* H, int ldh, int rows, int cols, T* J, i
     for (int step = 0; step < nb2 - 1; step++) {
       if constexpr (std::is_same<T, double>::value) {
         if (gram_steps) {
-          hipLaunchKernelGGL(k_bjac_gstep, dim3(nb2 / 2 * 8), dim3(1024), 0, st, H, ldh, rows, cols, J, ldj, nb2, step, s,
-                             state, dstate, tol, max_inner);
+          hipLaunchKernelGGL(k_bjac_gstep, dim3(nb2 / 2 * 8), dim3(1024), 0, st, (const double*)H, ldh, rows, cols,
+                             nb2, step, s, state, dstate, tol, max_inner);
+          hipLaunchKernelGGL(k_bjac_apply, dim3(nb2 / 2 * kBjacSlices), dim3(256), 0, st, H, ldh, rows, cols, J, ldj,
+                             nb2, step, (const int*)state, (const double*)dstate);
           continue;
         }
       }
@@ -462,7 +464,7 @@ int64_t carve(WsCursor& c, const Plan& p, SmallBufs<T>& b, double** mean, double
   b.status = c.take<int>(4);
   b.perm3 = c.take<int>(n); b.rank3 = c.take<int>(4);
   b.jstate = c.take<int>(kBjacMaxSweeps + 8);
-  b.jdstate = c.take<double>(4);
+  b.jdstate = c.take<double>(kBjacDstate);  // [0] negligible norm, then the Gram steps' pair rotations
   b.dsc = c.take<double>(4);
   b.Wpad = c.take<double>(int64_t(p.NB) * p.NB);
   int ncand = int(std::min<int64_t>(p.m, 4 * int64_t(n) + 16));
