@@ -339,18 +339,24 @@ __global__ __launch_bounds__(NT) void k_pchol_fast(const double* __restrict__ G,
 // Panels of 32 columns are factored in LDS; the block reflector
 // I - V T V^T (compact WY, T upper) updates the trailing columns with MFMA
 // (X = V^T A_t, X <- T^T X, A_t -= V X); Q is accumulated backwards the
-// same way.  work: n*n (the factored matrix) + n (tau) + 32*32*ceil(n/32) (T).
-// Latency is what costs here (one workgroup): wave sums use DPP, a column
-// of the panel costs one barrier (its scaling waits for the panel's end),
-// V^T V and the products with T run on MFMA, and the K-loops over global
-// memory load eight k-steps ahead of their products.
+// same way.  Latency is what costs in the panel (one workgroup): wave sums
+// use DPP, a column of the panel costs one barrier (its scaling waits for
+// the panel's end), V^T V and T's off-diagonal block run on MFMA.
+// The factorisation runs in launches that spread its FP64 MFMA work: the
+// panels are factored on one workgroup (k_hqr_panel: the column loop, V^T V,
+// T), while the block-reflector updates, of the trailing columns after each
+// panel and of Q (accumulated backwards) afterwards, run one workgroup per
+// 32-column strip (k_hqr_strip, the strip of B staged in LDS).  A panel's
+// update is ~3,800 FP64 MFMAs at 64 cycles each per SIMD, about 25 us on one
+// CU.  The strips form the same tiles with the same MFMA sequence as one
+// workgroup doing the whole update would, so Q and R do not depend on the
+// split.  work: n*n (the factored matrix) + n (tau) + 32*32*ceil(n/32) (T).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(NT) void k_house_qr_fast(const double* __restrict__ A, int lda, int n,
-                                                      double* __restrict__ Q, int ldq, double* __restrict__ R,
-                                                      int ldr, double* __restrict__ work) {
+__global__ __launch_bounds__(NT) void k_hqr_panel(const double* __restrict__ A, int lda, int n, int pp,
+                                                  double* __restrict__ work) {
   using namespace dense;
   __shared__ double Pn[PW * PLD];  // panel columns (column c at Pn[c * PLD], rows relative to the panel start)
-  __shared__ double Xs[PW * PLD];  // X = V^T A_t (row i at Xs[i * PLD]); also V^T V
+  __shared__ double Xs[PW * PLD];  // V^T V
   __shared__ double Ts[PW * (PW + 1)];
   __shared__ double tau_s[PW], sc_s[PW], beta_s[PW];
   constexpr int NW = NT / 64;
@@ -358,108 +364,13 @@ __global__ __launch_bounds__(NT) void k_house_qr_fast(const double* __restrict__
   double* Wm = work;
   double* tau = work + int64_t(n) * n;
   double* Tg = tau + n;
-  for (int i = w; i < n; i += NW)
-    for (int j = l; j < n; j += 64) Wm[int64_t(i) * n + j] = A[int64_t(i) * lda + j];
-  __syncthreads();
-  const int npan = (n + PW - 1) / PW;
-  // V element (row i of the panel, column c), unit diagonal, zero above
+  if (pp == 0) {
+    for (int i = w; i < n; i += NW)
+      for (int j = l; j < n; j += 64) Wm[int64_t(i) * n + j] = A[int64_t(i) * lda + j];
+    __syncthreads();
+  }
   auto vget = [&](int i, int c) -> double { return i < c ? 0.0 : (i == c ? 1.0 : Pn[c * PLD + i]); };
-  // X = V^T B for a K = mr tall B (row k, column col at B[k * ldb]), two
-  // tile rows of 16 x ntl tiles, KD k-steps of loads ahead of the products
-  constexpr int KD = 4;
-  auto vt_times = [&](const double* B, int64_t ldb, int mr, int bs, int ncols, int ntl) {
-    for (int t = w; t < 2 * ntl; t += NW) {
-      const int I = t / ntl, J = t % ntl, col = 16 * J + lr, ca = 16 * I + lr;
-      d4s acc = {0.0, 0.0, 0.0, 0.0};
-      const double* Bc = B + col;
-      for (int k0 = 0; k0 < mr; k0 += 4 * KD) {
-        double bv[KD];
-#pragma unroll
-        for (int u = 0; u < KD; u++) {
-          const int k = k0 + 4 * u + lk;
-          bv[u] = (k < mr && col < ncols) ? Bc[int64_t(k) * ldb] : 0.0;
-        }
-#pragma unroll
-        for (int u = 0; u < KD; u++) {
-          const int k = k0 + 4 * u + lk;
-          acc = mfma64s((k < mr && ca < bs) ? vget(k, ca) : 0.0, bv[u], acc);
-        }
-      }
-#pragma unroll
-      for (int r = 0; r < 4; r++) Xs[(16 * I + 4 * r + lk) * PLD + col] = acc[r];
-    }
-  };
-  // X <- T^T X (trans) or T X, on MFMA; every wave holds its (at most two)
-  // result tiles in registers until all have read X
-  auto t_times = [&](bool trans, int ntl) {
-    d4s y[2];
-#pragma unroll
-    for (int u = 0; u < 2; u++) {
-      const int t = w + u * NW;
-      y[u] = d4s{0.0, 0.0, 0.0, 0.0};
-      if (t < 2 * ntl) {
-        const int I = t / ntl, J = t % ntl, col = 16 * J + lr, i = 16 * I + lr;
-#pragma unroll
-        for (int k0 = 0; k0 < PW; k0 += 4) {
-          const int k = k0 + lk;
-          const double tv = trans ? Ts[k * (PW + 1) + i] : Ts[i * (PW + 1) + k];
-          y[u] = mfma64s(tv, Xs[k * PLD + col], y[u]);
-        }
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int u = 0; u < 2; u++) {
-      const int t = w + u * NW;
-      if (t < 2 * ntl) {
-        const int I = t / ntl, J = t % ntl, col = 16 * J + lr;
-#pragma unroll
-        for (int r = 0; r < 4; r++) Xs[(16 * I + 4 * r + lk) * PLD + col] = y[u][r];
-      }
-    }
-    __syncthreads();
-  };
-  // C -= V X for an mr x ncols block C (row-major, ldc), two tiles per wave
-  // at a time: both tiles' loads go out before their products
-  auto minus_vx = [&](double* C, int64_t ldc, int mr, int bs, int ncols, int nml, int ntl) {
-    const int ntiles = nml * ntl;
-    for (int t = w; t < ntiles; t += 2 * NW) {
-      d4s acc[2];
-#pragma unroll
-      for (int u = 0; u < 2; u++) {
-        const int tt = t + u * NW, I = tt / ntl, J = tt % ntl, col = 16 * J + lr;
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-          const int row = 16 * I + 4 * r + lk;
-          acc[u][r] = (tt < ntiles && row < mr && col < ncols) ? -C[int64_t(row) * ldc + col] : 0.0;
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < 2; u++) {
-        const int tt = t + u * NW, I = tt / ntl, J = tt % ntl, col = 16 * J + lr, arow = 16 * I + lr;
-        if (tt < ntiles) {
-#pragma unroll
-          for (int k0 = 0; k0 < PW; k0 += 4) {
-            const int kc = k0 + lk;
-            const double av = (arow < mr && kc < bs) ? vget(arow, kc) : 0.0;
-            acc[u] = mfma64s(av, Xs[kc * PLD + col], acc[u]);
-          }
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < 2; u++) {
-        const int tt = t + u * NW, I = tt / ntl, J = tt % ntl, col = 16 * J + lr;
-        if (tt < ntiles) {
-#pragma unroll
-          for (int r = 0; r < 4; r++) {
-            const int row = 16 * I + 4 * r + lk;
-            if (row < mr && col < ncols) C[int64_t(row) * ldc + col] = -acc[u][r];
-          }
-        }
-      }
-    }
-  };
-  for (int pp = 0; pp < npan; pp++) {
+  {
     const int c0 = pp * PW, bs = min(PW, n - c0), mr = n - c0;
     for (int c = w; c < PW; c += NW)
       for (int i = l; i < PLD; i += 64) Pn[c * PLD + i] = (c < bs && i < mr) ? Wm[int64_t(c0 + i) * n + c0 + c] : 0.0;
@@ -467,7 +378,11 @@ __global__ __launch_bounds__(NT) void k_house_qr_fast(const double* __restrict__
     for (int j = 0; j < bs; j++) {
       // every wave forms the reflector of column j itself (same arithmetic, same result)
       double s = 0.0;
-      for (int i = j + 1 + l; i < mr; i += 64) s += Pn[j * PLD + i] * Pn[j * PLD + i];
+#pragma unroll
+      for (int q = 0; q < NMAX / 64; q++) {  // rows j + 1 + l + 64 q < mr: the reads go out together
+        const int i = j + 1 + l + 64 * q;
+        if (i < mr) s += Pn[j * PLD + i] * Pn[j * PLD + i];
+      }
       s = wave_sum_dpp(s);
       const double al = Pn[j * PLD + j];
       double tj, sc, beta;
@@ -483,15 +398,23 @@ __global__ __launch_bounds__(NT) void k_house_qr_fast(const double* __restrict__
       // apply H_j to the panel columns c > j (column j itself is read, not written)
       for (int c = j + 1 + w; c < bs; c += NW) {
         double d = 0.0;
-        for (int i = j + l; i < mr; i += 64) {
-          double v = i == j ? 1.0 : Pn[j * PLD + i] * sc;
-          d += v * Pn[c * PLD + i];
+#pragma unroll
+        for (int q = 0; q < NMAX / 64; q++) {
+          const int i = j + l + 64 * q;
+          if (i < mr) {
+            double v = i == j ? 1.0 : Pn[j * PLD + i] * sc;
+            d += v * Pn[c * PLD + i];
+          }
         }
         d = wave_sum_dpp(d);
         double f = tj * d;
-        for (int i = j + l; i < mr; i += 64) {
-          double v = i == j ? 1.0 : Pn[j * PLD + i] * sc;
-          Pn[c * PLD + i] -= v * f;
+#pragma unroll
+        for (int q = 0; q < NMAX / 64; q++) {
+          const int i = j + l + 64 * q;
+          if (i < mr) {
+            double v = i == j ? 1.0 : Pn[j * PLD + i] * sc;
+            Pn[c * PLD + i] -= v * f;
+          }
         }
       }
       __syncthreads();
@@ -517,7 +440,7 @@ __global__ __launch_bounds__(NT) void k_house_qr_fast(const double* __restrict__
         a0 = mfma64s(x0, y0, a0);
         a1 = mfma64s(x1, y1, a1);
       }
-#pragma unroll
+  #pragma unroll
       for (int r = 0; r < 4; r++) Xs[(16 * I + 4 * r + lk) * PLD + cb] = a0[r] + a1[r];
     }
     __syncthreads();
@@ -538,10 +461,10 @@ __global__ __launch_bounds__(NT) void k_house_qr_fast(const double* __restrict__
         // its column lives in Ts (row stride PW + 1), rows from the bottom up
         const int base = 16 * ((l >> 4) & 1), jl = l & 15;
         double* tc = Ts + base * (PW + 1) + base + jl;
-#pragma unroll 1
+  #pragma unroll 1
         for (int i = 15; i >= 0; i--) {
           double acc = 0.0;
-#pragma unroll
+  #pragma unroll
           for (int k = 0; k < 16; k++)
             if (k > i && k <= jl) acc += Xs[(base + i) * PLD + base + k] * tc[k * (PW + 1)];
           const double ti = base + i < bs ? tau_s[base + i] : 0.0;
@@ -554,17 +477,17 @@ __global__ __launch_bounds__(NT) void k_house_qr_fast(const double* __restrict__
       wave_sync();
       // W = (V^T V)_12 T22 into the lower-left block of Xs (not read again)
       d4s acc = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
+  #pragma unroll
       for (int k0 = 0; k0 < 16; k0 += 4)
         acc = mfma64s(Xs[lr * PLD + 16 + k0 + lk], Ts[(16 + k0 + lk) * (PW + 1) + 16 + lr], acc);
-#pragma unroll
+  #pragma unroll
       for (int r = 0; r < 4; r++) Xs[(16 + 4 * r + lk) * PLD + lr] = acc[r];
       wave_sync();
       acc = d4s{0.0, 0.0, 0.0, 0.0};
-#pragma unroll
+  #pragma unroll
       for (int k0 = 0; k0 < 16; k0 += 4)
         acc = mfma64s(Ts[lr * (PW + 1) + k0 + lk], Xs[(16 + k0 + lk) * PLD + lr], acc);
-#pragma unroll
+  #pragma unroll
       for (int r = 0; r < 4; r++) Ts[(4 * r + lk) * (PW + 1) + 16 + lr] = -acc[r];
     }
     __syncthreads();
@@ -573,39 +496,109 @@ __global__ __launch_bounds__(NT) void k_house_qr_fast(const double* __restrict__
       for (int i = l; i < mr; i += 64) Wm[int64_t(c0 + i) * n + c0 + c] = Pn[c * PLD + i];
     for (int idx = tid; idx < PW * PW; idx += NT) Tg[int64_t(pp) * PW * PW + idx] = Ts[(idx >> 5) * (PW + 1) + (idx & 31)];
     if (tid < bs) tau[c0 + tid] = tau_s[tid];
-    // trailing update of columns c1.. (rows c0..)
-    const int c1 = c0 + bs, nt = n - c1;
-    if (nt > 0) {
-      const int ntl = (nt + 15) >> 4, nml = (mr + 15) >> 4;
-      vt_times(Wm + int64_t(c0) * n + c1, n, mr, PW, nt, ntl);
-      __syncthreads();
-      t_times(true, ntl);
-      minus_vx(Wm + int64_t(c0) * n + c1, n, mr, PW, nt, nml, ntl);
+  }
+}
+
+// one workgroup per 32-column strip: qmode 0 updates the trailing columns of
+// Wm after panel pp, qmode 1 applies panel pp's block reflector to Q
+__global__ __launch_bounds__(256) void k_hqr_strip(int n, int pp, int qmode, double* __restrict__ work,
+                                                   double* __restrict__ Q, int ldq) {
+  using namespace dense;
+  constexpr int NWS = 4, XLD = PW + 1, BLD = PW + 1;
+  __shared__ double Pn[PW * PLD];
+  __shared__ double Bs[NMAX * BLD];  // the strip of B (rows relative to the panel start)
+  __shared__ double Xs[PW * XLD];
+  __shared__ double Ts[PW * (PW + 1)];
+  const int tid = threadIdx.x, w = tid >> 6, l = tid & 63, lr = l & 15, lk = l >> 4;
+  double* Wm = work;
+  const double* Tg = work + int64_t(n) * n + n;
+  const int c0 = pp * PW, bs = min(PW, n - c0), mr = n - c0;
+  const int cst = qmode ? c0 : c0 + bs;        // first column of the updated block
+  const int ncols = n - cst, s0 = 32 * blockIdx.x;
+  if (s0 >= ncols) return;
+  double* B = qmode ? Q + int64_t(c0) * ldq + cst : Wm + int64_t(c0) * n + cst;
+  const int64_t ldb = qmode ? ldq : n;
+  const int bsv = qmode ? bs : PW;             // reflector columns the products read
+  // stage V (below the diagonal), T and the strip of B: every load of a
+  // thread is issued before its first LDS store
+  {
+    constexpr int PER = PW * NMAX / 256;  // 32 elements of V and of B per thread
+    double vv[PER], bb[PER];
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+      const int idx = tid + 256 * q, i = idx >> 5, c = idx & 31;  // row i, column c
+      vv[q] = (c < bs && i < mr && i > c) ? Wm[int64_t(c0 + i) * n + c0 + c] : 0.0;
+      bb[q] = (i < mr && s0 + c < ncols) ? B[int64_t(i) * ldb + s0 + c] : 0.0;
+    }
+    for (int idx = tid; idx < PW * PW; idx += 256) Ts[(idx >> 5) * (PW + 1) + (idx & 31)] = Tg[int64_t(pp) * PW * PW + idx];
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+      const int idx = tid + 256 * q, i = idx >> 5, c = idx & 31;
+      Pn[c * PLD + i] = vv[q];
+      Bs[i * BLD + c] = bb[q];
+    }
+  }
+  __syncthreads();
+  auto vget = [&](int i, int c) -> double { return i < c ? 0.0 : (i == c ? 1.0 : Pn[c * PLD + i]); };
+  // X = V^T B for the strip: wave w forms tile (I, Jl) = (w >> 1, w & 1)
+  {
+    const int I = w >> 1, Jl = w & 1, cl = 16 * Jl + lr, ca = 16 * I + lr;
+    d4s acc = {0.0, 0.0, 0.0, 0.0};
+    for (int k0 = 0; k0 < mr; k0 += 4) {
+      const int k = k0 + lk;
+      acc = mfma64s((k < mr && ca < bsv) ? vget(k, ca) : 0.0, k < mr ? Bs[k * BLD + cl] : 0.0, acc);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++) Xs[(16 * I + 4 * r + lk) * XLD + cl] = acc[r];
+  }
+  __syncthreads();
+  // X <- T^T X (trailing) or T X (Q)
+  {
+    const int I = w >> 1, Jl = w & 1, i = 16 * I + lr;
+    d4s y = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int k0 = 0; k0 < PW; k0 += 4) {
+      const int k = k0 + lk;
+      const double tv = qmode ? Ts[i * (PW + 1) + k] : Ts[k * (PW + 1) + i];
+      y = mfma64s(tv, Xs[k * XLD + 16 * Jl + lr], y);
     }
     __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 4; r++) Xs[(16 * I + 4 * r + lk) * XLD + 16 * Jl + lr] = y[r];
   }
-  // R
-  for (int i = w; i < n; i += NW)
-    for (int j = l; j < n; j += 64) R[int64_t(i) * ldr + j] = i <= j ? Wm[int64_t(i) * n + j] : 0.0;
-  // Q = H_0 ... H_{n-1} I, accumulated backwards over the panels
-  for (int i = w; i < n; i += NW)
-    for (int j = l; j < n; j += 64) Q[int64_t(i) * ldq + j] = i == j ? 1.0 : 0.0;
   __syncthreads();
-  for (int pp = npan - 1; pp >= 0; pp--) {
-    const int c0 = pp * PW, bs = min(PW, n - c0), mr = n - c0, nq = n - c0;
-    for (int c = w; c < PW; c += NW)
-      for (int i = l; i < PLD; i += 64)
-        Pn[c * PLD + i] = (c < bs && i < mr && i > c) ? Wm[int64_t(c0 + i) * n + c0 + c] : 0.0;
-    for (int idx = tid; idx < PW * PW; idx += NT) Ts[(idx >> 5) * (PW + 1) + (idx & 31)] = Tg[int64_t(pp) * PW * PW + idx];
-    __syncthreads();
-    const int ntl = (nq + 15) >> 4, nml = (mr + 15) >> 4;
-    double* Qs = Q + int64_t(c0) * ldq + c0;
-    vt_times(Qs, ldq, mr, bs, nq, ntl);
-    __syncthreads();
-    t_times(false, ntl);
-    minus_vx(Qs, ldq, mr, bs, nq, nml, ntl);
-    __syncthreads();
+  // B -= V X: tiles (row block I, column block Jl) of the strip, B from LDS
+  const int nml = (mr + 15) >> 4, ntiles = nml * 2;
+  for (int t = w; t < ntiles; t += NWS) {
+    const int I = t >> 1, Jl = t & 1, cl = 16 * Jl + lr, col = s0 + cl, arow = 16 * I + lr;
+    d4s acc;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int row = 16 * I + 4 * r + lk;
+      acc[r] = row < mr ? -Bs[row * BLD + cl] : 0.0;
+    }
+#pragma unroll
+    for (int k0 = 0; k0 < PW; k0 += 4) {
+      const int kc = k0 + lk;
+      const double av = (arow < mr && kc < bsv) ? vget(arow, kc) : 0.0;
+      acc = mfma64s(av, Xs[kc * XLD + cl], acc);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int row = 16 * I + 4 * r + lk;
+      if (row < mr && col < ncols) B[int64_t(row) * ldb + col] = -acc[r];
+    }
   }
+}
+
+// R (upper part of the factored matrix) and Q = I before the backward accumulation
+__global__ __launch_bounds__(256) void k_hqr_rq(int n, const double* __restrict__ work, double* __restrict__ R,
+                                                int ldr, double* __restrict__ Q, int ldq) {
+  const int64_t idx = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (idx >= int64_t(n) * n) return;
+  const int i = int(idx / n), j = int(idx % n);
+  R[int64_t(i) * ldr + j] = i <= j ? work[idx] : 0.0;
+  Q[int64_t(i) * ldq + j] = i == j ? 1.0 : 0.0;
 }
 
 // W = R^-1 for an upper triangular R (n <= 256, row-major), one wavefront per

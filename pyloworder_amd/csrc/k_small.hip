@@ -777,7 +777,7 @@ __global__ __launch_bounds__(NT) void k_bjac_gstep(const double* H, int ldh, int
 // ---------------------------------------------------------------------------
 constexpr int kBjacSlices = GROWS / 32;
 __global__ __launch_bounds__(256) void k_bjac_apply(double* H, int ldh, int rows, int cols, double* J, int ldj,
-                                                    int nb2, int step, const int* state, const double* dstate) {
+                                                    int nb2, int step, int sweep, int* state, const double* dstate) {
   __shared__ double Qs[GC * GLD];
   const int p = blockIdx.x / kBjacSlices, sl = blockIdx.x % kBjacSlices;
   const int bi = rr_player(p, step, nb2), bj = rr_player(nb2 - 1 - p, step, nb2);
@@ -800,6 +800,9 @@ __global__ __launch_bounds__(256) void k_bjac_apply(double* H, int ldh, int rows
   for (int t = 0; t < GC * GC / 256; t++) qv[t] = qin[threadIdx.x + 256 * t];
   const int done = state[0];
   const double rot = dstate[kBjacQOff + int64_t(nb2 / 2) * GC * GC + p];
+  // the last step of a sweep also closes it (k_bjac_check's test): every
+  // step's k_bjac_gstep has set its rotation flag by now
+  if (step == nb2 - 2 && blockIdx.x == 0 && threadIdx.x == 0 && !done && state[1 + sweep] == 0) state[0] = 1;
   if (done || rot == 0.0) return;
 #pragma unroll
   for (int t = 0; t < GC * GC / 256; t++) {

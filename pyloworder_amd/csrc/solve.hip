@@ -106,7 +106,7 @@ void launch_block_jacobi(pc_ctx* ctx, T* H, int ldh, int rows, int cols, T* J, i
           hipLaunchKernelGGL(k_bjac_gstep, dim3(nb2 / 2 * 8), dim3(1024), 0, st, (const double*)H, ldh, rows, cols,
                              nb2, step, s, state, dstate, tol, max_inner);
           hipLaunchKernelGGL(k_bjac_apply, dim3(nb2 / 2 * kBjacSlices), dim3(256), 0, st, H, ldh, rows, cols, J, ldj,
-                             nb2, step, (const int*)state, (const double*)dstate);
+                             nb2, step, s, state, (const double*)dstate);
           continue;
         }
       }
@@ -118,7 +118,8 @@ void launch_block_jacobi(pc_ctx* ctx, T* H, int ldh, int rows, int cols, T* J, i
       hipLaunchKernelGGL((k_bjac_step<T>), dim3(nb2 / 2), dim3(1024), 0, st, H, ldh, rows, cols, J, ldj, b, nb2, step,
                          s, state, dstate, tol, max_inner);
     }
-    hipLaunchKernelGGL(k_bjac_check, dim3(1, 1, nz), dim3(64), 0, st, state, s, zb);
+    if (!(std::is_same<T, double>::value && gram_steps))  // the Gram steps close a sweep in k_bjac_apply
+      hipLaunchKernelGGL(k_bjac_check, dim3(1, 1, nz), dim3(64), 0, st, state, s, zb);
   }
   hipLaunchKernelGGL((k_bjac_norms<T>), dim3(1, 1, nz), dim3(1024), 0, st, H, ldh, rows, cols, sig, state, status, zb);
 }
@@ -201,7 +202,18 @@ template <class T>
 void run_house_qr_sq(hipStream_t st, const T* A, int n, T* Q, T* R, T* work) {
   if constexpr (std::is_same<T, double>::value)
     if (n <= dense::NMAX) {
-      hipLaunchKernelGGL(k_house_qr_fast, dim3(1), dim3(NT), 0, st, A, n, n, Q, n, R, n, work);
+      // panels on one workgroup, the block-reflector updates on one
+      // workgroup per 32-column strip
+      const int npan = (n + dense::PW - 1) / dense::PW;
+      for (int pp = 0; pp < npan; pp++) {
+        hipLaunchKernelGGL(k_hqr_panel, dim3(1), dim3(NT), 0, st, A, n, n, pp, work);
+        const int nt = n - std::min(n, (pp + 1) * dense::PW);
+        if (nt > 0) hipLaunchKernelGGL(k_hqr_strip, dim3((nt + 31) / 32), dim3(256), 0, st, n, pp, 0, work, Q, n);
+      }
+      hipLaunchKernelGGL(k_hqr_rq, dim3(nblk(int64_t(n) * n)), dim3(256), 0, st, n, (const double*)work, R, n, Q, n);
+      for (int pp = npan - 1; pp >= 0; pp--)
+        hipLaunchKernelGGL(k_hqr_strip, dim3((n - pp * dense::PW + 31) / 32), dim3(256), 0, st, n, pp, 1, work, Q,
+                           n);
       return;
     }
   if (use_mw(n, 1)) {
