@@ -98,6 +98,10 @@ struct pc_ctx {
   // one sub-context per internal stream (own workspace and status words),
   // driven by its own host thread during a grouped solve
   pc_ctx* sub[kGroupStreams] = {};
+  // set on those sub-contexts: the small stage takes its one-launch kernels
+  // (same results), since launches, not one CU's step time, bound it when
+  // many solves share the GPU
+  bool concurrent = false;
   std::string err;
   // launch-geometry tuning of the product pass (pc_autotune)
   int autotune = 0;

@@ -334,6 +334,279 @@ __global__ __launch_bounds__(NT) void k_pchol_fast(const double* __restrict__ G,
 }
 
 // ---------------------------------------------------------------------------
+// Householder QR of a real n x n matrix (n <= 256) in one launch on one
+// workgroup: the same panels, block reflectors and MFMA sequence as the
+// launch sequence below (k_hqr_panel / k_hqr_strip / k_hqr_rq), so Q and R
+// are the same; it serves solves that share the GPU with other streams,
+// where one launch costs less than the sequence's 3 npan launches.
+// LAPACK conventions (beta = -sign(alpha) ||x||, tau = (beta - alpha) / beta).
+// work: n*n (the factored matrix) + n (tau) + 32*32*ceil(n/32) (T).  The
+// K-loops over global memory load eight k-steps ahead of their products.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(NT) void k_house_qr_fast(const double* __restrict__ A, int lda, int n,
+                                                      double* __restrict__ Q, int ldq, double* __restrict__ R,
+                                                      int ldr, double* __restrict__ work) {
+  using namespace dense;
+  __shared__ double Pn[PW * PLD];  // panel columns (column c at Pn[c * PLD], rows relative to the panel start)
+  __shared__ double Xs[PW * PLD];  // X = V^T A_t (row i at Xs[i * PLD]); also V^T V
+  __shared__ double Ts[PW * (PW + 1)];
+  __shared__ double tau_s[PW], sc_s[PW], beta_s[PW];
+  constexpr int NW = NT / 64;
+  const int tid = threadIdx.x, w = tid >> 6, l = tid & 63, lr = l & 15, lk = l >> 4;
+  double* Wm = work;
+  double* tau = work + int64_t(n) * n;
+  double* Tg = tau + n;
+  for (int i = w; i < n; i += NW)
+    for (int j = l; j < n; j += 64) Wm[int64_t(i) * n + j] = A[int64_t(i) * lda + j];
+  __syncthreads();
+  const int npan = (n + PW - 1) / PW;
+  // V element (row i of the panel, column c), unit diagonal, zero above
+  auto vget = [&](int i, int c) -> double { return i < c ? 0.0 : (i == c ? 1.0 : Pn[c * PLD + i]); };
+  // X = V^T B for a K = mr tall B (row k, column col at B[k * ldb]), two
+  // tile rows of 16 x ntl tiles, KD k-steps of loads ahead of the products
+  constexpr int KD = 4;
+  auto vt_times = [&](const double* B, int64_t ldb, int mr, int bs, int ncols, int ntl) {
+    for (int t = w; t < 2 * ntl; t += NW) {
+      const int I = t / ntl, J = t % ntl, col = 16 * J + lr, ca = 16 * I + lr;
+      d4s acc = {0.0, 0.0, 0.0, 0.0};
+      const double* Bc = B + col;
+      for (int k0 = 0; k0 < mr; k0 += 4 * KD) {
+        double bv[KD];
+#pragma unroll
+        for (int u = 0; u < KD; u++) {
+          const int k = k0 + 4 * u + lk;
+          bv[u] = (k < mr && col < ncols) ? Bc[int64_t(k) * ldb] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < KD; u++) {
+          const int k = k0 + 4 * u + lk;
+          acc = mfma64s((k < mr && ca < bs) ? vget(k, ca) : 0.0, bv[u], acc);
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < 4; r++) Xs[(16 * I + 4 * r + lk) * PLD + col] = acc[r];
+    }
+  };
+  // X <- T^T X (trans) or T X, on MFMA; every wave holds its (at most two)
+  // result tiles in registers until all have read X
+  auto t_times = [&](bool trans, int ntl) {
+    d4s y[2];
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+      const int t = w + u * NW;
+      y[u] = d4s{0.0, 0.0, 0.0, 0.0};
+      if (t < 2 * ntl) {
+        const int I = t / ntl, J = t % ntl, col = 16 * J + lr, i = 16 * I + lr;
+#pragma unroll
+        for (int k0 = 0; k0 < PW; k0 += 4) {
+          const int k = k0 + lk;
+          const double tv = trans ? Ts[k * (PW + 1) + i] : Ts[i * (PW + 1) + k];
+          y[u] = mfma64s(tv, Xs[k * PLD + col], y[u]);
+        }
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+      const int t = w + u * NW;
+      if (t < 2 * ntl) {
+        const int I = t / ntl, J = t % ntl, col = 16 * J + lr;
+#pragma unroll
+        for (int r = 0; r < 4; r++) Xs[(16 * I + 4 * r + lk) * PLD + col] = y[u][r];
+      }
+    }
+    __syncthreads();
+  };
+  // C -= V X for an mr x ncols block C (row-major, ldc), two tiles per wave
+  // at a time: both tiles' loads go out before their products
+  auto minus_vx = [&](double* C, int64_t ldc, int mr, int bs, int ncols, int nml, int ntl) {
+    const int ntiles = nml * ntl;
+    for (int t = w; t < ntiles; t += 2 * NW) {
+      d4s acc[2];
+#pragma unroll
+      for (int u = 0; u < 2; u++) {
+        const int tt = t + u * NW, I = tt / ntl, J = tt % ntl, col = 16 * J + lr;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int row = 16 * I + 4 * r + lk;
+          acc[u][r] = (tt < ntiles && row < mr && col < ncols) ? -C[int64_t(row) * ldc + col] : 0.0;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 2; u++) {
+        const int tt = t + u * NW, I = tt / ntl, J = tt % ntl, col = 16 * J + lr, arow = 16 * I + lr;
+        if (tt < ntiles) {
+#pragma unroll
+          for (int k0 = 0; k0 < PW; k0 += 4) {
+            const int kc = k0 + lk;
+            const double av = (arow < mr && kc < bs) ? vget(arow, kc) : 0.0;
+            acc[u] = mfma64s(av, Xs[kc * PLD + col], acc[u]);
+          }
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 2; u++) {
+        const int tt = t + u * NW, I = tt / ntl, J = tt % ntl, col = 16 * J + lr;
+        if (tt < ntiles) {
+#pragma unroll
+          for (int r = 0; r < 4; r++) {
+            const int row = 16 * I + 4 * r + lk;
+            if (row < mr && col < ncols) C[int64_t(row) * ldc + col] = -acc[u][r];
+          }
+        }
+      }
+    }
+  };
+  for (int pp = 0; pp < npan; pp++) {
+    const int c0 = pp * PW, bs = min(PW, n - c0), mr = n - c0;
+    for (int c = w; c < PW; c += NW)
+      for (int i = l; i < PLD; i += 64) Pn[c * PLD + i] = (c < bs && i < mr) ? Wm[int64_t(c0 + i) * n + c0 + c] : 0.0;
+    __syncthreads();
+    for (int j = 0; j < bs; j++) {
+      // every wave forms the reflector of column j itself (same arithmetic, same result)
+      double s = 0.0;
+      for (int i = j + 1 + l; i < mr; i += 64) s += Pn[j * PLD + i] * Pn[j * PLD + i];
+      s = wave_sum_dpp(s);
+      const double al = Pn[j * PLD + j];
+      double tj, sc, beta;
+      if (s == 0.0) {
+        tj = 0.0; sc = 0.0; beta = al;
+      } else {
+        double anorm = sqrt(al * al + s);
+        beta = al >= 0 ? -anorm : anorm;
+        tj = (beta - al) / beta;
+        sc = 1.0 / (al - beta);
+      }
+      if (tid == 0) { tau_s[j] = tj; sc_s[j] = sc; beta_s[j] = beta; }
+      // apply H_j to the panel columns c > j (column j itself is read, not written)
+      for (int c = j + 1 + w; c < bs; c += NW) {
+        double d = 0.0;
+        for (int i = j + l; i < mr; i += 64) {
+          double v = i == j ? 1.0 : Pn[j * PLD + i] * sc;
+          d += v * Pn[c * PLD + i];
+        }
+        d = wave_sum_dpp(d);
+        double f = tj * d;
+        for (int i = j + l; i < mr; i += 64) {
+          double v = i == j ? 1.0 : Pn[j * PLD + i] * sc;
+          Pn[c * PLD + i] -= v * f;
+        }
+      }
+      __syncthreads();
+    }
+    // the panel's reflectors in place: v below the diagonal, beta on it
+    for (int c = w; c < bs; c += NW) {
+      const double sc = sc_s[c];
+      for (int i = c + 1 + l; i < mr; i += 64) Pn[c * PLD + i] *= sc;
+      if (l == 0) Pn[c * PLD + c] = beta_s[c];
+    }
+    __syncthreads();
+    // V^T V (bs x bs) into Xs: four 16 x 16 tiles, one wave each, two
+    // accumulators over alternating k-steps
+    if (w < 4) {
+      const int I = w >> 1, J = w & 1, ca = 16 * I + lr, cb = 16 * J + lr;
+      d4s a0 = {0.0, 0.0, 0.0, 0.0}, a1 = {0.0, 0.0, 0.0, 0.0};
+      for (int k0 = 0; k0 < mr; k0 += 8) {
+        const int k = k0 + lk, k2 = k0 + 4 + lk;
+        const double x0 = (k < mr && ca < bs) ? vget(k, ca) : 0.0;
+        const double y0 = (k < mr && cb < bs) ? vget(k, cb) : 0.0;
+        const double x1 = (k2 < mr && ca < bs) ? vget(k2, ca) : 0.0;
+        const double y1 = (k2 < mr && cb < bs) ? vget(k2, cb) : 0.0;
+        a0 = mfma64s(x0, y0, a0);
+        a1 = mfma64s(x1, y1, a1);
+      }
+#pragma unroll
+      for (int r = 0; r < 4; r++) Xs[(16 * I + 4 * r + lk) * PLD + cb] = a0[r] + a1[r];
+    }
+    __syncthreads();
+    // T of the compact WY form.  T^-1 = diag(1 / tau) + strictly upper
+    // (V^T V) (Puglisi 1992), so in 16 x 16 blocks T11 and T22 come from back
+    // substitution, one lane per column in registers: T_jj = tau_j,
+    // T_ij = -tau_i sum_{k = i+1..j} (V^T V)_ik T_kj; multiplying by tau_i
+    // (not dividing by 1 / tau_i) keeps a trivial reflector (tau_i = 0) exact,
+    // its row and column of T zero.  Then T12 = -T11 (V^T V)_12 T22 on MFMA.
+    if (w == 0) {
+      auto wave_sync = [] {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      };
+      {
+        // lane jl of half-wave `base / 16` owns column jl of block (base, base);
+        // its column lives in Ts (row stride PW + 1), rows from the bottom up
+        const int base = 16 * ((l >> 4) & 1), jl = l & 15;
+        double* tc = Ts + base * (PW + 1) + base + jl;
+#pragma unroll 1
+        for (int i = 15; i >= 0; i--) {
+          double acc = 0.0;
+#pragma unroll
+          for (int k = 0; k < 16; k++)
+            if (k > i && k <= jl) acc += Xs[(base + i) * PLD + base + k] * tc[k * (PW + 1)];
+          const double ti = base + i < bs ? tau_s[base + i] : 0.0;
+          const double v = i == jl ? ti : (i < jl ? -ti * acc : 0.0);
+          if (l < 32) tc[i * (PW + 1)] = v;
+        }
+        if (l < 16)
+          for (int i = 0; i < 16; i++) Ts[(16 + i) * (PW + 1) + jl] = 0.0;
+      }
+      wave_sync();
+      // W = (V^T V)_12 T22 into the lower-left block of Xs (not read again)
+      d4s acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int k0 = 0; k0 < 16; k0 += 4)
+        acc = mfma64s(Xs[lr * PLD + 16 + k0 + lk], Ts[(16 + k0 + lk) * (PW + 1) + 16 + lr], acc);
+#pragma unroll
+      for (int r = 0; r < 4; r++) Xs[(16 + 4 * r + lk) * PLD + lr] = acc[r];
+      wave_sync();
+      acc = d4s{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int k0 = 0; k0 < 16; k0 += 4)
+        acc = mfma64s(Ts[lr * (PW + 1) + k0 + lk], Xs[(16 + k0 + lk) * PLD + lr], acc);
+#pragma unroll
+      for (int r = 0; r < 4; r++) Ts[(4 * r + lk) * (PW + 1) + 16 + lr] = -acc[r];
+    }
+    __syncthreads();
+    // store the panel (R part and V) and T
+    for (int c = w; c < bs; c += NW)
+      for (int i = l; i < mr; i += 64) Wm[int64_t(c0 + i) * n + c0 + c] = Pn[c * PLD + i];
+    for (int idx = tid; idx < PW * PW; idx += NT) Tg[int64_t(pp) * PW * PW + idx] = Ts[(idx >> 5) * (PW + 1) + (idx & 31)];
+    if (tid < bs) tau[c0 + tid] = tau_s[tid];
+    // trailing update of columns c1.. (rows c0..)
+    const int c1 = c0 + bs, nt = n - c1;
+    if (nt > 0) {
+      const int ntl = (nt + 15) >> 4, nml = (mr + 15) >> 4;
+      vt_times(Wm + int64_t(c0) * n + c1, n, mr, PW, nt, ntl);
+      __syncthreads();
+      t_times(true, ntl);
+      minus_vx(Wm + int64_t(c0) * n + c1, n, mr, PW, nt, nml, ntl);
+    }
+    __syncthreads();
+  }
+  // R
+  for (int i = w; i < n; i += NW)
+    for (int j = l; j < n; j += 64) R[int64_t(i) * ldr + j] = i <= j ? Wm[int64_t(i) * n + j] : 0.0;
+  // Q = H_0 ... H_{n-1} I, accumulated backwards over the panels
+  for (int i = w; i < n; i += NW)
+    for (int j = l; j < n; j += 64) Q[int64_t(i) * ldq + j] = i == j ? 1.0 : 0.0;
+  __syncthreads();
+  for (int pp = npan - 1; pp >= 0; pp--) {
+    const int c0 = pp * PW, bs = min(PW, n - c0), mr = n - c0, nq = n - c0;
+    for (int c = w; c < PW; c += NW)
+      for (int i = l; i < PLD; i += 64)
+        Pn[c * PLD + i] = (c < bs && i < mr && i > c) ? Wm[int64_t(c0 + i) * n + c0 + c] : 0.0;
+    for (int idx = tid; idx < PW * PW; idx += NT) Ts[(idx >> 5) * (PW + 1) + (idx & 31)] = Tg[int64_t(pp) * PW * PW + idx];
+    __syncthreads();
+    const int ntl = (nq + 15) >> 4, nml = (mr + 15) >> 4;
+    double* Qs = Q + int64_t(c0) * ldq + c0;
+    vt_times(Qs, ldq, mr, bs, nq, ntl);
+    __syncthreads();
+    t_times(false, ntl);
+    minus_vx(Qs, ldq, mr, bs, nq, nml, ntl);
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Householder QR of a real n x n matrix (n <= 256), A = Q R, Q explicit.
 // LAPACK conventions (beta = -sign(alpha) ||x||, tau = (beta - alpha) / beta).
 // Panels of 32 columns are factored in LDS; the block reflector

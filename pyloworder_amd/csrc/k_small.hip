@@ -576,8 +576,14 @@ constexpr int kBjacDstate = kBjacQOff + (GROWS / GB / 2) * GC * GC + GROWS / GB 
 typedef double d4s __attribute__((ext_vector_type(4)));
 PC_DEV d4s mfma64s(double a, double b, d4s c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
 
-__global__ __launch_bounds__(NT) void k_bjac_gstep(const double* H, int ldh, int rows, int cols, int nb2, int step,
-                                                   int sweep, int* state, double* dstate, double tol, int max_inner) {
+// FUSED: the step also forms H[:, pair] Q and J[:, pair] Q itself, on its one
+// workgroup (one launch per step, for solves that share the GPU with other
+// streams); otherwise it stores Q for k_bjac_apply.  Both form the same
+// products in the same k order, so H and J do not depend on the choice.
+template <bool FUSED>
+__global__ __launch_bounds__(NT) void k_bjac_gstep(double* H, int ldh, int rows, int cols, double* J, int ldj,
+                                                   int nb2, int step, int sweep, int* state, double* dstate,
+                                                   double tol, int max_inner) {
   __shared__ double Hs[GC * (GROWS + 1)];
   __shared__ double G[GC * GLD], G2[GC * GLD];
   __shared__ double Q[GC * GLD], Q2[GC * GLD];
@@ -666,7 +672,7 @@ __global__ __launch_bounds__(NT) void k_bjac_gstep(const double* H, int ldh, int
   double* qout = dstate + kBjacQOff + int64_t(p) * GC * GC;
   double* qflag = dstate + kBjacQOff + int64_t(nb2 / 2) * GC * GC + p;
   if (!need) {
-    if (threadIdx.x == 0) *qflag = 0.0;  // k_bjac_apply leaves this pair as it is
+    if (!FUSED && threadIdx.x == 0) *qflag = 0.0;  // k_bjac_apply leaves this pair as it is
     return;
   }
   if (threadIdx.x == 0) atomicOr(&state[1 + sweep], 1);
@@ -761,10 +767,48 @@ __global__ __launch_bounds__(NT) void k_bjac_gstep(const double* H, int ldh, int
     }
     if (!__syncthreads_or(rotated)) break;
   }
-  // the pair's rotation Q (row-major) for k_bjac_apply, which forms
-  // H[:, pair] Q and J[:, pair] Q over many workgroups
-  for (int idx = threadIdx.x; idx < GC * GC; idx += NT) qout[idx] = Q[(idx / GC) * GLD + idx % GC];
-  if (threadIdx.x == 0) *qflag = 1.0;
+  if constexpr (!FUSED) {
+    // the pair's rotation Q (row-major) for k_bjac_apply, which forms
+    // H[:, pair] Q and J[:, pair] Q over many workgroups
+    for (int idx = threadIdx.x; idx < GC * GC; idx += NT) qout[idx] = Q[(idx / GC) * GLD + idx % GC];
+    if (threadIdx.x == 0) *qflag = 1.0;
+    return;
+  }
+  // H[:, pair] = Hs Q  (row blocks of 16 rows, one wave each)
+  for (int rb = w; rb < RP / 16; rb += NT / 64) {
+    for (int cB = 0; cB < 2; cB++) {
+      d4s acc = {0.0, 0.0, 0.0, 0.0};
+      for (int k0 = 0; k0 < GC; k0 += 4)
+        acc = mfma64s(Hs[(k0 + lk) * ld + 16 * rb + lr], Q[(k0 + lk) * GLD + 16 * cB + lr], acc);
+      const int gc = gcol(16 * cB + lr);
+      for (int r = 0; r < 4; r++) {
+        const int row = 16 * rb + 4 * r + lk;
+        if (row < rows && gc < cols) H[int64_t(gc) * ldh + row] = acc[r];
+      }
+    }
+  }
+  // J[:, pair] = J[:, pair] Q  (J is cols x cols, column-major)
+  const int CP = (cols + 15) & ~15;
+  for (int rb = w; rb < CP / 16; rb += NT / 64) {
+    double av[GC / 4];
+    const int row = 16 * rb + lr;
+    for (int kk = 0; kk < GC / 4; kk++) {
+      const int gc = gcol(4 * kk + lk);
+      av[kk] = (gc < cols && row < cols) ? J[int64_t(gc) * ldj + row] : 0.0;
+    }
+    d4s acc[2];
+    for (int cB = 0; cB < 2; cB++) {
+      acc[cB] = d4s{0.0, 0.0, 0.0, 0.0};
+      for (int kk = 0; kk < GC / 4; kk++) acc[cB] = mfma64s(av[kk], Q[(4 * kk + lk) * GLD + 16 * cB + lr], acc[cB]);
+    }
+    for (int cB = 0; cB < 2; cB++) {
+      const int gc = gcol(16 * cB + lr);
+      for (int r = 0; r < 4; r++) {
+        const int orow = 16 * rb + 4 * r + lk;
+        if (orow < cols && gc < cols) J[int64_t(gc) * ldj + orow] = acc[cB][r];
+      }
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------

@@ -73,6 +73,10 @@ void launch_block_jacobi(pc_ctx* ctx, T* H, int ldh, int rows, int cols, T* J, i
   bool gram_steps;
   int b, nb2;
   jacobi_blocks<T>(rows, cols, gram_steps, b, nb2);
+  // a context that shares the GPU with other streams (grouped solve) takes
+  // the one-launch steps: there the launch count costs more than the one-CU
+  // step time the split steps save
+  const bool fused = ctx->concurrent;
   hipLaunchKernelGGL((k_bjac_init<T>), dim3(1, 1, nz), dim3(1024), 0, st, H, ldh, rows, cols, J, ldj, state, dstate,
                      kBjacMaxSweeps + 1, zb);
   // one block pair (all columns in one workgroup): each launch iterates to
@@ -103,7 +107,12 @@ void launch_block_jacobi(pc_ctx* ctx, T* H, int ldh, int rows, int cols, T* J, i
     for (int step = 0; step < nb2 - 1; step++) {
       if constexpr (std::is_same<T, double>::value) {
         if (gram_steps) {
-          hipLaunchKernelGGL(k_bjac_gstep, dim3(nb2 / 2 * 8), dim3(1024), 0, st, (const double*)H, ldh, rows, cols,
+          if (fused) {
+            hipLaunchKernelGGL(k_bjac_gstep<true>, dim3(nb2 / 2 * 8), dim3(1024), 0, st, H, ldh, rows, cols, J, ldj,
+                               nb2, step, s, state, dstate, tol, max_inner);
+            continue;
+          }
+          hipLaunchKernelGGL(k_bjac_gstep<false>, dim3(nb2 / 2 * 8), dim3(1024), 0, st, H, ldh, rows, cols, J, ldj,
                              nb2, step, s, state, dstate, tol, max_inner);
           hipLaunchKernelGGL(k_bjac_apply, dim3(nb2 / 2 * kBjacSlices), dim3(256), 0, st, H, ldh, rows, cols, J, ldj,
                              nb2, step, s, state, (const double*)dstate);
@@ -118,7 +127,7 @@ void launch_block_jacobi(pc_ctx* ctx, T* H, int ldh, int rows, int cols, T* J, i
       hipLaunchKernelGGL((k_bjac_step<T>), dim3(nb2 / 2), dim3(1024), 0, st, H, ldh, rows, cols, J, ldj, b, nb2, step,
                          s, state, dstate, tol, max_inner);
     }
-    if (!(std::is_same<T, double>::value && gram_steps))  // the Gram steps close a sweep in k_bjac_apply
+    if (!(std::is_same<T, double>::value && gram_steps && !fused))  // else the sweep closes in k_bjac_apply
       hipLaunchKernelGGL(k_bjac_check, dim3(1, 1, nz), dim3(64), 0, st, state, s, zb);
   }
   hipLaunchKernelGGL((k_bjac_norms<T>), dim3(1, 1, nz), dim3(1024), 0, st, H, ldh, rows, cols, sig, state, status, zb);
@@ -197,9 +206,16 @@ void run_pchol(hipStream_t st, const T* G, int n, double tol, T* A, T* R, int* p
   }
   hipLaunchKernelGGL((k_pchol<T>), dim3(1, 1, nz), dim3(NT), 0, st, G, n, n, tol, A, R, n, perm, rank, status, zb);
 }
-// square Householder QR (n x n), Q and R explicit; work: 2 n^2 + n of T
+// square Householder QR (n x n), Q and R explicit; work: 2 n^2 + n of T.
+// fused: one launch (k_house_qr_fast) instead of the panel / strip sequence,
+// for contexts that share the GPU with other streams; same Q and R.
 template <class T>
-void run_house_qr_sq(hipStream_t st, const T* A, int n, T* Q, T* R, T* work) {
+void run_house_qr_sq(hipStream_t st, const T* A, int n, T* Q, T* R, T* work, bool fused = false) {
+  if constexpr (std::is_same<T, double>::value)
+    if (n <= dense::NMAX && fused) {
+      hipLaunchKernelGGL(k_house_qr_fast, dim3(1), dim3(NT), 0, st, A, n, n, Q, n, R, n, work);
+      return;
+    }
   if constexpr (std::is_same<T, double>::value)
     if (n <= dense::NMAX) {
       // panels on one workgroup, the block-reflector updates on one
@@ -1154,7 +1170,7 @@ static int core_t(pc_ctx* ctx, const Plan& p, const void* A, int64_t lda, int ce
                        b.tmp, n, (const int*)nullptr, T(0.0), b.G1, n, 0);
     run_pchol<T>(st, b.G1, n, 1e-30, b.Awk, b.R1, b.perm3, b.rank3, b.status, 1, 0, b.rank3 + 1);
     hipLaunchKernelGGL((k_copy2d<T>), dim3(nblk(nn)), dim3(256), 0, st, b.R1, n, n, n, b.Awk, n, 2);
-    run_house_qr_sq<T>(st, b.Awk, n, b.UM, b.G1, nwork);
+    run_house_qr_sq<T>(st, b.Awk, n, b.UM, b.G1, nwork, ctx->concurrent);
     hipLaunchKernelGGL((k_permute_rows<T>), dim3(nblk(nn)), dim3(256), 0, st, b.UM, n, n, (const int*)b.perm3, b.Awk,
                        n);
     hipLaunchKernelGGL((k_gemm_small<T>), gg, dim3(256), 0, st, 0, 0, n, n, n, T(1.0), b.tmp, n, (const int*)nullptr,
@@ -1173,7 +1189,7 @@ static int core_t(pc_ctx* ctx, const Plan& p, const void* A, int64_t lda, int ce
   } else {
     // QR of M (n x n, rows >= rank zero): M = Q_M R_M;  R output, UM := Q_M
     hipLaunchKernelGGL((k_copy2d<T>), dim3(nblk(nn)), dim3(256), 0, st, b.Hc, n, n, n, b.tmp, n, 1);
-    run_house_qr_sq<T>(st, b.tmp, n, b.UM, b.J, nwork);
+    run_house_qr_sq<T>(st, b.tmp, n, b.UM, b.J, nwork, ctx->concurrent);
     // diag(R) real and non-negative (the phases go to Q_M's columns); tmp is free here
     hipLaunchKernelGGL((k_qr_phase<T>), dim3(nblk(n)), dim3(256), 0, st, b.J, n, n, b.tmp);
     hipLaunchKernelGGL((k_qr_apply_phase<T>), dim3(nblk(nn)), dim3(256), 0, st, b.J, n, b.UM, n, n,

@@ -49,6 +49,18 @@ def test_grouped_bitwise_equals_single(rng):
         assert np.array_equal(S, S1) and np.array_equal(U, U1) and np.array_equal(V, V1)
 
 
+def test_grouped_mixed_bitwise_equals_single(rng):
+    """float32 mixed: the grouped members' one-launch Jacobi steps and
+    Householder QR (n = 256 and 100 take the preconditioned block Jacobi)
+    give the single solves' split-launch results bit for bit."""
+    shapes = [(40000, 256), (7000, 100), (3000, 33), (9000, 256)]
+    As = [(rng.standard_normal(s) @ np.diag(np.logspace(0, -3, s[1]))).astype(np.float32) for s in shapes]
+    Ug, Sg, Vg = pl.math.tsqr_svd_grouped(As, precision="mixed")
+    for A, U, S, V in zip(As, Ug, Sg, Vg):
+        U1, S1, V1 = pl.math.tsqr_svd(A, precision="mixed")
+        assert np.array_equal(S, S1) and np.array_equal(U, U1) and np.array_equal(V, V1)
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_grouped_complex64_uneven(precision, rng):
     shapes = [(4000, 12), (900, 33), (15000, 5)]
