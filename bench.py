@@ -232,11 +232,15 @@ def pass_flops(name, m, n):
     """fp16 MFMA flops a mixed pass issues (n = 32 NBLK padded columns): P1
     G = Ys^T Ys, three split products on the NU = NBLK (NBLK + 1) / 2 upper
     32 x 32 blocks: 3 NU 2048 m; P2 Z = Ys W1 (hi and lo of Ys against fp16
-    W1: 4 m n^2) plus the same three-product Gram of Z; P3 C = Ys W, three
-    products: 6 m n^2.  Implementation work, not algorithmic work."""
+    W1, 2 products) over the nonzero k range of the upper triangular W1
+    (32 (J + 1) rows for column block J): 2 NU 2048 m, plus the same
+    three-product Gram of Z; P3 C = Ys W, three products: 6 m n^2.
+    Implementation work, not algorithmic work (rocprofv3's
+    SQ_INSTS_VALU_MFMA_MOPS_F16 x 512 reads the same per dispatch)."""
     nblk = -(-n // 32)
-    gram = 3 * (nblk * (nblk + 1) // 2) * 32 * 32 * 2 * m
-    return {"gram": gram, "zgram": 4 * m * (32 * nblk) ** 2 + gram, "gemm": 6 * m * (32 * nblk) ** 2}[name]
+    nu = nblk * (nblk + 1) // 2
+    gram = 3 * nu * 32 * 32 * 2 * m
+    return {"gram": gram, "zgram": 2 * nu * 32 * 32 * 2 * m + gram, "gemm": 6 * m * (32 * nblk) ** 2}[name]
 
 
 def algorithmic_work(name, m, n, b):
