@@ -680,9 +680,15 @@ __global__ __launch_bounds__(NT2, 1) void k_zgram_mx2(const float* __restrict__ 
   constexpr int NU = NBLK * (NBLK + 1) / 2;
   constexpr int S = (NU + NW2 - 1) / NW2;
   constexpr int KSG = RT::TR / 16;  // Gram k-steps per tile
-  __shared__ __align__(16) char Fs[RT::FBYTES];
+  // Y fragments and row exponents alternate between two buffers: while the
+  // Z products of tile t read one, the waves split tile t + 1 into the other
+  // (one row between consecutive chunks of products), so the vector ALU work
+  // of one wave meets the matrix work of the other wave on each SIMD, and a
+  // tile costs two barriers.  Every product and split is the one the
+  // unpipelined order formed, so the results do not change.
+  __shared__ __align__(16) char Fs[2][RT::FBYTES];
   __shared__ h8 F2[KSG * NBLK * 2 * 64];
-  __shared__ int rexp[RT::TR];
+  __shared__ int rexp[2][RT::TR];
   __shared__ float red[NW2];
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63, hl = l >> 5, c = l & 31;
   // Column block of Z this wave forms.  With an upper triangular W1 (upper
@@ -711,42 +717,59 @@ __global__ __launch_bounds__(NT2, 1) void k_zgram_mx2(const float* __restrict__ 
   const int64_t ntiles = ceil_div(m, RT::TR);
   const int64_t t0 = int64_t(blockIdx.x) * tpc;
   const int64_t t1 = t0 + tpc < ntiles ? t0 + tpc : ntiles;
-  // one tile in flight (registers are taken by W1 and the Gram accumulators);
-  // the load is always issued, clamped to the chunk's last tile
+  auto clamp = [&](int64_t t) { return t < t1 ? t : t1 - 1; };
+  // one tile in registers (the rest are taken by W1 and the Gram
+  // accumulators); a load is always issued, clamped to the chunk's last tile
   f4 va[RT::NLD];
   if (t0 < t1) rows_load<NB, VEC>(X, ldx, m, n, t0, va);
   // the W fragments are loop invariant: settle every prologue load here so that
   // no wait for them is placed inside the loop (it would also wait for the
   // prefetch of the next tiles)
   __builtin_amdgcn_s_waitcnt(0);
-  auto body = [&](int64_t tile, f4* v) {
-    rows_split<NB, CENTER>(v, m, n, mean, var, tile, Fs, rexp);
-    lds_sync();
-    rows_load<NB, VEC>(X, ldx, m, n, tile + 1 < t1 ? tile + 1 : t1 - 1, v);
+  if (t0 < t1) {
+    rows_split<NB, CENTER>(va, m, n, mean, var, t0, Fs[0], rexp[0]);
+    rows_load<NB, VEC>(X, ldx, m, n, clamp(t0 + 1), va);
+  }
+  lds_sync();
+  // tile t: products from (F, rx), split of tile t + 1 (in va) into (Fn, rxn)
+  auto body = [&](int64_t tile, const char* F, const int* rx, char* Fn, int* rxn) {
+    constexpr int NIT = RT::NR * KSW;  // (R, ks) products, in NLD chunks
+    const int64_t nt = clamp(tile + 1);
     f16v z[RT::NR];
-    float zmx = 0.0f;
 #pragma unroll
-    for (int R = 0; R < RT::NR; R++) {
-      z[R] = f16v{};
+    for (int R = 0; R < RT::NR; R++) z[R] = f16v{};
+#pragma unroll
+    for (int q = 0; q < RT::NLD; q++) {
       if (own) {
 #pragma unroll
-        for (int ks = 0; ks < KSW; ks++) {
+        for (int it = q * NIT / RT::NLD; it < (q + 1) * NIT / RT::NLD; it++) {
+          const int R = it / KSW, ks = it % KSW;
           if (ks < kend) {  // wave-uniform
-            z[R] = mfma(frag(Fs, (R * KSW + ks) * 2 + 0, RT::FRAGB, l), bw[ks], z[R]);
-            z[R] = mfma(frag(Fs, (R * KSW + ks) * 2 + 1, RT::FRAGB, l), bw[ks], z[R]);
+            z[R] = mfma(frag(F, (R * KSW + ks) * 2 + 0, RT::FRAGB, l), bw[ks], z[R]);
+            z[R] = mfma(frag(F, (R * KSW + ks) * 2 + 1, RT::FRAGB, l), bw[ks], z[R]);
           }
         }
+      }
+      __builtin_amdgcn_sched_barrier(0);
+      split_row<NB, CENTER>(va[q], q, m, n, mean, var, nt, Fn, rxn);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    rows_load<NB, VEC>(X, ldx, m, n, clamp(tile + 2), va);
+    float zmx = 0.0f;
+    if (own) {
+#pragma unroll
+      for (int R = 0; R < RT::NR; R++)
 #pragma unroll
         for (int r = 0; r < 16; r++) {
           const int rt = 32 * R + 8 * (r >> 2) + 4 * hl + (r & 3);
-          const float zz = ldexpf(z[R][r], -rexp[rt]) * isc;
+          const float zz = ldexpf(z[R][r], -rx[rt]) * isc;
           z[R][r] = zz;
           zmx = fmaxf(zmx, fabsf(zz));
         }
-      }
     }
     zmx = group_max(zmx, 64);
     if (l == 0) red[w] = zmx;
+    // also publishes the split of tile t + 1
     lds_sync();
     float tmax = red[0];
 #pragma unroll
@@ -773,11 +796,15 @@ __global__ __launch_bounds__(NT2, 1) void k_zgram_mx2(const float* __restrict__ 
         }
     }
     lds_sync();
+    // F2 is next written after the next tile's first barrier, which every
+    // wave reaches only once its reads here are done
 #pragma unroll
     for (int ks = 0; ks < KSG; ks++) gram_step<S>(F2 + ks * NBLK * 2 * 64, sI, sJ, sv, acc, l);
-    lds_sync();
   };
-  for (int64_t tile = t0; tile < t1; tile++) body(tile, va);
+  for (int64_t tile = t0; tile < t1; tile++) {
+    const int pb = int(tile - t0) & 1;
+    body(tile, Fs[pb], rexp[pb], Fs[pb ^ 1], rexp[pb ^ 1]);
+  }
   const double inv = ldexp(1.0, e_run == EUN ? 0 : -2 * e_run);
   write_upper<NB, S>(acc, sI, sJ, sv, inv, part + int64_t(blockIdx.x) * NB * NB, l,
                      [](int blk, int pos) { return blk * 32 + pos; });
