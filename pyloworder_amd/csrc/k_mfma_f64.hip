@@ -118,12 +118,17 @@ PC_DEV void gram_tile(double (*S)[NB + PADD], const int* bI, const int* bJ, int 
 }
 
 // Z = S W for the tile (TR x NB times NB x NB), result into Zs.  Slot s of
-// wave w holds block b = w + 8 s (row block b & 1, 16-column block
-// J = b >> 1 = w / 2 + 4 s).  With an upper triangular W (upper != 0) block J
-// only needs the k-steps kk < 4 (J + 1) (rows < 16 (J + 1)): the k range is
-// cut into segments, segment f running the slots s >= f up to slot f's end,
-// so the loops carry no per-step conditions.  The products skipped are exact
-// zeros: Z does not change.
+// wave w holds row block w & 1 of 16-column block J(w, s).  With an upper
+// triangular W (upper != 0) block J only needs the k-steps kk < 4 (J + 1)
+// (rows < 16 (J + 1)): the k range is cut into segments, segment f running
+// the slots s >= f up to slot f's end, so the loops carry no per-step
+// conditions.  The products skipped are exact zeros: Z does not change.
+// Waves 2k and 2k + 1 share a SIMD and hold the same blocks; with 8 s
+// column blocks per wave pair (NBLK % 8 == 0) pair k takes
+// J = 4 s + (s odd ? 3 - k : k), i.e. k, 7 - k, 8 + k, 15 - k at NB = 256:
+// ascending in s, and every SIMD does the same share of the k-steps.
+// Otherwise J = w / 2 + 4 s.  Which wave forms a block does not change its
+// products.
 template <int NB>
 PC_DEV void zmul_tile(double (*S)[NB + PADD], const double* __restrict__ W, double (*Zs)[NB + PADD], int upper) {
   constexpr int nb = NB / 16;
@@ -134,13 +139,17 @@ PC_DEV void zmul_tile(double (*S)[NB + PADD], const double* __restrict__ W, doub
   d4 acc[SLZ];
 #pragma unroll
   for (int s = 0; s < SLZ; s++) acc[s] = d4{0, 0, 0, 0};
+  auto jblk = [&](int s) {
+    if constexpr (NBLK % 8 == 0) return 4 * s + ((s & 1) ? 3 - (w >> 1) : (w >> 1));
+    else return (w + 8 * s) >> 1;
+  };
   auto step = [&](int kk, int first) {
     const double* wrow = W + int64_t(kk * 4 + kr) * NB;
 #pragma unroll
     for (int s = 0; s < SLZ; s++) {
       int b = w + 8 * s;
       if (s >= first && b < NBLK) {
-        int R = b & 1, J = b >> 1;
+        int R = b & 1, J = jblk(s);
         double a = S[R * 16 + cc][kk * 4 + kr];
         double bb = wrow[J * 16 + cc];
         acc[s] = mfma64(a, bb, acc[s]);
@@ -148,11 +157,10 @@ PC_DEV void zmul_tile(double (*S)[NB + PADD], const double* __restrict__ W, doub
     }
   };
   if (NBLK % 8 == 0 && upper) {
-    const int j0 = __builtin_amdgcn_readfirstlane(w >> 1);
     int kbeg = 0;
 #pragma unroll
     for (int f = 0; f < SLZ; f++) {
-      const int kend = 4 * (j0 + 4 * f + 1);
+      const int kend = __builtin_amdgcn_readfirstlane(4 * (jblk(f) + 1));
 #pragma unroll 4
       for (int kk = kbeg; kk < kend; kk++) step(kk, f);
       kbeg = kend;
@@ -165,7 +173,7 @@ PC_DEV void zmul_tile(double (*S)[NB + PADD], const double* __restrict__ W, doub
   for (int s = 0; s < SLZ; s++) {
     int b = w + 8 * s;
     if (b < NBLK) {
-      int R = b & 1, J = b >> 1;
+      int R = b & 1, J = jblk(s);
 #pragma unroll
       for (int r = 0; r < 4; r++) Zs[R * 16 + 4 * r + kr][J * 16 + cc] = acc[s][r];
     }
