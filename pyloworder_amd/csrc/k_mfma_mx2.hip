@@ -112,6 +112,20 @@ PC_DEV bool upper_slot(int idx, int& I, int& J) {
   return false;
 }
 
+// Contiguous range [start, start + cnt) of upper-block indices that wave w
+// accumulates.  Waves 2k and 2k + 1 share a SIMD, so the NU % NW2 blocks
+// beyond an even share go first to the even waves (one each), then to the
+// odd ones: every SIMD's pair then holds NU / 4 blocks, give or take one.
+// Which wave accumulates a block does not change its products.
+template <int NU>
+PC_DEV void wave_slots(int w, int& start, int& cnt) {
+  constexpr int base = NU / NW2, rem = NU % NW2;
+  auto extra = [](int v) { return ((v & 1) ? NW2 / 2 + (v >> 1) : (v >> 1)) < rem ? 1 : 0; };
+  start = w * base;
+  for (int v = 0; v < w; v++) start += extra(v);
+  cnt = base + extra(w);
+}
+
 // running workgroup exponent: a larger tile maximum lowers it and rescales acc
 template <int S>
 PC_DEV int update_scale(float tmax, int e_run, f16v* acc) {
@@ -205,8 +219,12 @@ __global__ __launch_bounds__(NT2, 1) void k_gram_mx2(const float* __restrict__ X
   const int wk = w % KS, wg = w / KS;
   int sI[S], sJ[S];
   bool sv[S];
+  {
+    int s0, sc;
+    wave_slots<NU>(w, s0, sc);
 #pragma unroll
-  for (int s = 0; s < S; s++) sv[s] = upper_slot<NBLK>(w * S + s, sI[s], sJ[s]);
+    for (int s = 0; s < S; s++) sv[s] = upper_slot<NBLK>(s0 + s, sI[s], sJ[s]) && s < sc;
+  }
   f16v acc[S];
 #pragma unroll
   for (int s = 0; s < S; s++) acc[s] = f16v{};
@@ -708,8 +726,12 @@ __global__ __launch_bounds__(NT2, 1) void k_zgram_mx2(const float* __restrict__ 
   }
   int sI[S], sJ[S];
   bool sv[S];
+  {
+    int s0, sc;
+    wave_slots<NU>(w, s0, sc);
 #pragma unroll
-  for (int s = 0; s < S; s++) sv[s] = upper_slot<NBLK>(w * S + s, sI[s], sJ[s]);
+    for (int s = 0; s < S; s++) sv[s] = upper_slot<NBLK>(s0 + s, sI[s], sJ[s]) && s < sc;
+  }
   f16v acc[S];
 #pragma unroll
   for (int s = 0; s < S; s++) acc[s] = f16v{};
