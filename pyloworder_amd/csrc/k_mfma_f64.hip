@@ -118,17 +118,19 @@ PC_DEV void gram_tile(double (*S)[NB + PADD], const int* bI, const int* bJ, int 
 }
 
 // Z = S W for the tile (TR x NB times NB x NB), result into Zs.  Slot s of
-// wave w holds row block w & 1 of 16-column block J(w, s).  With an upper
+// wave w holds row block R and 16-column block J of Z.  With an upper
 // triangular W (upper != 0) block J only needs the k-steps kk < 4 (J + 1)
 // (rows < 16 (J + 1)): the k range is cut into segments, segment f running
 // the slots s >= f up to slot f's end, so the loops carry no per-step
 // conditions.  The products skipped are exact zeros: Z does not change.
-// Waves 2k and 2k + 1 share a SIMD and hold the same blocks; with 8 s
-// column blocks per wave pair (NBLK % 8 == 0) pair k takes
-// J = 4 s + (s odd ? 3 - k : k), i.e. k, 7 - k, 8 + k, 15 - k at NB = 256:
-// ascending in s, and every SIMD does the same share of the k-steps.
-// Otherwise J = w / 2 + 4 s.  Which wave forms a block does not change its
-// products.
+// Waves 2k and 2k + 1 share a SIMD.  The column blocks are listed per wave
+// pair as J_q = 4 q + (q odd ? 3 - k : k), i.e. k, 7 - k, 8 + k, 15 - k at
+// NB = 256, so every SIMD does the same share of the k-steps.  With 2 or 4
+// slots a wave takes both row blocks of its column blocks (wave 2k: J_0 and
+// J_3, wave 2k + 1: J_1 and J_2 at NB = 256; J_0 and J_1 at NB = 128), so
+// each W element it loads feeds two products; otherwise slot s takes row
+// block w & 1 of J_s.  Slots are ascending in J.  Which wave forms a block
+// does not change its products.
 template <int NB>
 PC_DEV void zmul_tile(double (*S)[NB + PADD], const double* __restrict__ W, double (*Zs)[NB + PADD], int upper) {
   constexpr int nb = NB / 16;
@@ -139,17 +141,27 @@ PC_DEV void zmul_tile(double (*S)[NB + PADD], const double* __restrict__ W, doub
   d4 acc[SLZ];
 #pragma unroll
   for (int s = 0; s < SLZ; s++) acc[s] = d4{0, 0, 0, 0};
-  auto jblk = [&](int s) {
-    if constexpr (NBLK % 8 == 0) return 4 * s + ((s & 1) ? 3 - (w >> 1) : (w >> 1));
-    else return (w + 8 * s) >> 1;
+  auto zblk = [&](int s, int& R, int& J) {
+    const int k = w >> 1;
+    if constexpr (NBLK == 8 * SLZ && (SLZ == 2 || SLZ == 4)) {
+      const int h = w & 1, i = s >> 1;
+      const int q = SLZ == 4 ? (h ? 1 + i : 3 * i) : h;
+      R = s & 1;
+      J = 4 * q + ((q & 1) ? 3 - k : k);
+      return true;
+    } else {
+      const int b = w + 8 * s;
+      R = b & 1;
+      J = NBLK % 8 == 0 ? 4 * s + ((s & 1) ? 3 - k : k) : b >> 1;
+      return b < NBLK;
+    }
   };
   auto step = [&](int kk, int first) {
     const double* wrow = W + int64_t(kk * 4 + kr) * NB;
 #pragma unroll
     for (int s = 0; s < SLZ; s++) {
-      int b = w + 8 * s;
-      if (s >= first && b < NBLK) {
-        int R = b & 1, J = jblk(s);
+      int R, J;
+      if (zblk(s, R, J) && s >= first) {
         double a = S[R * 16 + cc][kk * 4 + kr];
         double bb = wrow[J * 16 + cc];
         acc[s] = mfma64(a, bb, acc[s]);
@@ -160,7 +172,9 @@ PC_DEV void zmul_tile(double (*S)[NB + PADD], const double* __restrict__ W, doub
     int kbeg = 0;
 #pragma unroll
     for (int f = 0; f < SLZ; f++) {
-      const int kend = __builtin_amdgcn_readfirstlane(4 * (jblk(f) + 1));
+      int Rf, Jf;
+      zblk(f, Rf, Jf);
+      const int kend = __builtin_amdgcn_readfirstlane(4 * (Jf + 1));
 #pragma unroll 4
       for (int kk = kbeg; kk < kend; kk++) step(kk, f);
       kbeg = kend;
@@ -171,9 +185,8 @@ PC_DEV void zmul_tile(double (*S)[NB + PADD], const double* __restrict__ W, doub
   }
 #pragma unroll
   for (int s = 0; s < SLZ; s++) {
-    int b = w + 8 * s;
-    if (b < NBLK) {
-      int R = b & 1, J = jblk(s);
+    int R, J;
+    if (zblk(s, R, J)) {
 #pragma unroll
       for (int r = 0; r < 4; r++) Zs[R * 16 + 4 * r + kr][J * 16 + cc] = acc[s][r];
     }
@@ -258,6 +271,23 @@ __global__ __launch_bounds__(NTH) void k_gemm_f64(const TS* __restrict__ X, int6
   __shared__ double S[TR][NB + PADD];
   int w = threadIdx.x >> 6, l = threadIdx.x & 63;
   int kr = l >> 4, cc = l & 15;
+  // Slot s of wave w: row block R, 16-column block J.  With an even number of
+  // slots per wave every wave takes both row blocks of SLZ / 2 column blocks,
+  // so each W element it loads feeds two products (it fed one, and the two
+  // waves of a column block loaded it twice).  The products of a block do not
+  // depend on the wave that forms them.
+  auto slot_blk = [&](int s, int& R, int& J) {
+    if constexpr (SLZ % 2 == 0 && NBLK == 8 * SLZ) {
+      R = s & 1;
+      J = (w * SLZ + s) >> 1;
+      return true;
+    } else {
+      const int b = w + 8 * s;
+      R = b & 1;
+      J = b >> 1;
+      return b < NBLK;
+    }
+  };
   int64_t ntiles = ceil_div(m, TR);
   int64_t t0 = int64_t(blockIdx.x) * tiles_per_chunk;
   int64_t t1 = t0 + tiles_per_chunk < ntiles ? t0 + tiles_per_chunk : ntiles;
@@ -273,18 +303,14 @@ __global__ __launch_bounds__(NTH) void k_gemm_f64(const TS* __restrict__ X, int6
       const double* wrow = W + int64_t(kk * 4 + kr) * NCB;
 #pragma unroll
       for (int s = 0; s < SLZ; s++) {
-        int b = w + 8 * s;
-        if (b < NBLK) {
-          int R = b & 1, J = b >> 1;
-          acc[s] = mfma64(S[R * 16 + cc][kk * 4 + kr], wrow[J * 16 + cc], acc[s]);
-        }
+        int R, J;
+        if (slot_blk(s, R, J)) acc[s] = mfma64(S[R * 16 + cc][kk * 4 + kr], wrow[J * 16 + cc], acc[s]);
       }
     }
 #pragma unroll
     for (int s = 0; s < SLZ; s++) {
-      int b = w + 8 * s;
-      if (b < NBLK) {
-        int R = b & 1, J = b >> 1;
+      int R, J;
+      if (slot_blk(s, R, J)) {
         int col = J * 16 + cc;
 #pragma unroll
         for (int r = 0; r < 4; r++) {
