@@ -109,22 +109,21 @@ __global__ __launch_bounds__(WNT) void k_wgemm_f64(const TS* __restrict__ X, int
         S[r][c] = v;
       }
       __syncthreads();
+      // wave w forms both 16-row blocks (slot s = row block) of column
+      // block w, so each W element it loads feeds two products
       const int kw = K - k0 < WS ? K - k0 : WS;
+      const int col = c0 + w * 16 + cc;
       for (int kk = 0; kk < (kw + 3) / 4; kk++) {
         const int k = k0 + kk * 4 + kr;
         const double* wrow = Wm + int64_t(k < K ? k : 0) * ldw;
+        const double bv = (k < K && col < N) ? wrow[col] : 0.0;
 #pragma unroll
-        for (int s = 0; s < 2; s++) {
-          const int b = w * 2 + s, R = b >> 3, Jb = b & 7;
-          const int col = c0 + Jb * 16 + cc;
-          const double bv = (k < K && col < N) ? wrow[col] : 0.0;
-          acc[s] = mfma64(S[R * 16 + cc][kk * 4 + kr], bv, acc[s]);
-        }
+        for (int s = 0; s < 2; s++) acc[s] = mfma64(S[s * 16 + cc][kk * 4 + kr], bv, acc[s]);
       }
     }
 #pragma unroll
     for (int s = 0; s < 2; s++) {
-      const int b = w * 2 + s, R = b >> 3, Jb = b & 7;
+      const int R = s, Jb = w;
       const int col = c0 + Jb * 16 + cc;
 #pragma unroll
       for (int r = 0; r < 4; r++) {
