@@ -572,14 +572,23 @@ constexpr int GROWS = 256;
 // from kBjacQOff the nb2 / 2 pair rotations (GC x GC each) and their flags
 constexpr int kBjacQOff = 4;
 constexpr int kBjacDstate = kBjacQOff + (GROWS / GB / 2) * GC * GC + GROWS / GB / 2;
+// dstate length for an n-column problem (at least kBjacDstate)
+inline int64_t bjac_dstate_len(int n) {
+  int64_t nb2 = (int64_t(n) + GB - 1) / GB;
+  nb2 += nb2 & 1;
+  const int64_t len = kBjacQOff + (nb2 / 2) * GC * GC + nb2 / 2;
+  return len > kBjacDstate ? len : kBjacDstate;
+}
 
 typedef double d4s __attribute__((ext_vector_type(4)));
 PC_DEV d4s mfma64s(double a, double b, d4s c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
 
 // FUSED: the step also forms H[:, pair] Q and J[:, pair] Q itself, on its one
 // workgroup (one launch per step, for solves that share the GPU with other
-// streams); otherwise it stores Q for k_bjac_apply.  Both form the same
-// products in the same k order, so H and J do not depend on the choice.
+// streams; rows <= GROWS); otherwise it stores Q for k_bjac_apply.  Both
+// form the same products in the same k order, so H and J do not depend on
+// the choice.  Unfused, the pair's Gram is summed over row chunks of GROWS
+// staged in turn, so any number of rows is taken (one chunk up to GROWS).
 template <bool FUSED>
 __global__ __launch_bounds__(NT) void k_bjac_gstep(double* H, int ldh, int rows, int cols, double* J, int ldj,
                                                    int nb2, int step, int sweep, int* state, double* dstate,
@@ -598,52 +607,68 @@ __global__ __launch_bounds__(NT) void k_bjac_gstep(double* H, int ldh, int rows,
   const int bi = rr_player(p, step, nb2), bj = rr_player(nb2 - 1 - p, step, nb2);
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63, lr = l & 15, lk = l >> 4;
   auto gcol = [&](int lc) { return lc < GB ? bi * GB + lc : bj * GB + (lc - GB); };
-  {
-    // every load of the pair's columns is issued before the first LDS store,
-    // and together with the done flag (a converged iteration discards them)
+  // Stage row chunk c (rows c GROWS ... of the pair's columns) into Hs with
+  // leading dimension RPc + 1.  Every load is issued before the first LDS
+  // store, and those of chunk 0 together with the done flag (a converged
+  // iteration discards them): false when the iteration has converged.
+  const int nch = FUSED ? 1 : (rows + GROWS - 1) / GROWS;
+  auto chunk_rows = [&](int c) { const int rc = rows - c * GROWS; return rc < GROWS ? rc : GROWS; };
+  auto stage = [&](int c) {
     constexpr int KL = GC * GROWS / NT;
+    const int rc = chunk_rows(c), RPc = (rc + 15) & ~15, ldc = RPc + 1;
+    const int64_t r0 = int64_t(c) * GROWS;
     double hv[KL];
 #pragma unroll
     for (int k = 0; k < KL; k++) {
-      const int idx = threadIdx.x + k * NT, lc = idx / RP, i = idx - lc * RP, gc = gcol(lc);
-      hv[k] = (idx < GC * RP && gc < cols && i < rows) ? H[int64_t(gc) * ldh + i] : 0.0;
+      const int idx = threadIdx.x + k * NT, lc = idx / RPc, i = idx - lc * RPc, gc = gcol(lc);
+      hv[k] = (idx < GC * RPc && gc < cols && i < rc) ? H[int64_t(gc) * ldh + r0 + i] : 0.0;
     }
-    if (state[0]) return;
+    if (c == 0 && state[0]) return false;
 #pragma unroll
     for (int k = 0; k < KL; k++) {
-      const int idx = threadIdx.x + k * NT, lc = idx / RP, i = idx - lc * RP;
-      if (idx < GC * RP) Hs[lc * ld + i] = hv[k];
+      const int idx = threadIdx.x + k * NT, lc = idx / RPc, i = idx - lc * RPc;
+      if (idx < GC * RPc) Hs[lc * ldc + i] = hv[k];
     }
-  }
+    return true;
+  };
+  if (!stage(0)) return;
   const double negl = dstate[0];
   if (threadIdx.x == 0) need = 0;
   __syncthreads();
   // G = Hs^T Hs, symmetric: waves w < 12 form the upper 16 x 16 tiles
-  // t = w >> 2 ((0,0), (0,1), (1,1)) over row quarter q = w & 3, and tile
-  // (0,1) is mirrored into (1,0)
+  // t = w >> 2 ((0,0), (0,1), (1,1)) over row quarter q = w & 3 of each
+  // chunk, and tile (0,1) is mirrored into (1,0)
   {
     const int t = w >> 2, q = w & 3, bI = t == 2 ? 1 : 0, bJ = t == 0 ? 0 : 1;
-    const int kq = RP / 4;
-    const double* ca = Hs + (16 * bI + lr) * ld;
-    const double* cb = Hs + (16 * bJ + lr) * ld;
     d4s acc = {0.0, 0.0, 0.0, 0.0};
-    if (t < 3) {
-      // the quarter's operands (kq / 4 <= GROWS / 16 k-steps) are read from
-      // LDS before the products, so the products do not wait on each read
-      constexpr int KS = GROWS / 16;
-      double xa[KS], xb[KS];
-#pragma unroll
-      for (int k = 0; k < KS; k++) {
-        const int k0 = q * kq + 4 * k;
-        xa[k] = 4 * k < kq ? ca[k0 + lk] : 0.0;
-        xb[k] = 4 * k < kq ? cb[k0 + lk] : 0.0;
+    for (int c = 0; c < nch; c++) {
+      if (c > 0) {
+        __syncthreads();  // the previous chunk is read
+        stage(c);
+        __syncthreads();
       }
+      const int RPc = (chunk_rows(c) + 15) & ~15, ldc = RPc + 1;
+      const int kq = RPc / 4;
+      const double* ca = Hs + (16 * bI + lr) * ldc;
+      const double* cb = Hs + (16 * bJ + lr) * ldc;
+      if (t < 3) {
+        // the quarter's operands (kq / 4 <= GROWS / 16 k-steps) are read from
+        // LDS before the products, so the products do not wait on each read
+        constexpr int KS = GROWS / 16;
+        double xa[KS], xb[KS];
 #pragma unroll
-      for (int k = 0; k < KS; k++)
-        if (4 * k < kq) acc = mfma64s(xa[k], xb[k], acc);
-      if (q > 0)
-        for (int r = 0; r < 4; r++) Pp[((q - 1) * 3 + t) * 256 + r * 64 + l] = acc[r];
+        for (int k = 0; k < KS; k++) {
+          const int k0 = q * kq + 4 * k;
+          xa[k] = 4 * k < kq ? ca[k0 + lk] : 0.0;
+          xb[k] = 4 * k < kq ? cb[k0 + lk] : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < KS; k++)
+          if (4 * k < kq) acc = mfma64s(xa[k], xb[k], acc);
+      }
     }
+    if (t < 3 && q > 0)
+      for (int r = 0; r < 4; r++) Pp[((q - 1) * 3 + t) * 256 + r * 64 + l] = acc[r];
     __syncthreads();
     if (t < 3 && q == 0) {
       for (int q2 = 0; q2 < 3; q2++)
@@ -817,13 +842,15 @@ __global__ __launch_bounds__(NT) void k_bjac_gstep(double* H, int ldh, int rows,
 // workgroup per (pair, 32-row slice), so the FP64 MFMA work of the step is
 // spread over 8 (nb2 / 2) workgroups per pair instead of one; every slice
 // reads and writes only its own rows, and the products are the ones the
-// single-workgroup version formed (same k order).
+// single-workgroup version formed (same k order).  nsl slices per pair
+// cover max(rows, cols) (kBjacSlices up to GROWS).
 // ---------------------------------------------------------------------------
 constexpr int kBjacSlices = GROWS / 32;
 __global__ __launch_bounds__(256) void k_bjac_apply(double* H, int ldh, int rows, int cols, double* J, int ldj,
-                                                    int nb2, int step, int sweep, int* state, const double* dstate) {
+                                                    int nb2, int step, int sweep, int* state, const double* dstate,
+                                                    int nsl) {
   __shared__ double Qs[GC * GLD];
-  const int p = blockIdx.x / kBjacSlices, sl = blockIdx.x % kBjacSlices;
+  const int p = blockIdx.x / nsl, sl = blockIdx.x % nsl;
   const int bi = rr_player(p, step, nb2), bj = rr_player(nb2 - 1 - p, step, nb2);
   auto gcol = [&](int lc) { return lc < GB ? bi * GB + lc : bj * GB + (lc - GB); };
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63, lr = l & 15, lk = l >> 4;

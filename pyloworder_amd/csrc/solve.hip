@@ -57,9 +57,11 @@ constexpr double kJacTolMx = 1e-8;  // Jacobi orthogonality tolerance, mixed pat
 // with rows <= 256 takes Gram-driven MFMA steps on 16-column blocks, other
 // problems column-pair rotations staged in LDS.  b = block width, nb2 = even
 // number of blocks (>= 2); nb2 == 2 means one block pair holds every column.
+// Gram-driven steps for real problems (the fused one-launch form, which
+// keeps the pair's rows in LDS, up to GROWS rows)
 template <class T>
-void jacobi_blocks(int rows, int cols, bool& gram_steps, int& b, int& nb2) {
-  gram_steps = std::is_same<T, double>::value && rows <= GROWS;
+void jacobi_blocks(int rows, int cols, bool& gram_steps, int& b, int& nb2, bool fused = false) {
+  gram_steps = std::is_same<T, double>::value && (rows <= GROWS || !fused);
   b = gram_steps ? GB : bjac_width(std::max(rows, cols), int(sizeof(T)));
   nb2 = int(ceil_div(cols, b));
   nb2 += nb2 & 1;
@@ -72,11 +74,13 @@ void launch_block_jacobi(pc_ctx* ctx, T* H, int ldh, int rows, int cols, T* J, i
   hipStream_t st = ctx->stream;
   bool gram_steps;
   int b, nb2;
-  jacobi_blocks<T>(rows, cols, gram_steps, b, nb2);
   // a context that shares the GPU with other streams (grouped solve) takes
   // the one-launch steps: there the launch count costs more than the one-CU
   // step time the split steps save
   const bool fused = ctx->concurrent;
+  jacobi_blocks<T>(rows, cols, gram_steps, b, nb2, fused);
+  // row slices of the apply launch
+  const int nsl = std::max(kBjacSlices, int(ceil_div(std::max(rows, cols), 32)));
   hipLaunchKernelGGL((k_bjac_init<T>), dim3(1, 1, nz), dim3(1024), 0, st, H, ldh, rows, cols, J, ldj, state, dstate,
                      kBjacMaxSweeps + 1, zb);
   // one block pair (all columns in one workgroup): each launch iterates to
@@ -114,8 +118,8 @@ void launch_block_jacobi(pc_ctx* ctx, T* H, int ldh, int rows, int cols, T* J, i
           }
           hipLaunchKernelGGL(k_bjac_gstep<false>, dim3(nb2 / 2 * 8), dim3(1024), 0, st, H, ldh, rows, cols, J, ldj,
                              nb2, step, s, state, dstate, tol, max_inner);
-          hipLaunchKernelGGL(k_bjac_apply, dim3(nb2 / 2 * kBjacSlices), dim3(256), 0, st, H, ldh, rows, cols, J, ldj,
-                             nb2, step, s, state, (const double*)dstate);
+          hipLaunchKernelGGL(k_bjac_apply, dim3(nb2 / 2 * nsl), dim3(256), 0, st, H, ldh, rows, cols, J, ldj, nb2,
+                             step, s, state, (const double*)dstate, nsl);
           continue;
         }
       }
@@ -492,7 +496,7 @@ int64_t carve(WsCursor& c, const Plan& p, SmallBufs<T>& b, double** mean, double
   b.status = c.take<int>(4);
   b.perm3 = c.take<int>(n); b.rank3 = c.take<int>(4);
   b.jstate = c.take<int>(kBjacMaxSweeps + 8);
-  b.jdstate = c.take<double>(kBjacDstate);  // [0] negligible norm, then the Gram steps' pair rotations
+  b.jdstate = c.take<double>(bjac_dstate_len(n));  // [0] negligible norm, then the Gram steps' pair rotations
   b.dsc = c.take<double>(4);
   b.Wpad = c.take<double>(int64_t(p.NB) * p.NB);
   int ncand = int(std::min<int64_t>(p.m, 4 * int64_t(n) + 16));
