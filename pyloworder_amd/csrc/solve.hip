@@ -57,11 +57,10 @@ constexpr double kJacTolMx = 1e-8;  // Jacobi orthogonality tolerance, mixed pat
 // with rows <= 256 takes Gram-driven MFMA steps on 16-column blocks, other
 // problems column-pair rotations staged in LDS.  b = block width, nb2 = even
 // number of blocks (>= 2); nb2 == 2 means one block pair holds every column.
-// Gram-driven steps for real problems (the fused one-launch form, which
-// keeps the pair's rows in LDS, up to GROWS rows)
+// Gram-driven steps for real problems
 template <class T>
-void jacobi_blocks(int rows, int cols, bool& gram_steps, int& b, int& nb2, bool fused = false) {
-  gram_steps = std::is_same<T, double>::value && (rows <= GROWS || !fused);
+void jacobi_blocks(int rows, int cols, bool& gram_steps, int& b, int& nb2) {
+  gram_steps = std::is_same<T, double>::value;
   b = gram_steps ? GB : bjac_width(std::max(rows, cols), int(sizeof(T)));
   nb2 = int(ceil_div(cols, b));
   nb2 += nb2 & 1;
@@ -75,10 +74,11 @@ void launch_block_jacobi(pc_ctx* ctx, T* H, int ldh, int rows, int cols, T* J, i
   bool gram_steps;
   int b, nb2;
   // a context that shares the GPU with other streams (grouped solve) takes
-  // the one-launch steps: there the launch count costs more than the one-CU
-  // step time the split steps save
-  const bool fused = ctx->concurrent;
-  jacobi_blocks<T>(rows, cols, gram_steps, b, nb2, fused);
+  // the one-launch steps (they hold the pair's rows in LDS: up to GROWS):
+  // there the launch count costs more than the one-CU step time the split
+  // steps save.  Both forms give the same H and J.
+  const bool fused = ctx->concurrent && rows <= GROWS;
+  jacobi_blocks<T>(rows, cols, gram_steps, b, nb2);
   // row slices of the apply launch
   const int nsl = std::max(kBjacSlices, int(ceil_div(std::max(rows, cols), 32)));
   hipLaunchKernelGGL((k_bjac_init<T>), dim3(1, 1, nz), dim3(1024), 0, st, H, ldh, rows, cols, J, ldj, state, dstate,

@@ -41,7 +41,7 @@ def _flow(M, N, seed, dtype=np.float64):
 
 
 def test_grouped_bitwise_equals_single(rng):
-    shapes = [(3000, 7), (12000, 40), (800, 1), (5000, 256), (20000, 17)]
+    shapes = [(3000, 7), (12000, 40), (800, 1), (5000, 256), (20000, 17), (4000, 300)]
     As = [rng.standard_normal(s) @ np.diag(np.logspace(0, -4, s[1])) for s in shapes]
     Ug, Sg, Vg = pl.math.tsqr_svd_grouped(As)
     for A, U, S, V in zip(As, Ug, Sg, Vg):
