@@ -473,6 +473,13 @@ int make_plan(pc_ctx* ctx, pc_dtype dt, int64_t m, int64_t n, int precision, Pla
     return PC_OK;
   }
   p.chunks = stream_chunks(m, p.tr);
+  // mixed P1: at most 512 chunks (two workgroups per CU in turn for long
+  // inputs), half the partial slabs to write and sum
+  if (p.mixed) {
+    const int64_t tiles = ceil_div(m, p.tr);
+    const int c512 = int(ceil_div(tiles, ceil_div(tiles, 512)));
+    if (c512 < p.chunks) p.chunks = c512;
+  }
   p.tpc = ceil_div(ceil_div(m, p.tr), p.chunks);
   p.chunks3 = stream_chunks(m, p.tr3);
   p.tpc3 = ceil_div(ceil_div(m, p.tr3), p.chunks3);
