@@ -424,6 +424,8 @@ struct Plan {
   int64_t tpc;
   int tr3, chunks3;  // P2/P3 kernels (mixed uses other tiles)
   int64_t tpc3;
+  int chunks2;       // mixed P2 (row tiles of tr3, at most 512 chunks)
+  int64_t tpc2;
   int nparts;  // partial slabs per P1 launch
   bool wide;   // 256 < real columns <= 2048: stripe-pair FP64 kernels, Z materialised
 };
@@ -469,20 +471,24 @@ int make_plan(pc_ctx* ctx, pc_dtype dt, int64_t m, int64_t n, int precision, Pla
     p.tpc = ceil_div(ceil_div(m, p.tr), p.chunks);
     p.chunks3 = stream_chunks(m, p.tr3);
     p.tpc3 = ceil_div(ceil_div(m, p.tr3), p.chunks3);
+    p.chunks2 = p.chunks3;
+    p.tpc2 = p.tpc3;
     p.nparts = p.chunks;
     return PC_OK;
   }
   p.chunks = stream_chunks(m, p.tr);
-  // mixed P1: at most 512 chunks (two workgroups per CU in turn for long
-  // inputs), half the partial slabs to write and sum
-  if (p.mixed) {
-    const int64_t tiles = ceil_div(m, p.tr);
+  // mixed P1 and P2: at most 512 chunks (two workgroups per CU in turn for
+  // long inputs), half the partial slabs to write and sum
+  auto cap512 = [](int64_t tiles, int c) {
     const int c512 = int(ceil_div(tiles, ceil_div(tiles, 512)));
-    if (c512 < p.chunks) p.chunks = c512;
-  }
+    return c512 < c ? c512 : c;
+  };
+  if (p.mixed) p.chunks = cap512(ceil_div(m, p.tr), p.chunks);
   p.tpc = ceil_div(ceil_div(m, p.tr), p.chunks);
   p.chunks3 = stream_chunks(m, p.tr3);
   p.tpc3 = ceil_div(ceil_div(m, p.tr3), p.chunks3);
+  p.chunks2 = p.mixed ? cap512(ceil_div(m, p.tr3), p.chunks3) : p.chunks3;
+  p.tpc2 = ceil_div(ceil_div(m, p.tr3), p.chunks2);
   p.nparts = p.chunks;
   return PC_OK;
 }
@@ -914,8 +920,8 @@ static int launch_streams(pc_ctx* ctx, const Plan& p, const void* A, int64_t lda
     if (pass == 1)
       rc = launch_gram_mx2(st, (const float*)A, ldr, p.m, p.nr, center, mean, var, p.chunks, p.tpc, part, bz);
     else
-      rc = launch_zgram_mx2(st, (const float*)A, ldr, p.m, p.nr, center, mean, var, b.Wh, b.wscale, p.chunks3,
-                            p.tpc3, part, bz, /*upper: W1 = R1^-1, zero-padded*/ true);
+      rc = launch_zgram_mx2(st, (const float*)A, ldr, p.m, p.nr, center, mean, var, b.Wh, b.wscale, p.chunks2,
+                            p.tpc2, part, bz, /*upper: W1 = R1^-1, zero-padded*/ true);
   } else {
     if (pass == 1)
       rc = launch_gram_f64(st, sdt, A, ldr, p.m, p.nr, center, 0, nullptr, nullptr, center ? mean : nullptr,
@@ -1137,7 +1143,7 @@ static int core_t(pc_ctx* ctx, const Plan& p, const void* A, int64_t lda, int ce
     b.ldz = p.cplx ? 2 * out.ldu : out.ldu;
   }
   PC_CHECK(ctx, launch_streams<T>(ctx, p, A, lda, center, mean, var, part, b, 2));
-  reduce_gram<T>(st, p, part, p.mixed ? p.chunks3 : p.chunks, b.G2);
+  reduce_gram<T>(st, p, part, p.mixed ? p.chunks2 : p.chunks, b.G2);
   run_pchol<T>(st, b.G2, n, p.mixed ? kTolMx : kTolF64, b.Awk, b.R2, b.perm, b.rank, b.status, 1, 0, b.rank + 1);
   // M = R2 (rows < rank) * R1h[perm, :], stored column-major into Hc
   dim3 gg(unsigned(ceil_div(n, 32)), unsigned(ceil_div(n, 32)));
@@ -1315,7 +1321,7 @@ static int core_batch_t(pc_ctx* ctx, const Plan& p, const pc_problem* pr, int64_
   run_trinv<T>(st, b.W1, n, b.R1h, nz, zb);
   // ---- P2
   PC_CHECK(ctx, launch_streams<T>(ctx, p, pr[0].A, lda, center, mean, var, part, b, 2, pb));
-  reduce_gram<T>(st, p, part, p.mixed ? p.chunks3 : p.chunks, b.G2, nz, zb);
+  reduce_gram<T>(st, p, part, p.mixed ? p.chunks2 : p.chunks, b.G2, nz, zb);
   run_pchol<T>(st, b.G2, n, p.mixed ? kTolMx : kTolF64, b.Awk, b.R2, b.perm, b.rank, b.status, nz, zb);
   dim3 gg(unsigned(ceil_div(n, 32)), unsigned(ceil_div(n, 32)), z);
   hipLaunchKernelGGL((k_gemm_small<T>), gg, dim3(256), 0, st, 0, 0, n, n, n, T(1.0), b.R2, n, (const int*)nullptr,
