@@ -906,17 +906,25 @@ __global__ __launch_bounds__(64) void k_trinv_cols(const double* __restrict__ R,
       for (int k = 0; k < rows; k++) Rs[k * 65 + l] = R[int64_t(k) * ldr + c0 + l];
     }
     wave_sync();
+    // The stretch's diagonal and its reciprocals, one division per lane in
+    // parallel.  A step's quotient is then q0 = r y refined by one residual
+    // step, q0 + (r - d q0) y (the last step of Newton-Markstein division),
+    // so the division leaves the step's dependency chain.
+    const double dgl = l <= cl ? Rs[(c0 + l) * 65 + l] : 1.0;
+    const double dl = 1.0 / dgl;
+    auto lane_read = [](double v, int t) {  // t is wave-uniform: a lane read, not a shuffle
+      const long long b = __builtin_bit_cast(long long, v);
+      return __builtin_bit_cast(double, (long long)(unsigned)__builtin_amdgcn_readlane(int(b), t) |
+                                            ((long long)__builtin_amdgcn_readlane(int(b >> 32), t) << 32));
+    };
     for (int t = cl; t >= 0; --t) {
       const int i = c0 + t;
       double rv = r[0];
 #pragma unroll
       for (int q = 1; q < 4; q++) rv = g == q ? r[q] : rv;
-      // lane t's right-hand side (t is wave-uniform: a lane read, not a shuffle)
-      const long long rb = __builtin_bit_cast(long long, rv);
-      const double rt = __builtin_bit_cast(
-          double, (long long)(unsigned)__builtin_amdgcn_readlane(int(rb), t) |
-                      ((long long)__builtin_amdgcn_readlane(int(rb >> 32), t) << 32));
-      const double wi = rt / Rs[i * 65 + t];
+      const double rt = lane_read(rv, t), d = lane_read(dgl, t), y = lane_read(dl, t);
+      const double q0 = rt * y;
+      const double wi = fma(fma(-d, q0, rt), y, q0);
 #pragma unroll
       for (int q = 0; q < 4; q++) {
         if (g == q && l == t) w[q] = wi;
