@@ -102,6 +102,10 @@ struct pc_ctx {
   // (same results), since launches, not one CU's step time, bound it when
   // many solves share the GPU
   bool concurrent = false;
+  // side stream (default priority) and fork / join events for the small
+  // stage's off-path triangular inverses of a lone solve; created on first use
+  hipStream_t side = nullptr;
+  hipEvent_t sev[2] = {};
   std::string err;
   // launch-geometry tuning of the product pass (pc_autotune)
   int autotune = 0;

@@ -61,6 +61,12 @@ extern "C" int pc_ctx_destroy(pc_ctx* ctx) {
   }
   for (hipEvent_t e : ctx->gev)
     if (e) (void)hipEventDestroy(e);
+  if (ctx->side) {
+    (void)hipStreamSynchronize(ctx->side);
+    (void)hipStreamDestroy(ctx->side);
+  }
+  for (hipEvent_t e : ctx->sev)
+    if (e) (void)hipEventDestroy(e);
   if (ctx->ws) (void)hipFree(ctx->ws);
   if (ctx->dflag) (void)hipFree(ctx->dflag);
   if (ctx->hflag) (void)hipHostFree(ctx->hflag);

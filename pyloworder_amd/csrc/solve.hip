@@ -1099,6 +1099,62 @@ template <class T, class ST>
 static int core_rand_tail(pc_ctx* ctx, const Plan& p, const void* A, int64_t lda, double* mean, double* var,
                           SmallBufs<double>& b, WsCursor& cur, const RandArgs& ra);
 
+// Work of a lone solve that nothing on the main stream reads until a later
+// point runs on the context's side stream: fork() orders the side stream
+// after the main stream's work so far, join() orders the main stream after
+// the side work.  A fork without its join is joined when the guard goes out
+// of scope, on error returns too.  Grouped members (already concurrent), the
+// allocation-free entry point (borrowed workspace: it makes no stream
+// either) and contexts whose side stream cannot be made run everything on
+// the main stream.  Fork and join are capture-compatible.
+struct SideWork {
+  pc_ctx* ctx;
+  hipStream_t st;
+  bool pending = false;
+  bool use = false;
+  SideWork(pc_ctx* c, hipStream_t s) : ctx(c), st(s) {
+    if (c->concurrent || c->ws_fixed) return;
+    if (!c->side) {
+      // the stream and events are made outside any capture of st
+      hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+      if (hipStreamIsCapturing(st, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone) {
+        (void)hipGetLastError();
+        return;
+      }
+    }
+    if (!c->side && hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess) {
+      (void)hipGetLastError();
+      c->side = nullptr;
+      return;
+    }
+    for (hipEvent_t& e : c->sev)
+      if (!e && hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {  // made with the stream
+        (void)hipGetLastError();
+        e = nullptr;
+        return;
+      }
+    use = true;
+  }
+  // the stream the off-path work goes to
+  hipStream_t fork() {
+    if (!use) return st;
+    if (hipEventRecord(ctx->sev[0], st) != hipSuccess || hipStreamWaitEvent(ctx->side, ctx->sev[0], 0) != hipSuccess) {
+      (void)hipGetLastError();
+      use = false;
+      return st;
+    }
+    pending = true;
+    return ctx->side;
+  }
+  void join() {
+    if (!pending) return;
+    pending = false;
+    (void)hipEventRecord(ctx->sev[1], ctx->side);
+    (void)hipStreamWaitEvent(st, ctx->sev[1], 0);
+  }
+  ~SideWork() { join(); }
+};
+
 template <class T, class ST>
 static int core_t(pc_ctx* ctx, const Plan& p, const void* A, int64_t lda, int center, int mode, CoreOut out,
                   const RandArgs* ra = nullptr) {
@@ -1136,7 +1192,10 @@ static int core_t(pc_ctx* ctx, const Plan& p, const void* A, int64_t lda, int ce
   run_trinv<T>(st, b.R1, n, b.W1);
   to_operand<T>(st, p, b.W1, n, b, false);
   if (p.mixed) rounded_back<T>(st, p, b, b.W1);  // W1 := the operand actually applied
-  run_trinv<T>(st, b.W1, n, b.R1h);
+  // R1h = W1^-1 is read first after P2: it runs beside P2 (P2 reads only the
+  // operand made from W1, and no kernel writes W1 again)
+  SideWork side(ctx, st);
+  run_trinv<T>(side.fork(), b.W1, n, b.R1h);
   // ---- P2
   if (p.wide && !b.Zw) {  // FP64 storage: Z lives in U until P3 overwrites it
     b.Zw = static_cast<double*>(out.U);
@@ -1146,6 +1205,7 @@ static int core_t(pc_ctx* ctx, const Plan& p, const void* A, int64_t lda, int ce
   reduce_gram<T>(st, p, part, p.mixed ? p.chunks2 : p.chunks, b.G2);
   run_pchol<T>(st, b.G2, n, p.mixed ? kTolMx : kTolF64, b.Awk, b.R2, b.perm, b.rank, b.status, 1, 0, b.rank + 1);
   // M = R2 (rows < rank) * R1h[perm, :], stored column-major into Hc
+  side.join();
   dim3 gg(unsigned(ceil_div(n, 32)), unsigned(ceil_div(n, 32)));
   hipLaunchKernelGGL((k_gemm_small<T>), gg, dim3(256), 0, st, 0, 0, n, n, n, T(1.0), b.R2, n, (const int*)nullptr,
                      b.R1h, n, (const int*)b.perm, T(0.0), b.Hc, n, 1);
@@ -1154,8 +1214,11 @@ static int core_t(pc_ctx* ctx, const Plan& p, const void* A, int64_t lda, int ce
   if (p.mixed && center)
     hipLaunchKernelGGL((k_center_right<T>), dim3(nblk(n)), dim3(256), 0, st, b.Hc, n, n);
   hipLaunchKernelGGL((k_pad_identity<T>), dim3(1), dim3(256), 0, st, b.R2, n, n, b.rank);
-  run_trinv<T>(st, b.R2, n, b.R2i);
+  // R2i = R2^-1 is read first in W, after the Jacobi: it runs beside the
+  // preconditioner and the Jacobi (no kernel between writes R2 or R2i)
+  run_trinv<T>(side.fork(), b.R2, n, b.R2i);
   if (mode == CORE_RAND) {
+    side.join();
     if constexpr (std::is_same<T, double>::value && std::is_same<ST, float>::value)
       return core_rand_tail(ctx, p, A, lda, mean, var, b, cur, *ra);
     else
@@ -1216,6 +1279,7 @@ static int core_t(pc_ctx* ctx, const Plan& p, const void* A, int64_t lda, int ce
     hipLaunchKernelGGL(k_null_count, dim3(1), dim3(64), 0, st, (const int*)b.rank, n, b.nnull);
   }
   // W = W1[:, perm] * (R2^-1 * UM)
+  side.join();
   hipLaunchKernelGGL((k_gemm_small<T>), gg, dim3(256), 0, st, 0, 0, n, n, n, T(1.0), b.R2i, n, (const int*)nullptr,
                      b.UM, n, (const int*)nullptr, T(0.0), b.tmp, n, 0);
   hipLaunchKernelGGL((k_gemm_small<T>), gg, dim3(256), 0, st, 0, 0, n, n, n, T(1.0), b.W1, n, (const int*)b.perm,
