@@ -1192,10 +1192,10 @@ static int core_t(pc_ctx* ctx, const Plan& p, const void* A, int64_t lda, int ce
   run_trinv<T>(st, b.R1, n, b.W1);
   to_operand<T>(st, p, b.W1, n, b, false);
   if (p.mixed) rounded_back<T>(st, p, b, b.W1);  // W1 := the operand actually applied
-  // R1h = W1^-1 is read first after P2: it runs beside P2 (P2 reads only the
-  // operand made from W1, and no kernel writes W1 again)
+  // (W1^-1 stays on the main stream: beside P2 its workgroups, with
+  // 133 KB of LDS each, would take whole CUs from P2)
+  run_trinv<T>(st, b.W1, n, b.R1h);
   SideWork side(ctx, st);
-  run_trinv<T>(side.fork(), b.W1, n, b.R1h);
   // ---- P2
   if (p.wide && !b.Zw) {  // FP64 storage: Z lives in U until P3 overwrites it
     b.Zw = static_cast<double*>(out.U);
@@ -1205,7 +1205,6 @@ static int core_t(pc_ctx* ctx, const Plan& p, const void* A, int64_t lda, int ce
   reduce_gram<T>(st, p, part, p.mixed ? p.chunks2 : p.chunks, b.G2);
   run_pchol<T>(st, b.G2, n, p.mixed ? kTolMx : kTolF64, b.Awk, b.R2, b.perm, b.rank, b.status, 1, 0, b.rank + 1);
   // M = R2 (rows < rank) * R1h[perm, :], stored column-major into Hc
-  side.join();
   dim3 gg(unsigned(ceil_div(n, 32)), unsigned(ceil_div(n, 32)));
   hipLaunchKernelGGL((k_gemm_small<T>), gg, dim3(256), 0, st, 0, 0, n, n, n, T(1.0), b.R2, n, (const int*)nullptr,
                      b.R1h, n, (const int*)b.perm, T(0.0), b.Hc, n, 1);
